@@ -1,0 +1,155 @@
+// Device-side building blocks shared by the scan kernels (gfx950, wave64).
+//
+// Results are ordered everywhere by the 64-bit key  ord(distance) << 32 | slot,
+// where ord() is a monotone map of IEEE binary32 onto uint32. Sorting keys
+// therefore sorts by (distance, slot) with ties going to the lower slot, the
+// rule of the reference CPU path (crates/engine/src/metrics.rs:125-129) and of
+// its flat scan kernel's strict-< insertion (kernels/src/sparse_hybrid.cu:60-73).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/flashvector/types.h"
+
+namespace fv {
+
+constexpr int kWave = 64;
+constexpr uint64_t kEmptyKey = ~0ull;
+constexpr float kAdmit = 1e30f;  // only distances below the reference's sentinel are kept
+
+__device__ __forceinline__ uint32_t ord_f32(float f) {
+    f = f + 0.0f;  // -0 -> +0 so that both zeros share one key
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ float unord_f32(uint32_t o) {
+    uint32_t u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+    return __uint_as_float(u);
+}
+
+__device__ __forceinline__ uint64_t make_key(float dist, uint32_t slot) {
+    return ((uint64_t)ord_f32(dist) << 32) | slot;
+}
+
+__device__ __forceinline__ float key_dist(uint64_t k) { return unord_f32((uint32_t)(k >> 32)); }
+__device__ __forceinline__ uint32_t key_slot(uint64_t k) { return (uint32_t)k; }
+
+// kernels/src/distance_metrics.cuh:17-20: LSB-first u32 words, NULL allows all.
+__device__ __forceinline__ bool mask_allows(const uint32_t* __restrict__ mask, uint32_t idx) {
+    return mask == nullptr || ((__ldg(mask + (idx >> 5)) >> (idx & 31u)) & 1u) != 0u;
+}
+
+__host__ __device__ __forceinline__ uint32_t next_pow2(uint32_t v) {
+    if (v <= 1u) return 1u;
+    --v;
+    v |= v >> 1; v |= v >> 2; v |= v >> 4; v |= v >> 8; v |= v >> 16;
+    return v + 1u;
+}
+
+// Orders LDS operations of one wavefront: lanes of a wave run in lockstep and
+// the LDS unit serves one wave's requests in order, so a compiler fence plus a
+// wave barrier is enough between dependent bitonic stages.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Ascending bitonic sort of keys[0..n) (n a power of two) by one wavefront.
+__device__ __forceinline__ void wave_bitonic_sort(uint64_t* keys, uint32_t n, int lane) {
+    for (uint32_t k = 2; k <= n; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t p = (uint32_t)lane; p < (n >> 1); p += kWave) {
+                uint32_t i = ((p / j) * (2u * j)) + (p % j);
+                uint32_t m = i + j;
+                uint64_t a = keys[i], b = keys[m];
+                bool up = (i & k) == 0;
+                if ((a > b) == up) {
+                    keys[i] = b;
+                    keys[m] = a;
+                }
+            }
+            wave_lds_sync();
+        }
+    }
+}
+
+// Same network executed by a whole workgroup (blockDim.x threads).
+__device__ __forceinline__ void block_bitonic_sort(uint64_t* keys, uint32_t n) {
+    for (uint32_t k = 2; k <= n; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t p = threadIdx.x; p < (n >> 1); p += blockDim.x) {
+                uint32_t i = ((p / j) * (2u * j)) + (p % j);
+                uint32_t m = i + j;
+                uint64_t a = keys[i], b = keys[m];
+                bool up = (i & k) == 0;
+                if ((a > b) == up) {
+                    keys[i] = b;
+                    keys[m] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Prune of one query's candidate buffer by one wavefront: sorts the `count`
+// buffered keys through LDS scratch (capacity >= next_pow2(count)), writes the
+// `keep` smallest back in ascending order and returns how many remain.
+// `scratch` is wave-private LDS.
+__device__ __forceinline__ uint32_t wave_prune(uint64_t* __restrict__ buf, uint32_t count,
+                                               uint32_t keep, uint64_t* scratch, int lane) {
+    uint32_t n = next_pow2(count < 2u ? 2u : count);
+    for (uint32_t i = (uint32_t)lane; i < n; i += kWave) scratch[i] = i < count ? buf[i] : kEmptyKey;
+    wave_lds_sync();
+    wave_bitonic_sort(scratch, n, lane);
+    uint32_t kept = count < keep ? count : keep;
+    for (uint32_t i = (uint32_t)lane; i < kept; i += kWave) buf[i] = scratch[i];
+    wave_lds_sync();
+    return kept;
+}
+
+// Exact FP32 distances in the reference CPU path's order: one sequential,
+// non-fused sum over d (the library is built with -ffp-contract=off).
+template <int METRIC>
+__device__ __forceinline__ float exact_distance(const float* __restrict__ q,
+                                                const float* __restrict__ x, uint32_t dim) {
+    if (METRIC == METRIC_INNER_PRODUCT) {
+        float dot = 0.0f;
+        for (uint32_t d = 0; d < dim; ++d) {
+            float p = q[d] * x[d];
+            dot = dot + p;
+        }
+        return -dot;
+    } else if (METRIC == METRIC_COSINE) {
+        float dot = 0.0f, nq = 0.0f, nx = 0.0f;
+        for (uint32_t d = 0; d < dim; ++d) {
+            float a = q[d], b = x[d];
+            float p = a * b, pa = a * a, pb = b * b;
+            dot = dot + p;
+            nq = nq + pa;
+            nx = nx + pb;
+        }
+        float denom = sqrtf(nq) * sqrtf(nx) + 1e-8f;
+        return 1.0f - dot / denom;
+    } else {
+        float acc = 0.0f;
+        for (uint32_t d = 0; d < dim; ++d) {
+            float diff = q[d] - x[d];
+            float sq = diff * diff;
+            acc = acc + sq;
+        }
+        return acc;
+    }
+}
+
+__device__ __forceinline__ float exact_distance_rt(const float* q, const float* x, uint32_t dim,
+                                                   int metric) {
+    if (metric == METRIC_INNER_PRODUCT) return exact_distance<METRIC_INNER_PRODUCT>(q, x, dim);
+    if (metric == METRIC_COSINE) return exact_distance<METRIC_COSINE>(q, x, dim);
+    return exact_distance<METRIC_L2>(q, x, dim);
+}
+
+}  // namespace fv

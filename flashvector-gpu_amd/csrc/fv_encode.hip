@@ -1,0 +1,69 @@
+// Device encoders for the flat tables (the step before the scan).
+//
+// Reference: crates/engine/src/quant.rs:306-327 (encode_fp16_padded: rows
+// padded to a multiple of 16 halves, |x|^2 of the FP32 originals). Here the
+// encoder runs on the GPU, rounds to nearest even (the reference rounds half
+// up on magnitude; the two differ only on exact binary16 ties) and also emits
+// BF16 codes and FP64 norms.
+#include "fv_flat_exact.hpp"
+
+namespace fv {
+
+__global__ void encode_16bit_kernel(const float* __restrict__ x, uint32_t n, uint32_t dim,
+                                    uint32_t pdim, uint32_t dtype, uint16_t* __restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * pdim) return;
+    uint32_t r = (uint32_t)(i / pdim), d = (uint32_t)(i % pdim);
+    float v = d < dim ? x[(size_t)r * dim + d] : 0.0f;
+    uint16_t h;
+    if (dtype == FV_DTYPE_F16) {
+        _Float16 f = (_Float16)v;
+        h = __builtin_bit_cast(uint16_t, f);
+    } else {
+        uint32_t u = __float_as_uint(v);
+        u += 0x7fffu + ((u >> 16) & 1u);
+        h = (uint16_t)(u >> 16);
+    }
+    out[i] = h;
+}
+
+__global__ void norms_kernel(const float* __restrict__ x, uint32_t n, uint32_t dim,
+                             float* __restrict__ out32, double* __restrict__ out64) {
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const float* row = x + (size_t)r * dim;
+    float a = 0.0f;
+    double b = 0.0;
+    for (uint32_t d = 0; d < dim; ++d) {
+        float v = row[d];
+        float sq = v * v;
+        a = a + sq;
+        b = __builtin_fma((double)v, (double)v, b);
+    }
+    if (out32) out32[r] = a;
+    if (out64) out64[r] = b;
+}
+
+}  // namespace fv
+
+extern "C" int fv_encode_flat(const float* d_vectors, uint32_t num_vectors, uint32_t dim,
+                              uint32_t padded_dim, uint32_t dtype, void* d_codes_out,
+                              float* d_norms_out, double* d_norms_f64_out,
+                              float* d_row_scales_out, void* stream) {
+    if (!d_vectors) return -1;
+    if (padded_dim < dim || dim == 0) return (int)hipErrorInvalidValue;
+    if (num_vectors == 0) return 0;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (d_codes_out) {
+        if (dtype != FV_DTYPE_F16 && dtype != FV_DTYPE_BF16) return (int)hipErrorInvalidValue;
+        size_t total = (size_t)num_vectors * padded_dim;
+        fv::encode_16bit_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(
+            d_vectors, num_vectors, dim, padded_dim, dtype, static_cast<uint16_t*>(d_codes_out));
+    }
+    if (d_norms_out || d_norms_f64_out)
+        fv::norms_kernel<<<(num_vectors + 255) / 256, 256, 0, s>>>(d_vectors, num_vectors, dim,
+                                                                  d_norms_out, d_norms_f64_out);
+    (void)d_row_scales_out;
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : (int)e;
+}

@@ -1,0 +1,332 @@
+// Exact FP32 flat scan fused with top-k (the cuda_flat_scan_batch path).
+//
+// Reference: kernels/src/sparse_hybrid.cu:13-107 (one block per query, one
+// vector per warp, thread-0 serial insertion). Here the scan is a tiled
+// queries x rows sweep: a workgroup owns 64 queries and one chunk of rows,
+// every thread holds a 4x4 register tile of exact distances accumulated in the
+// reference CPU path's order (sequential over d, non-fused), and a candidate
+// that beats its query's running threshold is appended to a per-(chunk, query)
+// buffer. Full buffers are pruned by one wavefront (LDS bitonic sort). A second
+// kernel merges the per-chunk sorted lists of each query.
+#include "fv_flat_exact.hpp"
+
+namespace fv {
+
+constexpr int kTQ = 64;        // queries per workgroup
+constexpr int kTN = 64;        // rows per tile
+constexpr int kDK = 32;        // dims per LDS stage
+constexpr int kLdsStride = 68; // padded LDS row (floats), keeps float4 alignment
+constexpr int kScanThreads = 256;
+
+// Row norms (cosine): sequential non-fused sums, bit-identical to the sums the
+// oracle forms inside its per-pair loop.
+__global__ void row_norms_kernel(const float* __restrict__ x, uint32_t n, uint32_t dim,
+                                 float* __restrict__ out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* r = x + (size_t)i * dim;
+    float acc = 0.0f;
+    for (uint32_t d = 0; d < dim; ++d) {
+        float v = r[d];
+        float sq = v * v;
+        acc = acc + sq;
+    }
+    out[i] = acc;
+}
+
+template <int METRIC>
+__global__ __launch_bounds__(kScanThreads) void flat_exact_scan_kernel(
+    const float* __restrict__ X, uint32_t n, uint32_t dim, const float* __restrict__ Qm,
+    uint32_t nq, const float* __restrict__ xnorm, const float* __restrict__ qnorm,
+    const uint32_t* __restrict__ mask, uint32_t keep, uint32_t cap, uint32_t rows_per_chunk,
+    uint64_t* __restrict__ cand, uint64_t* __restrict__ partial) {
+    __shared__ __attribute__((aligned(16))) float sq[kDK * kLdsStride];
+    __shared__ __attribute__((aligned(16))) float sx[kDK * kLdsStride];
+    __shared__ uint32_t s_cnt[kTQ];
+    __shared__ float s_thr[kTQ];
+    extern __shared__ uint64_t s_scratch[];  // 4 waves x cap keys
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    const uint32_t q0 = blockIdx.x * kTQ;
+    const uint32_t chunk = blockIdx.y;
+    const uint32_t r_begin = chunk * rows_per_chunk;
+    const uint32_t r_end = min(n, r_begin + rows_per_chunk);
+    const int tq = (tid & 15) * 4;
+    const int tr = (tid >> 4) * 4;
+    uint64_t* my_scratch = s_scratch + (size_t)wave * cap;
+
+    if (tid < kTQ) {
+        s_cnt[tid] = 0;
+        s_thr[tid] = kAdmit;
+    }
+    float qn[4];
+    if (METRIC == METRIC_COSINE) {
+        for (int i = 0; i < 4; ++i) qn[i] = (q0 + tq + i < nq) ? qnorm[q0 + tq + i] : 1.0f;
+    }
+
+    for (uint32_t r0 = r_begin; r0 < r_end; r0 += kTN) {
+        float acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+
+        for (uint32_t d0 = 0; d0 < dim; d0 += kDK) {
+            __syncthreads();
+            // Stage 64 queries x 32 dims and 64 rows x 32 dims, transposed.
+            for (int it = 0; it < 2; ++it) {
+                int qq = (tid >> 3) + it * 32;
+                int dd = (tid & 7) * 4;
+                uint32_t qg = q0 + qq;
+                uint32_t rg = r0 + qq;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    uint32_t dg = d0 + dd + c;
+                    sq[(dd + c) * kLdsStride + qq] =
+                        (qg < nq && dg < dim) ? Qm[(size_t)qg * dim + dg] : 0.0f;
+                    sx[(dd + c) * kLdsStride + qq] =
+                        (rg < r_end && dg < dim) ? X[(size_t)rg * dim + dg] : 0.0f;
+                }
+            }
+            __syncthreads();
+            const int dlim = min((int)kDK, (int)(dim - d0));
+            for (int dd = 0; dd < dlim; ++dd) {
+                float4 qv = *reinterpret_cast<const float4*>(&sq[dd * kLdsStride + tq]);
+                float4 xv = *reinterpret_cast<const float4*>(&sx[dd * kLdsStride + tr]);
+                float qa[4] = {qv.x, qv.y, qv.z, qv.w};
+                float xa[4] = {xv.x, xv.y, xv.z, xv.w};
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if (METRIC == METRIC_L2) {
+                            float diff = qa[i] - xa[j];
+                            float s = diff * diff;
+                            acc[i][j] = acc[i][j] + s;
+                        } else {
+                            float p = qa[i] * xa[j];
+                            acc[i][j] = acc[i][j] + p;
+                        }
+                    }
+            }
+        }
+
+        // Filter against each query's threshold and append survivors.
+        bool row_ok[4];
+        float xn[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            uint32_t r = r0 + tr + j;
+            row_ok[j] = r < r_end && mask_allows(mask, r);
+            if (METRIC == METRIC_COSINE) xn[j] = row_ok[j] ? xnorm[r] : 1.0f;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            uint32_t qg = q0 + tq + i;
+            if (qg >= nq) continue;
+            float thr = s_thr[tq + i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float dist;
+                if (METRIC == METRIC_L2) dist = acc[i][j];
+                else if (METRIC == METRIC_INNER_PRODUCT) dist = -acc[i][j];
+                else dist = 1.0f - acc[i][j] / (sqrtf(qn[i]) * sqrtf(xn[j]) + 1e-8f);
+                if (row_ok[j] && dist < thr) {
+                    uint32_t pos = atomicAdd(&s_cnt[tq + i], 1u);
+                    cand[((size_t)chunk * nq + qg) * cap + pos] = make_key(dist, r0 + tr + j);
+                }
+            }
+        }
+        __syncthreads();
+        // Prune every buffer that could overflow during the next tile.
+        for (int qq = wave; qq < kTQ; qq += 4) {
+            uint32_t qg = q0 + qq;
+            if (qg >= nq) break;
+            uint32_t c = s_cnt[qq];
+            if (c + kTN > cap) {
+                uint64_t* buf = cand + ((size_t)chunk * nq + qg) * cap;
+                uint32_t kept = wave_prune(buf, c, keep, my_scratch, lane);
+                if (lane == 0) {
+                    s_cnt[qq] = kept;
+                    if (kept == keep) s_thr[qq] = key_dist(my_scratch[keep - 1]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // Emit each query's sorted `keep` best of this chunk.
+    for (int qq = wave; qq < kTQ; qq += 4) {
+        uint32_t qg = q0 + qq;
+        if (qg >= nq) break;
+        uint32_t c = s_cnt[qq];
+        uint64_t* buf = cand + ((size_t)chunk * nq + qg) * cap;
+        uint32_t kept = wave_prune(buf, c, keep, my_scratch, lane);
+        uint64_t* out = partial + ((size_t)chunk * nq + qg) * keep;
+        for (uint32_t i = lane; i < keep; i += kWave) out[i] = i < kept ? my_scratch[i] : kEmptyKey;
+    }
+}
+
+// Merges the per-chunk sorted lists of one query (one workgroup per query) and
+// writes the first top_k results, remapping slots through vec_ids.
+__global__ __launch_bounds__(256) void merge_partials_kernel(
+    const uint64_t* __restrict__ partial, uint32_t n_chunks, uint32_t nq, uint32_t keep,
+    uint32_t top_k, uint32_t out_stride, const uint32_t* __restrict__ vec_ids,
+    GpuSearchResult* __restrict__ out) {
+    extern __shared__ uint64_t s_keys[];
+    const uint32_t q = blockIdx.x;
+    const uint32_t total = n_chunks * keep;
+    const uint32_t n = next_pow2(total);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        uint64_t k = kEmptyKey;
+        if (i < total) {
+            uint32_t c = i / keep, j = i % keep;
+            k = partial[((size_t)c * nq + q) * keep + j];
+        }
+        s_keys[i] = k;
+    }
+    __syncthreads();
+    block_bitonic_sort(s_keys, n);
+    for (uint32_t i = threadIdx.x; i < top_k; i += blockDim.x) {
+        uint64_t k = s_keys[i];
+        GpuSearchResult r;
+        if (k == kEmptyKey) {
+            r.id = FV_EMPTY_ID;
+            r.distance = FV_EMPTY_DIST;
+        } else {
+            uint32_t slot = key_slot(k);
+            r.id = vec_ids ? vec_ids[slot] : slot;
+            r.distance = key_dist(k);
+        }
+        out[(size_t)q * out_stride + i] = r;
+    }
+}
+
+FlatExactPlan plan_flat_exact(uint32_t n, uint32_t dim, uint32_t nq, uint32_t top_k, int metric) {
+    FlatExactPlan p{};
+    p.keep = top_k;
+    p.cap = next_pow2(top_k + kTN) < 128u ? 128u : next_pow2(top_k + kTN);
+    uint32_t qblocks = (nq + kTQ - 1) / kTQ;
+    uint32_t want = (2048u + qblocks - 1) / qblocks;
+    uint32_t max_by_rows = (n + 255u) / 256u;
+    uint32_t max_by_merge = 8192u / top_k;
+    uint32_t chunks = want;
+    if (chunks > max_by_rows) chunks = max_by_rows;
+    if (chunks > max_by_merge) chunks = max_by_merge;
+    if (chunks < 1) chunks = 1;
+    uint32_t rows = (n + chunks - 1) / chunks;
+    rows = (rows + kTN - 1) / kTN * kTN;
+    p.rows_per_chunk = rows;
+    p.n_chunks = (n + rows - 1) / rows;
+    p.qblocks = qblocks;
+    size_t off = 0;
+    p.off_cand = off;
+    off += (size_t)p.n_chunks * nq * p.cap * sizeof(uint64_t);
+    p.off_partial = off;
+    off += (size_t)p.n_chunks * nq * p.keep * sizeof(uint64_t);
+    p.off_norms = off;
+    if (metric == METRIC_COSINE) off += ((size_t)n + nq) * sizeof(float);
+    off = (off + 255) / 256 * 256;
+    p.workspace_bytes = off;
+    (void)dim;
+    return p;
+}
+
+hipError_t launch_flat_exact(const float* X, const uint32_t* vec_ids, uint32_t n, uint32_t dim,
+                             const float* Qm, uint32_t nq, uint32_t top_k, int metric,
+                             const uint32_t* mask, GpuSearchResult* out, uint32_t out_stride,
+                             void* workspace, const FlatExactPlan& p, hipStream_t stream) {
+    char* ws = static_cast<char*>(workspace);
+    uint64_t* cand = reinterpret_cast<uint64_t*>(ws + p.off_cand);
+    uint64_t* partial = reinterpret_cast<uint64_t*>(ws + p.off_partial);
+    float* xnorm = nullptr;
+    float* qnorm = nullptr;
+    if (metric == METRIC_COSINE) {
+        xnorm = reinterpret_cast<float*>(ws + p.off_norms);
+        qnorm = xnorm + n;
+        row_norms_kernel<<<(n + 255) / 256, 256, 0, stream>>>(X, n, dim, xnorm);
+        row_norms_kernel<<<(nq + 255) / 256, 256, 0, stream>>>(Qm, nq, dim, qnorm);
+    }
+    dim3 grid(p.qblocks, p.n_chunks);
+    size_t lds = (size_t)4 * p.cap * sizeof(uint64_t);
+    void* timer = timer_begin(stream);
+    if (metric == METRIC_INNER_PRODUCT)
+        flat_exact_scan_kernel<METRIC_INNER_PRODUCT><<<grid, kScanThreads, lds, stream>>>(
+            X, n, dim, Qm, nq, xnorm, qnorm, mask, p.keep, p.cap, p.rows_per_chunk, cand, partial);
+    else if (metric == METRIC_COSINE)
+        flat_exact_scan_kernel<METRIC_COSINE><<<grid, kScanThreads, lds, stream>>>(
+            X, n, dim, Qm, nq, xnorm, qnorm, mask, p.keep, p.cap, p.rows_per_chunk, cand, partial);
+    else
+        flat_exact_scan_kernel<METRIC_L2><<<grid, kScanThreads, lds, stream>>>(
+            X, n, dim, Qm, nq, xnorm, qnorm, mask, p.keep, p.cap, p.rows_per_chunk, cand, partial);
+    timer_end(timer, stream);
+    uint32_t total = next_pow2(p.n_chunks * p.keep);
+    merge_partials_kernel<<<nq, 256, (size_t)total * sizeof(uint64_t), stream>>>(
+        partial, p.n_chunks, nq, p.keep, top_k, out_stride, vec_ids, out);
+    return hipGetLastError();
+}
+
+}  // namespace fv
+
+extern "C" {
+
+// cuda_bridge.h:150-162 / kernels/src/sparse_hybrid.cu:504-524.
+int cuda_flat_scan_batch(const float* d_vectors, const uint32_t* d_vec_ids, uint32_t num_vectors,
+                         uint32_t dim, const float* d_queries, uint32_t num_queries,
+                         uint32_t top_k, MetricType metric, const uint32_t* d_valid_mask,
+                         GpuSearchResult* d_out_results, void* stream) {
+    if (!d_vectors || !d_queries || !d_out_results) return -1;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (num_queries == 0 || top_k == 0) return 0;
+    if (top_k > 256u) top_k = 256u;
+    if (num_vectors == 0) {
+        // All bytes 0xFF, as the reference's cudaMemsetAsync (sparse_hybrid.cu:98-101);
+        // a fill kernel rather than a memset node keeps the call graph-capturable.
+        hipError_t e = fv::fill_results(d_out_results, (size_t)num_queries * top_k, 0xFFFFFFFFu,
+                                        0xFFFFFFFFu, s);
+        return e == hipSuccess ? 0 : (int)e;
+    }
+    if (dim == 0) return (int)hipErrorInvalidValue;
+    fv::FlatExactPlan p = fv::plan_flat_exact(num_vectors, dim, num_queries, top_k, (int)metric);
+    void* ws = nullptr;
+    hipError_t e = hipMallocAsync(&ws, p.workspace_bytes, s);
+    if (e != hipSuccess) return (int)e;
+    e = fv::launch_flat_exact(d_vectors, d_vec_ids, num_vectors, dim, d_queries, num_queries,
+                              top_k, (int)metric, d_valid_mask, d_out_results, top_k, ws, p, s);
+    hipError_t e2 = hipFreeAsync(ws, s);
+    if (e != hipSuccess) return (int)e;
+    return e2 == hipSuccess ? 0 : (int)e2;
+}
+
+size_t fv_flat_exact_workspace_size(uint32_t num_vectors, uint32_t dim, uint32_t num_queries,
+                                    uint32_t top_k, MetricType metric) {
+    if (top_k == 0 || top_k > FV_MAX_TOPK || num_vectors == 0) return 0;
+    return fv::plan_flat_exact(num_vectors, dim, num_queries, top_k, (int)metric).workspace_bytes;
+}
+
+int fv_flat_exact_search(const float* d_vectors, uint32_t num_vectors, uint32_t dim,
+                         const float* d_queries, uint32_t num_queries, uint32_t top_k,
+                         MetricType metric, const uint32_t* d_valid_mask,
+                         GpuSearchResult* d_out_results, void* d_workspace,
+                         size_t workspace_bytes, void* stream) {
+    if (!d_vectors || !d_queries || !d_out_results) return -1;
+    if (top_k > FV_MAX_TOPK || dim == 0) return (int)hipErrorInvalidValue;
+    if (num_queries == 0 || top_k == 0) return 0;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (num_vectors == 0) {
+        // All bytes 0xFF, as the reference's cudaMemsetAsync (sparse_hybrid.cu:98-101);
+        // a fill kernel rather than a memset node keeps the call graph-capturable.
+        hipError_t e = fv::fill_results(d_out_results, (size_t)num_queries * top_k, 0xFFFFFFFFu,
+                                        0xFFFFFFFFu, s);
+        return e == hipSuccess ? 0 : (int)e;
+    }
+    fv::FlatExactPlan p = fv::plan_flat_exact(num_vectors, dim, num_queries, top_k, (int)metric);
+    if (!d_workspace || workspace_bytes < p.workspace_bytes) return (int)hipErrorInvalidValue;
+    hipError_t e = fv::launch_flat_exact(d_vectors, nullptr, num_vectors, dim, d_queries,
+                                         num_queries, top_k, (int)metric, d_valid_mask,
+                                         d_out_results, top_k, d_workspace, p, s);
+    return e == hipSuccess ? 0 : (int)e;
+}
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+"""Functional torch-facing wrappers over the C ABI.
+
+Each function takes device tensors, calls the corresponding extern "C" entry
+point on torch's current stream and returns device tensors. Results are int64
+tensors holding GpuSearchResult pairs (low 32 bits id, high 32 bits distance
+bits); `lib.split_results` turns them into numpy (ids, distances).
+"""
+from __future__ import annotations
+
+import ctypes
+
+from . import lib as L
+
+
+def _metric(metric) -> int:
+    return L.METRICS[metric] if isinstance(metric, str) else int(metric)
+
+
+def flat_scan_batch(vectors, queries, top_k: int, metric="l2", valid_mask=None, vec_ids=None,
+                    out=None):
+    """cuda_flat_scan_batch (kernels/src/sparse_hybrid.cu:504-524): exact FP32,
+    top_k clamped to 256."""
+    lib = L.load()
+    nq, dim = queries.shape
+    width = max(1, min(int(top_k), 256))
+    out = L.result_tensor(nq, width, queries.device) if out is None else out
+    rc = lib.cuda_flat_scan_batch(L.ptr(vectors), L.ptr(vec_ids), vectors.shape[0], dim,
+                                  L.ptr(queries), nq, top_k, _metric(metric), L.ptr(valid_mask),
+                                  L.ptr(out), L.stream_handle())
+    L.check(rc, "cuda_flat_scan_batch")
+    return out
+
+
+def flat_exact_search(vectors, queries, top_k: int, metric="l2", valid_mask=None, workspace=None):
+    """fv_flat_exact_search: exact FP32 scan with k up to 1024 and a caller workspace."""
+    torch = L._torch()
+    lib = L.load()
+    nq, dim = queries.shape
+    n = vectors.shape[0]
+    need = lib.fv_flat_exact_workspace_size(n, dim, nq, top_k, _metric(metric))
+    if workspace is None:
+        workspace = torch.empty(max(need, 1), dtype=torch.uint8, device=queries.device)
+    out = L.result_tensor(nq, top_k, queries.device)
+    rc = lib.fv_flat_exact_search(L.ptr(vectors), n, dim, L.ptr(queries), nq, top_k,
+                                  _metric(metric), L.ptr(valid_mask), L.ptr(out),
+                                  L.ptr(workspace), workspace.numel(), L.stream_handle())
+    L.check(rc, "fv_flat_exact_search")
+    return out
+
+
+def merge_topk(shard_results, out_top_k: int):
+    """cuda_merge_topk over an int64 [num_shards, num_queries, shard_top_k] tensor."""
+    lib = L.load()
+    s, nq, k = shard_results.shape
+    width = max(1, min(int(out_top_k), 256))
+    out = L.result_tensor(nq, width, shard_results.device)
+    rc = lib.cuda_merge_topk(L.ptr(shard_results), s, nq, k, out_top_k, L.ptr(out),
+                             L.stream_handle())
+    L.check(rc, "cuda_merge_topk")
+    return out
+
+
+def compute_distances(queries, dataset, metric="l2"):
+    """cuda_compute_distances_warp: dense [nq, n] exact FP32 distance matrix."""
+    torch = L._torch()
+    lib = L.load()
+    out = torch.empty((queries.shape[0], dataset.shape[0]), dtype=torch.float32,
+                      device=queries.device)
+    rc = lib.cuda_compute_distances_warp(L.ptr(queries), L.ptr(dataset), queries.shape[0],
+                                         dataset.shape[0], queries.shape[1], _metric(metric),
+                                         L.ptr(out), L.stream_handle())
+    L.check(rc, "cuda_compute_distances_warp")
+    return out
+
+
+def bitonic_sort_test(keys, vals):
+    """cuda_bitonic_sort_test: one-wavefront register sort of up to 64 pairs."""
+    torch = L._torch()
+    lib = L.load()
+    ko = torch.empty_like(keys)
+    vo = torch.empty_like(vals)
+    rc = lib.cuda_bitonic_sort_test(L.ptr(keys), L.ptr(vals), L.ptr(ko), L.ptr(vo),
+                                    keys.numel(), L.stream_handle())
+    L.check(rc, "cuda_bitonic_sort_test")
+    return ko, vo
+
+
+def device_count() -> int:
+    lib = L.load()
+    c = ctypes.c_int(0)
+    L.check(lib.cuda_get_device_count(ctypes.byref(c)), "cuda_get_device_count")
+    return c.value
+
+
+class FlatIndex:
+    """Device-resident flat tables for the matrix-core scan (fv_flat_search).
+
+    Holds the FP32 rows (for the exact rerank), their 16-bit codes padded to a
+    multiple of 32 dims, FP32 and FP64 norms, all encoded on the GPU.
+    """
+
+    def __init__(self, vectors, dtype="f16", keep_fp32=True, padded_dim=None):
+        torch = L._torch()
+        lib = L.load()
+        self.vectors = vectors.contiguous()
+        n, dim = self.vectors.shape
+        self.dim, self.n = dim, n
+        self.dtype = dtype
+        pd = padded_dim or (dim + 31) // 32 * 32
+        self.padded_dim = pd
+        dev = self.vectors.device
+        self.codes = torch.empty((n, pd), dtype=torch.int16, device=dev)
+        self.norms = torch.empty(n, dtype=torch.float32, device=dev)
+        self.norms64 = torch.empty(n, dtype=torch.float64, device=dev)
+        L.check(lib.fv_encode_flat(L.ptr(self.vectors), n, dim, pd, L.DTYPES[dtype],
+                                   L.ptr(self.codes), L.ptr(self.norms), L.ptr(self.norms64),
+                                   None, L.stream_handle()), "fv_encode_flat")
+        self.keep_fp32 = keep_fp32
+        self.tables = L.FvFlatTables(L.ptr(self.codes), None, L.ptr(self.norms),
+                                     L.ptr(self.norms64),
+                                     L.ptr(self.vectors) if keep_fp32 else None,
+                                     n, dim, pd, L.DTYPES[dtype])
+        self._ws = None
+
+    def workspace_size(self, nq, top_k, shortlist=0):
+        return L.load().fv_flat_workspace_size(ctypes.byref(self.tables), nq, top_k, shortlist)
+
+    def search(self, queries, top_k, metric="l2", valid_mask=None, shortlist=0, out=None):
+        torch = L._torch()
+        lib = L.load()
+        nq = queries.shape[0]
+        need = self.workspace_size(nq, top_k, shortlist)
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = torch.empty(max(need, 1), dtype=torch.uint8, device=queries.device)
+        out = L.result_tensor(nq, top_k, queries.device) if out is None else out
+        rc = lib.fv_flat_search(ctypes.byref(self.tables), L.ptr(queries), nq, top_k, shortlist,
+                                _metric(metric), L.ptr(valid_mask), L.ptr(out), L.ptr(self._ws),
+                                self._ws.numel(), L.stream_handle())
+        L.check(rc, "fv_flat_search")
+        return out
+
+
+def fp16_tc_search_batch(codes_fp16, norms, queries, top_k, dim, metric="l2", valid_mask=None,
+                         vectors=None):
+    """cuda_fp16_tc_search_batch over Fp16GpuTables built from caller tensors
+    (codes: int16/uint16 [n, padded_dim], norms: float32 [n])."""
+    lib = L.load()
+    n, pd = codes_fp16.shape
+    t = L.Fp16GpuTables(L.ptr(codes_fp16), L.ptr(norms), L.ptr(vectors), n, dim, pd)
+    nq = queries.shape[0]
+    width = max(1, min(int(top_k), 256))
+    out = L.result_tensor(nq, width, queries.device)
+    rc = lib.cuda_fp16_tc_search_batch(ctypes.byref(t), L.ptr(queries), nq, top_k,
+                                       _metric(metric), L.ptr(valid_mask), L.ptr(out),
+                                       L.stream_handle())
+    L.check(rc, "cuda_fp16_tc_search_batch")
+    return out

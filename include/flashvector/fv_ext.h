@@ -1,0 +1,85 @@
+/*
+ * Additive entry points of the MI355X query-scan library. None of these exist
+ * in the reference; they lift its host limits (top_k <= 256, per-call device
+ * allocation) and expose the reduced-precision scan, uneven batches and the
+ * top-k backward pass. The reference-named entry points in cuda_bridge.h are
+ * unchanged by them.
+ *
+ * Conventions as in cuda_bridge.h. Entry points that take a workspace never
+ * allocate: the caller passes at least *_workspace_size() bytes of device
+ * memory (256-byte aligned) and must not share it between concurrent calls.
+ */
+#ifndef FLASHVECTOR_AMD_FV_EXT_H
+#define FLASHVECTOR_AMD_FV_EXT_H
+
+#include "types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FV_MAX_TOPK 1024u
+#define FV_MAX_SHORTLIST 16384u
+
+/* ---- exact FP32 flat scan without the 256 clamp ------------------------ */
+size_t fv_flat_exact_workspace_size(uint32_t num_vectors, uint32_t dim, uint32_t num_queries,
+                                    uint32_t top_k, MetricType metric);
+int fv_flat_exact_search(const float* d_vectors, uint32_t num_vectors, uint32_t dim,
+                         const float* d_queries, uint32_t num_queries, uint32_t top_k,
+                         MetricType metric, const uint32_t* d_valid_mask,
+                         GpuSearchResult* d_out_results, void* d_workspace,
+                         size_t workspace_bytes, void* stream);
+
+/* ---- reduced-precision flat scan on the matrix cores --------------------
+ * Rows are stored as 16-bit codes (FP16 or BF16, RNE from FP32) padded to
+ * padded_dim (a multiple of 16, at most 256). The scan scores every row on
+ * MFMA with FP32 accumulation, keeps a shortlist of `shortlist` rows per query
+ * (0 = 8 * top_k), reranks it exactly in FP32 against d_vectors (the reference
+ * CPU path's arithmetic) and returns the top_k by (distance, row). With
+ * d_vectors NULL the reduced-precision scores are returned instead. */
+typedef enum {
+    FV_DTYPE_F32 = 0,
+    FV_DTYPE_F16 = 1,
+    FV_DTYPE_BF16 = 2,
+    FV_DTYPE_FP8_E4M3 = 3
+} FvDtype;
+
+typedef struct {
+    const void* d_codes;          /* [num_vectors * padded_dim] codes of `dtype` */
+    const float* d_row_scales;    /* [num_vectors] dequantisation scale (FP8) or NULL */
+    const float* d_norms_sq;      /* [num_vectors] |x|^2 of the FP32 rows */
+    const double* d_norms_sq_f64; /* [num_vectors] |x|^2 in FP64, or NULL */
+    const float* d_vectors;       /* [num_vectors * dim] FP32 rows for the rerank, or NULL */
+    uint32_t num_vectors;
+    uint32_t dim;
+    uint32_t padded_dim;
+    uint32_t dtype;               /* FvDtype */
+} FvFlatTables;
+
+/* Encodes FP32 rows into FvFlatTables storage on the device: codes (padded
+ * with zeros), FP32 norms as sequential non-fused sums, FP64 norms. Any output
+ * pointer may be NULL to skip it. */
+int fv_encode_flat(const float* d_vectors, uint32_t num_vectors, uint32_t dim,
+                   uint32_t padded_dim, uint32_t dtype, void* d_codes_out,
+                   float* d_norms_out, double* d_norms_f64_out, float* d_row_scales_out,
+                   void* stream);
+
+size_t fv_flat_workspace_size(const FvFlatTables* tables, uint32_t num_queries, uint32_t top_k,
+                              uint32_t shortlist);
+int fv_flat_search(const FvFlatTables* tables, const float* d_queries, uint32_t num_queries,
+                   uint32_t top_k, uint32_t shortlist, MetricType metric,
+                   const uint32_t* d_valid_mask, GpuSearchResult* d_out_results,
+                   void* d_workspace, size_t workspace_bytes, void* stream);
+
+/* ---- kernel timer --------------------------------------------------------
+ * While enabled, every search call brackets its dominant scan kernel with two
+ * HIP events recorded on the call's stream. fv_kernel_timer_read waits for the
+ * recorded events and returns their summed elapsed time and count (reset != 0
+ * clears them). Process-wide; meant for benchmarks. */
+int fv_kernel_timer_enable(int enable);
+int fv_kernel_timer_read(double* total_ms, uint32_t* launches, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLASHVECTOR_AMD_FV_EXT_H */

@@ -1,0 +1,121 @@
+"""GPU parity of the matrix-core flat scan with exact FP32 rerank.
+
+On integer mixture data every FP16/BF16 score is exact, so the shortlist
+holds the true top-S and the reranked output must equal the oracle's exact
+FP32 top-k bit for bit. On real-valued data the bar is recall against the
+oracle plus exact FP32 distances for every returned id."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from flashvector_amd import lib as L
+from flashvector_amd import search as F
+from flashvector_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    assert t.cuda.is_available(), "GPU tests need a ROCm device"
+    L.load()
+    O.build()
+    return t
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def run(torch, db, qs, k, metric="l2", dtype="f16", mask=None, shortlist=0, keep_fp32=True):
+    idx = F.FlatIndex(dev(torch, db), dtype, keep_fp32=keep_fp32)
+    out = idx.search(dev(torch, qs), k, metric, None if mask is None else dev(torch, mask),
+                     shortlist)
+    torch.cuda.synchronize()
+    return L.split_results(out)
+
+
+def assert_same(got, exp):
+    np.testing.assert_array_equal(got[0], exp[0])
+    np.testing.assert_array_equal(got[1].view(np.uint32), exp[1].view(np.uint32))
+
+
+@pytest.mark.parametrize("dtype", ["f16", "bf16"])
+@pytest.mark.parametrize("n,nq,dim,k,metric", [
+    (20000, 300, 128, 10, "l2"),
+    (20000, 300, 128, 10, "ip"),
+    (5000, 17, 100, 25, "l2"),      # padded dims, partial query block
+    (3000, 40, 16, 5, "l2"),        # one K step
+    (9000, 64, 256, 100, "l2"),     # eight K steps, deep shortlist
+    (150, 9, 64, 40, "l2"),         # fewer rows than the shortlist
+])
+def test_integer_data_exact(torch, dtype, n, nq, dim, k, metric):
+    db, qs = S.flat_workload(n, nq, dim, 32, seeds=(n + dim, nq, 1))
+    assert_same(run(torch, db, qs, k, metric, dtype), O.flat_topk(db, qs, k, metric))
+
+
+@pytest.mark.parametrize("dtype", ["f16", "bf16"])
+def test_integer_data_masked(torch, dtype):
+    db, qs = S.flat_workload(60000, 500, 128, 256, seeds=(11, 12, 13))
+    mask = S.random_valid_mask(60000, 0.05, 13)
+    got = run(torch, db, qs, 10, "l2", dtype, mask=mask)
+    assert_same(got, O.flat_topk(db, qs, 10, "l2", valid_mask=mask))
+    bits = S.unpack_bits(mask, 60000)
+    assert bits[got[0][got[0] != 0xFFFFFFFF]].all()
+
+
+def test_golden_fixture_fp16(torch):
+    g = np.load(os.path.join(GOLDEN, "flat_small.npz"))
+    db, qs = g["db_u8"].astype(np.float32), g["queries_u8"].astype(np.float32)
+    assert_same(run(torch, db, qs, 10, "l2"), (g["l2_ids"], g["l2_dist"]))
+    assert_same(run(torch, db, qs, 10, "ip"), (g["ip_ids"], g["ip_dist"]))
+    assert_same(run(torch, db, qs, 10, "l2", mask=g["mask"]),
+                (g["l2_masked_ids"], g["l2_masked_dist"]))
+
+
+def test_real_valued_recall_and_exact_distances(torch):
+    rng = np.random.default_rng(5)
+    db = rng.normal(size=(30000, 96)).astype(np.float32)
+    qs = rng.normal(size=(200, 96)).astype(np.float32)
+    for metric in ("l2", "ip", "cosine"):
+        ids, d = run(torch, db, qs, 10, metric, "f16")
+        e_ids, e_d = O.flat_topk(db, qs, 10, metric)
+        hits = sum(len(set(ids[i]) & set(e_ids[i])) for i in range(200))
+        assert hits / (200 * 10) >= 0.999, metric
+        exact = O.distance_matrix(qs, db, metric)
+        np.testing.assert_array_equal(d, np.take_along_axis(exact, ids.astype(np.int64), 1))
+
+
+def test_no_rerank_returns_approximate_scores(torch):
+    db, qs = S.flat_workload(8000, 50, 128, 32, seeds=(3, 4, 5))
+    ids, d = run(torch, db, qs, 10, "l2", keep_fp32=False)
+    e_ids, e_d = O.flat_topk(db, qs, 10, "l2")
+    np.testing.assert_array_equal(ids, e_ids)          # integer data: scores exact
+    np.testing.assert_allclose(d, e_d, rtol=1e-6)
+
+
+def test_fp16_tc_entry_point_reference_encoding(torch):
+    """cuda_fp16_tc_search_batch fed with the reference's own FP16 encoding
+    (crates/engine/src/quant.rs:306-327, via the oracle)."""
+    db, qs = S.flat_workload(10000, 70, 120, 32, seeds=(21, 22, 23))
+    codes, norms = O.encode_fp16_padded(db)
+    mask = S.random_valid_mask(10000, 0.5, 24)
+    out = F.fp16_tc_search_batch(dev(torch, codes.view(np.int16)), dev(torch, norms),
+                                 dev(torch, qs), 12, 120, "l2", dev(torch, mask), dev(torch, db))
+    torch.cuda.synchronize()
+    assert_same(L.split_results(out), O.flat_topk(db, qs, 12, "l2", valid_mask=mask))
+
+
+def test_mfma_layout_probe_matches_cpu(torch):
+    """The kernel's fragment layouts reproduce a plain product: scores of the
+    no-rerank path equal 0.5|x|^2 - q.x exactly on small integers."""
+    rng = np.random.default_rng(9)
+    db = rng.integers(0, 8, size=(64, 32)).astype(np.float32)
+    qs = rng.integers(0, 8, size=(16, 32)).astype(np.float32)
+    ids, d = run(torch, db, qs, 64, "ip", keep_fp32=False)
+    ref = -(qs.astype(np.float64) @ db.astype(np.float64).T)
+    np.testing.assert_array_equal(np.take_along_axis(ref, ids.astype(np.int64), 1), d)
