@@ -1,0 +1,30 @@
+// Internal launch plan of the IVF-PQ scan.
+#pragma once
+
+#include "fv_flat_exact.hpp"
+
+namespace fv {
+
+constexpr uint32_t kMaxNlist = 8192;  // coarse selection sorts all centroids in 64 KB of LDS
+
+struct IvfPqPlan {
+    uint32_t shortlist;   // S, the ADC shortlist depth (>= every k)
+    uint32_t max_nprobe;
+    uint32_t cap;         // candidate buffer capacity per query
+    uint32_t npow;        // next power of two >= nlist
+    bool gbuf;            // candidate buffer in global memory (deep shortlists)
+    size_t lds_scan;      // dynamic LDS of the scan kernel
+    size_t off_probes, off_gbuf;
+    size_t workspace_bytes;
+};
+
+IvfPqPlan plan_ivfpq(const IvfPqGpuTables* t, uint32_t nq, uint32_t max_nprobe,
+                     uint32_t shortlist, uint32_t max_k);
+
+hipError_t launch_ivfpq(const IvfPqGpuTables* t, const uint8_t* codes_lo, const float* Qm,
+                        uint32_t nq, const uint32_t* k_q, uint32_t k_u, const uint32_t* nprobe_q,
+                        uint32_t nprobe_u, int do_rerank, int metric, const uint32_t* mask,
+                        GpuSearchResult* out, uint32_t out_stride, void* workspace,
+                        const IvfPqPlan& p, hipStream_t stream);
+
+}  // namespace fv

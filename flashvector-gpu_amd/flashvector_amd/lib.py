@@ -102,6 +102,10 @@ SIGNATURES = {
     "fv_encode_flat": (I, [P, U, U, U, U, P, P, P, P, P]),
     "fv_flat_workspace_size": (SZ, [P, U, U, U]),
     "fv_flat_search": (I, [P, P, U, U, U, I, P, P, P, SZ, P]),
+    "cuda_ivf_pq_search_batch": (I, [P, P, U, U, U, U, I, P, P, P]),
+    "fv_ivf_pq_list_order_codes": (I, [P, P, P]),
+    "fv_ivf_pq_workspace_size": (SZ, [P, U, U, U, U]),
+    "fv_ivf_pq_search": (I, [P, P, P, U, P, U, P, U, U, I, I, P, P, P, SZ, P]),
     "fv_kernel_timer_enable": (I, [I]),
     "fv_kernel_timer_read": (I, [P, P, I]),
 }

@@ -155,3 +155,75 @@ def fp16_tc_search_batch(codes_fp16, norms, queries, top_k, dim, metric="l2", va
                                        L.stream_handle())
     L.check(rc, "cuda_fp16_tc_search_batch")
     return out
+
+
+class IvfPqIndex:
+    """Device-resident IVF-PQ tables (IvfPqGpuTables) plus a list-ordered copy
+    of the codes for coalesced streaming (fv_ivf_pq_list_order_codes)."""
+
+    def __init__(self, centroids, codebooks, codes, offsets, vec_ids, vectors=None,
+                 list_order=True):
+        torch = L._torch()
+        lib = L.load()
+        self.centroids = centroids.contiguous()
+        self.codebooks = codebooks.contiguous()
+        self.codes = codes.contiguous()
+        self.offsets = offsets.contiguous()
+        self.vec_ids = vec_ids.contiguous()
+        self.vectors = None if vectors is None else vectors.contiguous()
+        nlist, dim = self.centroids.shape
+        n, m_pq = self.codes.shape
+        self.tables = L.IvfPqGpuTables(
+            L.ptr(self.centroids), L.ptr(self.codebooks), L.ptr(self.codes), L.ptr(self.offsets),
+            L.ptr(self.vec_ids), L.ptr(self.vectors), n, dim, nlist, m_pq, dim // m_pq)
+        self.codes_lo = None
+        if list_order:
+            self.codes_lo = torch.empty_like(self.codes)
+            L.check(lib.fv_ivf_pq_list_order_codes(ctypes.byref(self.tables), L.ptr(self.codes_lo),
+                                                   L.stream_handle()),
+                    "fv_ivf_pq_list_order_codes")
+        self._ws = None
+
+    @classmethod
+    def from_host(cls, t, device="cuda", keep_vectors=True, list_order=True):
+        """From flashvector_amd.ivfpq_build.IvfPqTables (numpy)."""
+        torch = L._torch()
+
+        def d(a):
+            return torch.from_numpy(a).to(device)
+        return cls(d(t.centroids), d(t.codebooks.reshape(t.m_pq, -1)), d(t.codes),
+                   d(t.offsets.view("int32")), d(t.vec_ids.view("int32")),
+                   d(t.vectors) if keep_vectors else None, list_order=list_order)
+
+    def search_batch(self, queries, top_k, nprobe, rerank_k, metric="l2", valid_mask=None):
+        """cuda_ivf_pq_search_batch (kernels/src/ivf_pq_lookup.cu:450-473)."""
+        lib = L.load()
+        nq = queries.shape[0]
+        width = max(1, min(int(top_k), 256))
+        out = L.result_tensor(nq, width, queries.device)
+        rc = lib.cuda_ivf_pq_search_batch(ctypes.byref(self.tables), L.ptr(queries), nq, top_k,
+                                          nprobe, rerank_k, _metric(metric), L.ptr(valid_mask),
+                                          L.ptr(out), L.stream_handle())
+        L.check(rc, "cuda_ivf_pq_search_batch")
+        return out
+
+    def search(self, queries, max_top_k, max_nprobe, shortlist, top_k=None, nprobe=None,
+               rerank=True, metric="l2", valid_mask=None, use_list_order=True, out=None):
+        """fv_ivf_pq_search: optional per-query top_k / nprobe (uint32 device tensors)."""
+        torch = L._torch()
+        lib = L.load()
+        nq = queries.shape[0]
+        need = lib.fv_ivf_pq_workspace_size(ctypes.byref(self.tables), nq, max_nprobe, shortlist,
+                                            max_top_k)
+        if need == 0:
+            raise L.FvError("invalid IVF-PQ search shape")
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = torch.empty(need, dtype=torch.uint8, device=queries.device)
+        out = L.result_tensor(nq, max_top_k, queries.device) if out is None else out
+        lo = self.codes_lo if use_list_order else None
+        rc = lib.fv_ivf_pq_search(ctypes.byref(self.tables), L.ptr(lo), L.ptr(queries), nq,
+                                  L.ptr(top_k), max_top_k, L.ptr(nprobe), max_nprobe, shortlist,
+                                  int(rerank), _metric(metric), L.ptr(valid_mask), L.ptr(out),
+                                  L.ptr(self._ws), self._ws.numel(), L.stream_handle())
+        L.check(rc, "fv_ivf_pq_search")
+        return out

@@ -71,6 +71,25 @@ int fv_flat_search(const FvFlatTables* tables, const float* d_queries, uint32_t 
                    const uint32_t* d_valid_mask, GpuSearchResult* d_out_results,
                    void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* ---- IVF-PQ scan with uneven batches ------------------------------------
+ * Same arithmetic and ties rule as cuda_ivf_pq_search_batch, without its 256
+ * and 10240 clamps: per-query d_top_k[i] (<= max_top_k <= 1024) and
+ * d_nprobe[i] (<= max_nprobe) when those pointers are non-NULL (else the
+ * uniform max values apply), shortlist depth max(shortlist, k_i) <= 16384.
+ * Output rows have stride max_top_k and are padded past k_i. When
+ * d_codes_list_order is non-NULL it must hold the codes in posting-list order
+ * (fv_ivf_pq_list_order_codes), which turns the per-entry gathers into
+ * coalesced streaming reads. nlist <= 8192. */
+int fv_ivf_pq_list_order_codes(const IvfPqGpuTables* tables, uint8_t* d_codes_out, void* stream);
+size_t fv_ivf_pq_workspace_size(const IvfPqGpuTables* tables, uint32_t num_queries,
+                                uint32_t max_nprobe, uint32_t shortlist, uint32_t max_top_k);
+int fv_ivf_pq_search(const IvfPqGpuTables* tables, const uint8_t* d_codes_list_order,
+                     const float* d_queries, uint32_t num_queries, const uint32_t* d_top_k,
+                     uint32_t max_top_k, const uint32_t* d_nprobe, uint32_t max_nprobe,
+                     uint32_t shortlist, int rerank, MetricType metric,
+                     const uint32_t* d_valid_mask, GpuSearchResult* d_out_results,
+                     void* d_workspace, size_t workspace_bytes, void* stream);
+
 /* ---- kernel timer --------------------------------------------------------
  * While enabled, every search call brackets its dominant scan kernel with two
  * HIP events recorded on the call's stream. fv_kernel_timer_read waits for the

@@ -1,0 +1,109 @@
+"""GPU parity of the IVF-PQ ADC scan against the oracle and the golden
+fixture. Bar: bit-exact ids and distances given identical tables."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from flashvector_amd import lib as L
+from flashvector_amd import search as F
+from flashvector_amd import synthetic as S
+from flashvector_amd.ivfpq_build import build_ivfpq
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    assert t.cuda.is_available(), "GPU tests need a ROCm device"
+    L.load()
+    O.build()
+    return t
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def assert_same(got, exp):
+    np.testing.assert_array_equal(got[0], exp[0])
+    np.testing.assert_array_equal(got[1].view(np.uint32), exp[1].view(np.uint32))
+
+
+def host_tables(t):
+    return O.IvfPqHost(t.centroids, t.codebooks, t.codes, t.offsets, t.vec_ids, t.vectors, t.m_pq)
+
+
+@pytest.fixture(scope="module")
+def case(torch):
+    db, qs = S.flat_workload(30000, 150, 64, 48, seeds=(41, 42, 43))
+    t = build_ivfpq(db, 64, 16, seed=44, iters=4)
+    return db, qs, t, F.IvfPqIndex.from_host(t), host_tables(t)
+
+
+def test_golden_fixture(torch):
+    g = np.load(os.path.join(GOLDEN, "ivfpq_small.npz"))
+    db = g["db_u8"].astype(np.float32)
+    qs = g["queries_u8"].astype(np.float32)
+    idx = F.IvfPqIndex(dev(torch, g["centroids"]), dev(torch, g["codebooks"].reshape(8, -1)),
+                       dev(torch, g["codes"]), dev(torch, g["offsets"].view(np.int32)),
+                       dev(torch, g["vec_ids"].view(np.int32)), dev(torch, db))
+    ci = 0
+    while f"case{ci}_params" in g:
+        k, npb, rr, masked = (int(v) for v in g[f"case{ci}_params"])
+        m = dev(torch, g["mask"]) if masked else None
+        got = L.split_results(idx.search_batch(dev(torch, qs), k, npb, rr, "l2", m))
+        assert_same(got, (g[f"case{ci}_ids"], g[f"case{ci}_dist"]))
+        ci += 1
+    assert ci >= 5
+
+
+@pytest.mark.parametrize("k,nprobe,rerank_k,metric", [
+    (10, 8, 100, "l2"), (10, 8, 1, "l2"), (5, 3, 0, "l2"), (30, 16, 2000, "l2"),
+    (10, 64, 9000, "l2"),          # deep shortlist: candidate buffer in global memory
+    (20, 12, 300, "ip"), (20, 12, 300, "cosine"), (256, 20, 500, "l2"), (300, 4, 50, "l2"),
+])
+def test_legacy_entry_vs_oracle(torch, case, k, nprobe, rerank_k, metric):
+    db, qs, t, idx, h = case
+    got = L.split_results(idx.search_batch(dev(torch, qs), k, nprobe, rerank_k, metric))
+    assert_same(got, O.ivf_pq_search_batch(h, qs, k, nprobe, rerank_k, metric))
+
+
+def test_mask(torch, case):
+    db, qs, t, idx, h = case
+    mask = S.random_valid_mask(db.shape[0], 0.3, 5)
+    got = L.split_results(idx.search_batch(dev(torch, qs), 10, 8, 200, "l2", dev(torch, mask)))
+    assert_same(got, O.ivf_pq_search_batch(h, qs, 10, 8, 200, valid_mask=mask))
+    bits = S.unpack_bits(mask, db.shape[0])
+    assert bits[got[0][got[0] != 0xFFFFFFFF]].all()
+
+
+@pytest.mark.parametrize("list_order", [True, False])
+def test_uneven_batch_vs_oracle(torch, case, list_order):
+    db, qs, t, idx, h = case
+    rng = np.random.default_rng(7)
+    ks = rng.choice([10, 100], size=qs.shape[0]).astype(np.uint32)
+    nps = rng.choice([8, 16, 32, 64], size=qs.shape[0]).astype(np.uint32)
+    out = idx.search(dev(torch, qs), 100, 64, 1000, top_k=dev(torch, ks.view(np.int32)),
+                     nprobe=dev(torch, nps.view(np.int32)), use_list_order=list_order)
+    got = L.split_results(out)
+    assert_same(got, O.ivf_pq_search_uneven(h, qs, ks, nps, 1000, out_stride=100))
+
+
+def test_integer_codebooks_order_independent(torch):
+    db, qs = S.flat_workload(8000, 40, 32, 24, seeds=(51, 52, 53))
+    t = build_ivfpq(db, 16, 8, seed=54, iters=3, integer_codebooks=True)
+    idx = F.IvfPqIndex.from_host(t)
+    got = L.split_results(idx.search_batch(dev(torch, qs), 15, 5, 1))
+    assert_same(got, O.ivf_pq_search_batch(host_tables(t), qs, 15, 5, 1))
+
+
+def test_exhaustive_probe_matches_oracle_and_flat(torch, case):
+    db, qs, t, idx, h = case
+    got = L.split_results(idx.search_batch(dev(torch, qs[:20]), 10, t.nlist, 10240))
+    assert_same(got, O.ivf_pq_search_batch(h, qs[:20], 10, t.nlist, 10240))
+    flat = O.flat_scan_batch(db, qs[:20], 10)
+    assert np.mean(got[0] == flat[0]) >= 0.99
