@@ -62,7 +62,7 @@ __device__ __forceinline__ void wave_bitonic_sort(uint64_t* keys, uint32_t n, in
     for (uint32_t k = 2; k <= n; k <<= 1) {
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
             for (uint32_t p = (uint32_t)lane; p < (n >> 1); p += kWave) {
-                uint32_t i = ((p / j) * (2u * j)) + (p % j);
+                uint32_t i = ((p & ~(j - 1u)) << 1) | (p & (j - 1u));
                 uint32_t m = i + j;
                 uint64_t a = keys[i], b = keys[m];
                 bool up = (i & k) == 0;
@@ -81,7 +81,7 @@ __device__ __forceinline__ void block_bitonic_sort(uint64_t* keys, uint32_t n) {
     for (uint32_t k = 2; k <= n; k <<= 1) {
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
             for (uint32_t p = threadIdx.x; p < (n >> 1); p += blockDim.x) {
-                uint32_t i = ((p / j) * (2u * j)) + (p % j);
+                uint32_t i = ((p & ~(j - 1u)) << 1) | (p & (j - 1u));
                 uint32_t m = i + j;
                 uint64_t a = keys[i], b = keys[m];
                 bool up = (i & k) == 0;
@@ -109,6 +109,59 @@ __device__ __forceinline__ uint32_t wave_prune(uint64_t* __restrict__ buf, uint3
     for (uint32_t i = (uint32_t)lane; i < kept; i += kWave) buf[i] = scratch[i];
     wave_lds_sync();
     return kept;
+}
+
+// Prune of one query's candidate buffer (count <= 64*R keys in global memory)
+// by one wavefront without sorting: bisection over the ordered distance
+// finds a bound t with at least `keep` keys at or below it (ballot +
+// popcount per probe), then the keys above t are dropped by an in-order
+// compaction (mbcnt prefix). Every dropped key has at least `keep` kept keys
+// at or below its distance, and all of them come from earlier rows, so none
+// of the dropped keys (nor any later row at distance >= t) can enter the
+// chunk's top-`keep`. Returns the new count; *bound = t.
+template <int R>
+__device__ __forceinline__ uint32_t wave_bisect_prune(uint64_t* __restrict__ buf, uint32_t count,
+                                                      uint32_t keep, int lane, uint32_t* bound) {
+    uint64_t v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t i = (uint32_t)(r * kWave + lane);
+        v[r] = i < count ? buf[i] : kEmptyKey;
+    }
+    if (count <= keep) {
+        *bound = 0xFFFFFFFFu;
+        return count;
+    }
+    // Smallest t with |{key : ord(key) <= t}| >= keep, stopping early once the
+    // kept set is within one wavefront of keep.
+    uint32_t lo = 0u, hi = 0xFFFFFFFEu;
+    uint32_t t = hi;
+    while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        uint32_t c = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) c += __popcll(__ballot((uint32_t)(v[r] >> 32) <= mid));
+        if (c >= keep) {
+            hi = mid;
+            if (c <= keep + kWave) break;
+        } else {
+            lo = mid + 1u;
+        }
+    }
+    t = hi;
+    // In-order compaction of the keys at or below t.
+    uint32_t base = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const bool k = (uint32_t)(v[r] >> 32) <= t;
+        const uint64_t m = __ballot(k);
+        const uint32_t pos = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (k) buf[pos] = v[r];
+        base += __popcll(m);
+    }
+    *bound = t;
+    return base;
 }
 
 // Exact FP32 distances in the reference CPU path's order: one sequential,
@@ -143,6 +196,44 @@ __device__ __forceinline__ float exact_distance(const float* __restrict__ q,
         }
         return acc;
     }
+}
+
+// Same sums with 128-bit row loads (the order over d is unchanged); x must be
+// 16-byte aligned and dim a multiple of 4, else the scalar form runs.
+template <int METRIC>
+__device__ __forceinline__ float exact_distance_vec(const float* __restrict__ q,
+                                                    const float* __restrict__ x, uint32_t dim) {
+    if ((dim & 3u) || (reinterpret_cast<uintptr_t>(x) & 15u) || (reinterpret_cast<uintptr_t>(q) & 15u))
+        return exact_distance<METRIC>(q, x, dim);
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    const float4* q4 = reinterpret_cast<const float4*>(q);
+    float a = 0.0f, nq = 0.0f, nx = 0.0f;
+#pragma unroll 4
+    for (uint32_t i = 0; i < dim / 4; ++i) {
+        const float4 xv = x4[i], qv = q4[i];
+        const float xs[4] = {xv.x, xv.y, xv.z, xv.w};
+        const float qs[4] = {qv.x, qv.y, qv.z, qv.w};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if (METRIC == METRIC_L2) {
+                float diff = qs[c] - xs[c];
+                float sq = diff * diff;
+                a = a + sq;
+            } else {
+                float p = qs[c] * xs[c];
+                a = a + p;
+                if (METRIC == METRIC_COSINE) {
+                    float pa = qs[c] * qs[c], pb = xs[c] * xs[c];
+                    nq = nq + pa;
+                    nx = nx + pb;
+                }
+            }
+        }
+    }
+    if (METRIC == METRIC_L2) return a;
+    if (METRIC == METRIC_INNER_PRODUCT) return -a;
+    float denom = sqrtf(nq) * sqrtf(nx) + 1e-8f;
+    return 1.0f - a / denom;
 }
 
 __device__ __forceinline__ float exact_distance_rt(const float* q, const float* x, uint32_t dim,

@@ -11,23 +11,29 @@
 //    queries, C = 0.5*|x|^2 of each row. D = 0.5|x|^2 - q.x, so the L2
 //    distance is |q|^2 + 2D and the |x|^2 term costs nothing in the epilogue.
 //    Masked and out-of-range rows get C = +inf and never pass a threshold.
-//  * A workgroup owns 256 queries (4 waves x 64) and one chunk of rows. The
-//    queries' B fragments live in VGPRs for the whole kernel; 64-row database
-//    tiles are double-buffered through LDS (rows padded to dodge bank
-//    conflicts), one barrier per tile.
-//  * Selection: each lane reduces its 4 scores per (row tile, query tile) with
-//    min and compares against the query's threshold; the rare survivors are
-//    appended to a per-(chunk, query) buffer in global memory. Buffers near
-//    capacity are pruned by their owning wave (queries are wave-private, so
-//    prunes need no workgroup barrier). Thresholds combine the chunk's own
-//    S-th best (strict: rows arrive in ascending order) with a per-query bound
-//    shared by all chunks in global memory (non-strict: the S-th best of any
-//    chunk bounds the global S-th best).
+//  * A workgroup owns 256 queries (4 waves x 64) and one chunk of the row
+//    space. The queries' B fragments live in VGPRs for the whole kernel;
+//    64-row tiles are double-buffered through LDS (rows padded so the 128-bit
+//    fragment reads are conflict-free), one barrier per tile.
+//  * The row space is either every row (the mask, if any, is tested per row)
+//    or a compacted ascending list of the valid rows (sparse masks). Chunk
+//    bounds are computed on the device from the actual row count.
+//  * Selection: each lane takes the min of its 4 scores per (row tile, query
+//    tile) and compares it with the query's threshold; the rare survivors are
+//    appended to a per-(chunk, query) buffer in global memory. Near-full
+//    buffers are pruned to the query's shortlist depth S, by the owning wave
+//    (queries are wave-private) or, for deep shortlists, by the workgroup.
+//    Thresholds combine the chunk's own S-th best (strict: rows arrive in
+//    ascending order) with a per-query bound shared by all chunks through
+//    global memory (non-strict: the S-th best of any chunk bounds the global
+//    S-th best); the bound is read before the first tile and prefetched
+//    every tile so its latency hides behind the MFMA work.
 //  * Workgroup ids are mapped XCD-first so that the workgroups of one XCD
 //    sweep the same chunk together and share it through that XCD's L2.
-//  * A second kernel merges the sorted per-chunk lists of each query with a
-//    bitonic merge tree in LDS, reranks the S best exactly in FP32 (reference
-//    CPU path order) and writes the top k by (distance, row).
+//  * flat_select_rerank_kernel: per query, the candidates of every chunk that
+//    are within the final shared bound are streamed into an LDS top-S, which
+//    is reranked exactly in FP32 (reference CPU path order) and sorted by
+//    (distance, row); the first k are written.
 #include "fv_flat_mfma.hpp"
 
 namespace fv {
@@ -40,6 +46,7 @@ constexpr int kMfmaThreads = 256;
 constexpr int kQPerWave = 64;
 constexpr int kQPerBlock = 4 * kQPerWave;
 constexpr int kRowsPerTile = 64;
+constexpr uint32_t kSelectBatch = 256;
 
 template <int DT>
 struct MfmaOps;
@@ -69,47 +76,39 @@ struct MfmaOps<FV_DTYPE_BF16> {
     }
 };
 
+// Smallest float above f (finite f): a non-strict bound tested with the same
+// strict compare as the chunk's own threshold.
 __device__ __forceinline__ float next_up(float f) {
-    // Smallest float above f for finite non-negative-or-negative f: lets a
-    // non-strict bound be tested with the same strict compare.
     uint32_t u = __float_as_uint(f);
     if (f == 0.0f) return __uint_as_float(1u);
     return __uint_as_float((u & 0x80000000u) ? u - 1u : u + 1u);
 }
 
-__device__ __forceinline__ void atomic_min_ord(uint32_t* p, float f) {
-    atomicMin(p, ord_f32(f));
+__device__ __forceinline__ float bound_from_ord(uint32_t g) {
+    if (g == 0xFFFFFFFFu) return kAdmit;
+    float f = unord_f32(g);
+    return f < kAdmit ? next_up(f) : kAdmit;
 }
 
-// Per-row epilogue constant and scale of one tile: C init and post-MMA scale.
-template <int METRIC>
-__device__ __forceinline__ void row_terms(const float* __restrict__ norms, const uint32_t* mask,
-                                          uint32_t r, uint32_t r_end, float& cinit,
-                                          float& scale) {
-    bool ok = r < r_end && mask_allows(mask, r);
-    scale = 1.0f;
-    if (!ok) {
-        cinit = __builtin_inff();
-        return;
-    }
-    if (METRIC == METRIC_L2) {
-        cinit = 0.5f * norms[r];
-    } else {
-        cinit = 0.0f;
-        if (METRIC == METRIC_COSINE) {
-            float nx = norms[r];
-            scale = nx > 0.0f ? 1.0f / sqrtf(nx) : 0.0f;
-        }
-    }
+// Shortlist depth of one query: factor * k (or the fixed shortlist), at least
+// k, at most s_max.
+__device__ __forceinline__ void query_k_s(const FlatSearchArgs& a, uint32_t q, uint32_t& k,
+                                          uint32_t& S) {
+    k = a.d_top_k ? a.d_top_k[q] : a.top_k;
+    if (k > a.top_k) k = a.top_k;
+    if (k == 0) k = 1;
+    S = a.factor ? a.factor * k : (a.shortlist ? a.shortlist : 8u * k);
+    if (S < k) S = k;
+    if (S > a.s_max) S = a.s_max;
 }
 
-template <int DT, int METRIC, int KS>
+template <int DT, int METRIC, int KS, int R>
 __global__ __launch_bounds__(kMfmaThreads, 2) void flat_mfma_scan_kernel(
-    const uint16_t* __restrict__ X, uint32_t n, uint32_t row_stride, uint32_t dim,
-    const float* __restrict__ norms, const float* __restrict__ Qm, uint32_t nq,
-    const uint32_t* __restrict__ mask, uint32_t keep, uint32_t cap, uint32_t rows_per_chunk,
-    uint32_t n_chunks, uint32_t qblocks, uint32_t* __restrict__ gthr,
-    uint64_t* __restrict__ cand, uint64_t* __restrict__ partial) {
+    const uint16_t* __restrict__ X, uint32_t row_stride, const float* __restrict__ norms,
+    const uint16_t* __restrict__ Qh, uint32_t nq, FlatSearchArgs a, uint32_t n_chunks,
+    uint32_t qblocks, uint32_t cap, const uint32_t* __restrict__ rows,
+    const uint32_t* __restrict__ nrows, uint32_t n, uint32_t* __restrict__ gthr,
+    uint32_t* __restrict__ counts, uint64_t* __restrict__ cand) {
     using Ops = MfmaOps<DT>;
     using vec = typename Ops::vec;
     constexpr int kRowBytes = KS * 64;          // 32*KS 16-bit elements
@@ -119,8 +118,11 @@ __global__ __launch_bounds__(kMfmaThreads, 2) void flat_mfma_scan_kernel(
     __shared__ __attribute__((aligned(16))) uint8_t s_tile[2][kRowsPerTile * kLdsRow];
     __shared__ __attribute__((aligned(16))) float s_cinit[2][kRowsPerTile];
     __shared__ __attribute__((aligned(16))) float s_scale[2][kRowsPerTile];
+    __shared__ uint32_t s_rowid[2][kRowsPerTile];
     __shared__ uint32_t s_cnt[kQPerBlock];
+    __shared__ uint32_t s_keep[kQPerBlock];
     __shared__ float s_thr[kQPerBlock];
+    __shared__ uint32_t s_need[kQPerBlock / 32];
     extern __shared__ uint64_t s_scratch[];
 
     const int tid = threadIdx.x;
@@ -128,6 +130,7 @@ __global__ __launch_bounds__(kMfmaThreads, 2) void flat_mfma_scan_kernel(
     const int wave = tid / kWave;
     const int l16 = lane & 15;
     const int grp = lane >> 4;
+    const uint32_t dim = a.dim;
 
     // XCD-first work mapping: consecutive block ids land on different XCDs;
     // within an XCD the chunk varies slowest.
@@ -142,74 +145,115 @@ __global__ __launch_bounds__(kMfmaThreads, 2) void flat_mfma_scan_kernel(
         chunk = b / qblocks;
         qblock = b % qblocks;
     }
-    const uint32_t r_begin = chunk * rows_per_chunk;
-    const uint32_t r_end = min(n, r_begin + rows_per_chunk);
-    const uint32_t qw0 = qblock * kQPerBlock + wave * kQPerWave;  // first query of this wave
-    uint64_t* my_scratch = s_scratch + (size_t)wave * cap;
+    // Chunk bounds in the (possibly compacted) row space.
+    const uint32_t n_eff = nrows ? *nrows : n;
+    uint32_t rpc = (n_eff + n_chunks - 1) / n_chunks;
+    rpc = (rpc + kRowsPerTile - 1) / kRowsPerTile * kRowsPerTile;
+    const uint32_t r_begin = min(n_eff, chunk * rpc);
+    const uint32_t r_end = min(n_eff, r_begin + rpc);
+    const uint32_t qb0 = qblock * kQPerBlock;
+    const uint32_t qw0 = qb0 + wave * kQPerWave;
+    constexpr bool kBlockPrune = R == 0;
+    uint64_t* my_scratch = kBlockPrune ? s_scratch : s_scratch + (size_t)wave * cap;
 
     // B fragments: lane (l16, grp) holds query qw0 + 16j + l16, dims
     // 8*KS*grp + 8s + t for step s and t < 8 (the same permutation of K is
     // used for the A fragments, so every dim is paired exactly once).
+    // Queries arrive pre-encoded (negated, 16-bit, rows of 32*KS elements,
+    // padded to whole 256-query blocks), so each fragment is one 128-bit load.
     vec bfrag[4][KS];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        uint32_t qg = qw0 + 16 * j + l16;
+        const uint16_t* qrow = Qh + (size_t)(qw0 + 16 * j + l16) * (32 * KS) + 8 * KS * grp;
 #pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            uint16_t h[8];
-#pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                uint32_t d = 8u * KS * grp + 8u * s + t;
-                float v = (qg < nq && d < dim) ? -Qm[(size_t)qg * dim + d] : 0.0f;
-                h[t] = Ops::from_f32(v);
-            }
-            vec tmp;
-            __builtin_memcpy(&tmp, h, 16);
-            bfrag[j][s] = tmp;
-        }
+        for (int s = 0; s < KS; ++s) bfrag[j][s] = *reinterpret_cast<const vec*>(qrow + 8 * s);
     }
-    for (int i = tid; i < kQPerBlock; i += kMfmaThreads) {
-        s_cnt[i] = 0;
-        s_thr[i] = kAdmit;
+    // Retire the fragment loads here: otherwise the wait-count tracking keeps
+    // them "in flight" around the tile loop and every tile waits for its own
+    // prefetches (vmcnt(0)) before its last MMAs. 0x0F70 = vmcnt(0).
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    // Per-query state; thread t owns query qb0 + t for bookkeeping.
+    const uint32_t my_q = qb0 + tid;
+    const bool my_live = my_q < nq;
+    {
+        uint32_t k, S;
+        if (my_live) query_k_s(a, my_q, k, S);
+        else S = 1;
+        s_keep[tid] = S;
+        s_cnt[tid] = 0;
+        s_thr[tid] = my_live
+            ? bound_from_ord(__hip_atomic_load(&gthr[my_q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            : kAdmit;
     }
 
-    const uint32_t ntiles = r_end > r_begin ? (r_end - r_begin + kRowsPerTile - 1) / kRowsPerTile : 0;
+    const uint32_t ntiles = (r_end - r_begin + kRowsPerTile - 1) / kRowsPerTile;
     const int st_row = tid >> 2;   // staging: 4 threads per row
     const int st_seg = tid & 3;
+    // Staging registers of the next tile. Nothing loaded here is used until
+    // store_tile, after this tile's MMAs, so the loads overlap the compute.
     uint4 pre[KS];
-
-    auto load_tile = [&](uint32_t r0) {
-        uint32_t r = r0 + st_row;
-        const uint16_t* src = X + (size_t)r * row_stride;
+    uint32_t pre_row = 0, pre_mword = 0xFFFFFFFFu;
+    float pre_norm = 0.0f;
+    bool pre_in = false;
+    // Compacted row space: row ids are fetched one tile ahead of the data.
+    // All prefetch loads are unconditional from clamped, valid addresses (rows
+    // past the chunk end get C = +inf, so their data never matters): no
+    // exec-masked loads, so no conservative waits around them.
+    uint32_t rid_next = 0;
+    auto fetch_rid = [&](uint32_t p0) {
+        const uint32_t p = p0 + st_row;
+        rid_next = rows ? rows[p < r_end ? p : r_end - 1] : p;
+    };
+    auto load_tile = [&](uint32_t p0) {
+        const uint32_t p = p0 + st_row;
+        pre_in = p < r_end;
+        pre_row = pre_in ? rid_next : (rows ? rows[r_begin] : r_begin);
+        const uint16_t* src = X + (size_t)pre_row * row_stride;
 #pragma unroll
         for (int i = 0; i < KS; ++i) {
-            uint32_t e = (uint32_t)(st_seg * (kSegBytes / 2) + i * 8);
-            if (r < r_end && e + 8 <= row_stride) pre[i] = *reinterpret_cast<const uint4*>(src + e);
+            const uint32_t e = (uint32_t)(st_seg * (kSegBytes / 2) + i * 8);
+            if (e + 8 <= row_stride) pre[i] = *reinterpret_cast<const uint4*>(src + e);
             else pre[i] = make_uint4(0, 0, 0, 0);
         }
+        if (METRIC != METRIC_INNER_PRODUCT) pre_norm = norms[pre_row];
+        pre_mword = (a.mask && !rows) ? a.mask[pre_row >> 5] : 0xFFFFFFFFu;
     };
-    auto store_tile = [&](int buf, uint32_t r0) {
+    auto store_tile = [&](int buf) {
         uint8_t* dst = &s_tile[buf][st_row * kLdsRow + st_seg * kSegBytes];
 #pragma unroll
         for (int i = 0; i < KS; ++i) *reinterpret_cast<uint4*>(dst + 16 * i) = pre[i];
-        if (tid < kRowsPerTile) {
-            float c, sc;
-            row_terms<METRIC>(norms, mask, r0 + tid, r_end, c, sc);
-            s_cinit[buf][tid] = c;
-            s_scale[buf][tid] = sc;
+        if (st_seg == 0) {
+            const bool ok = pre_in && ((pre_mword >> (pre_row & 31u)) & 1u);
+            float c = __builtin_inff(), sc = 1.0f;
+            if (ok) {
+                if (METRIC == METRIC_L2) c = 0.5f * pre_norm;
+                else c = 0.0f;
+                if (METRIC == METRIC_COSINE) sc = pre_norm > 0.0f ? 1.0f / sqrtf(pre_norm) : 0.0f;
+            }
+            s_cinit[buf][st_row] = c;
+            s_scale[buf][st_row] = sc;
+            s_rowid[buf][st_row] = pre_row;
         }
     };
 
     if (ntiles > 0) {
+        fetch_rid(r_begin);
         load_tile(r_begin);
-        store_tile(0, r_begin);
+        if (ntiles > 1) fetch_rid(r_begin + kRowsPerTile);
+        store_tile(0);
     }
     __syncthreads();
 
     for (uint32_t t = 0; t < ntiles; ++t) {
         const int buf = t & 1;
-        const uint32_t r0 = r_begin + t * kRowsPerTile;
-        if (t + 1 < ntiles) load_tile(r0 + kRowsPerTile);
+        const uint32_t p0 = r_begin + t * kRowsPerTile;
+        if (t + 1 < ntiles) {
+            load_tile(p0 + kRowsPerTile);
+            if (t + 2 < ntiles) fetch_rid(p0 + 2 * kRowsPerTile);
+        }
+        // Prefetch the shared bound; it is consumed after this tile's MMAs.
+        const uint32_t gpre =
+            __hip_atomic_load(&gthr[my_live ? my_q : 0u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
         float thr[4];
 #pragma unroll
@@ -222,213 +266,344 @@ __global__ __launch_bounds__(kMfmaThreads, 2) void flat_mfma_scan_kernel(
 #pragma unroll
             for (int s = 0; s < KS; ++s) afrag[s] = *reinterpret_cast<const vec*>(arow + 16 * s);
             const floatx4 cin = *reinterpret_cast<const floatx4*>(&s_cinit[buf][rt * 16 + 4 * grp]);
-            floatx4 scl;
-            if (METRIC == METRIC_COSINE)
-                scl = *reinterpret_cast<const floatx4*>(&s_scale[buf][rt * 16 + 4 * grp]);
+            floatx4 acc[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[j] = cin;
+            // Four independent accumulation chains, interleaved.
+#pragma unroll
+            for (int s = 0; s < KS; ++s)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[j] = Ops::mma(afrag[s], bfrag[j][s], acc[j]);
+            if (METRIC == METRIC_COSINE) {
+                const floatx4 scl = *reinterpret_cast<const floatx4*>(&s_scale[buf][rt * 16 + 4 * grp]);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[j] = acc[j] * scl;
+            }
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                floatx4 acc = cin;
-#pragma unroll
-                for (int s = 0; s < KS; ++s) acc = Ops::mma(afrag[s], bfrag[j][s], acc);
-                if (METRIC == METRIC_COSINE) acc = acc * scl;
-                float m = fminf(fminf(acc[0], acc[1]), fminf(acc[2], acc[3]));
+                const float m = fminf(fminf(acc[j][0], acc[j][1]), fminf(acc[j][2], acc[j][3]));
                 if (m < thr[j]) {
                     const uint32_t ql = wave * kQPerWave + 16 * j + l16;
-                    const uint32_t qg = qblock * kQPerBlock + ql;
+                    const uint32_t qg = qb0 + ql;
                     if (qg < nq) {
-                        uint64_t* buf_q = cand + ((size_t)chunk * nq + qg) * cap;
+                        uint64_t* bq = cand + ((size_t)chunk * nq + qg) * cap;
 #pragma unroll
                         for (int i = 0; i < 4; ++i) {
-                            if (acc[i] < thr[j]) {
-                                uint32_t pos = atomicAdd(&s_cnt[ql], 1u);
-                                buf_q[pos] = make_key(acc[i], r0 + rt * 16 + 4 * grp + i);
+                            if (acc[j][i] < thr[j]) {
+                                const uint32_t pos = atomicAdd(&s_cnt[ql], 1u);
+                                bq[pos] = make_key(acc[j][i], s_rowid[buf][rt * 16 + 4 * grp + i]);
                             }
                         }
                     }
                 }
             }
         }
-        if (t + 1 < ntiles) store_tile(buf ^ 1, r0 + kRowsPerTile);
+        if (t + 1 < ntiles) store_tile(buf ^ 1);
         __syncthreads();
 
-        // Wave-private prunes of buffers that could overflow on the next tile,
-        // then refresh thresholds from the bound shared across chunks.
-        {
-            const uint32_t ql = wave * kQPerWave + lane;
-            const uint32_t qg = qblock * kQPerBlock + ql;
-            const bool live = qg < nq;
-            const uint32_t c = live ? s_cnt[ql] : 0u;
-            uint64_t need = __ballot(live && c + kRowsPerTile > cap);
+        if (!kBlockPrune) {
+            // Wave-private prunes of buffers that could overflow on the next tile.
+            const uint32_t c = my_live ? s_cnt[tid] : 0u;
+            uint64_t need = __ballot(my_live && c + kRowsPerTile > cap);
             while (need) {
                 const int bl = __builtin_ctzll(need);
                 need &= need - 1;
                 const uint32_t qlb = wave * kQPerWave + bl;
-                const uint32_t qgb = qblock * kQPerBlock + qlb;
-                const uint32_t cb = s_cnt[qlb];
-                uint64_t* buf_q = cand + ((size_t)chunk * nq + qgb) * cap;
-                const uint32_t kept = wave_prune(buf_q, cb, keep, my_scratch, lane);
+                const uint32_t qgb = qb0 + qlb;
+                const uint32_t keep = s_keep[qlb];
+                uint64_t* bq = cand + ((size_t)chunk * nq + qgb) * cap;
+                uint32_t t;
+                uint32_t kept = wave_bisect_prune<(R > 0 ? R : 1)>(bq, s_cnt[qlb], keep, lane, &t);
+                if (kept + kRowsPerTile > cap) {
+                    // Heavy ties at the bound: exact (distance, row) selection.
+                    kept = wave_prune(bq, kept, keep, my_scratch, lane);
+                    t = kept == keep ? (uint32_t)(my_scratch[keep - 1] >> 32) : 0xFFFFFFFFu;
+                }
                 if (lane == 0) {
                     s_cnt[qlb] = kept;
-                    if (kept == keep) {
-                        float th = key_dist(my_scratch[keep - 1]);
-                        s_thr[qlb] = fminf(s_thr[qlb], th);
-                        atomic_min_ord(&gthr[qgb], th);
+                    if (t != 0xFFFFFFFFu) {
+                        s_thr[qlb] = fminf(s_thr[qlb], unord_f32(t));
+                        atomicMin(&gthr[qgb], t);
                     }
                 }
                 wave_lds_sync();
             }
-            if (live) {
-                float g = unord_f32(__atomic_load_n(&gthr[qg], __ATOMIC_RELAXED));
-                if (g < kAdmit) s_thr[ql] = fminf(s_thr[ql], next_up(g));
+        } else {
+            // Workgroup prunes, one query at a time, through shared scratch.
+            if (tid < kQPerBlock / 32) s_need[tid] = 0;
+            __syncthreads();
+            if (my_live && s_cnt[tid] + kRowsPerTile > cap) atomicOr(&s_need[tid >> 5], 1u << (tid & 31));
+            __syncthreads();
+            for (int w = 0; w < kQPerBlock / 32; ++w) {
+                uint32_t bits = s_need[w];
+                while (bits) {
+                    const int bl = __builtin_ctz(bits);
+                    bits &= bits - 1;
+                    const uint32_t ql = w * 32 + bl;
+                    const uint32_t qg = qb0 + ql;
+                    const uint32_t c = s_cnt[ql];
+                    const uint32_t keep = s_keep[ql];
+                    uint64_t* bq = cand + ((size_t)chunk * nq + qg) * cap;
+                    const uint32_t np = next_pow2(c < 2u ? 2u : c);
+                    for (uint32_t i = tid; i < np; i += kMfmaThreads) my_scratch[i] = i < c ? bq[i] : kEmptyKey;
+                    __syncthreads();
+                    block_bitonic_sort(my_scratch, np);
+                    const uint32_t kept = c < keep ? c : keep;
+                    for (uint32_t i = tid; i < kept; i += kMfmaThreads) bq[i] = my_scratch[i];
+                    if (tid == 0) {
+                        s_cnt[ql] = kept;
+                        if (kept == keep) {
+                            const float th = key_dist(my_scratch[keep - 1]);
+                            s_thr[ql] = fminf(s_thr[ql], th);
+                            atomicMin(&gthr[qg], ord_f32(th));
+                        }
+                    }
+                    __syncthreads();
+                }
             }
-            wave_lds_sync();
         }
-    }
-    __syncthreads();
-    // Emit each query's sorted `keep` best of this chunk and publish its bound.
-    for (int qi = 0; qi < kQPerWave; ++qi) {
-        const uint32_t ql = wave * kQPerWave + qi;
-        const uint32_t qg = qblock * kQPerBlock + ql;
-        if (qg >= nq) break;
-        uint64_t* buf_q = cand + ((size_t)chunk * nq + qg) * cap;
-        const uint32_t kept = wave_prune(buf_q, s_cnt[ql], keep, my_scratch, lane);
-        uint64_t* out = partial + ((size_t)chunk * nq + qg) * keep;
-        for (uint32_t i = lane; i < keep; i += kWave) out[i] = i < kept ? my_scratch[i] : kEmptyKey;
-        if (lane == 0 && kept == keep) atomic_min_ord(&gthr[qg], key_dist(my_scratch[keep - 1]));
+        // Tighten thresholds with the prefetched shared bound. Each wave only
+        // reads its own queries' thresholds, so a wave-level sync suffices.
+        if (my_live) s_thr[tid] = fminf(s_thr[tid], bound_from_ord(gpre));
         wave_lds_sync();
     }
+    if (my_live) counts[(size_t)chunk * nq + my_q] = s_cnt[tid];
 }
 
-// One workgroup per query: merge the n_chunks sorted lists (bitonic merge
-// tree in LDS, keeping the smallest `lpow` of every merge), rerank the best
-// `keep` candidates exactly in FP32 and write the top_k by (distance, row).
+// One workgroup per query: streams the candidates of every chunk that are
+// within the final shared bound into an LDS top-S, reranks them exactly in
+// FP32 and writes the top k by (distance, row).
 template <int METRIC>
-__global__ __launch_bounds__(256) void flat_rerank_kernel(
-    const uint64_t* __restrict__ partial, uint32_t n_chunks, uint32_t nq, uint32_t keep,
-    uint32_t lpow, const float* __restrict__ X32, uint32_t dim, const float* __restrict__ Qm,
-    const float* __restrict__ norms, uint32_t top_k, uint32_t out_stride,
+__global__ __launch_bounds__(256) void flat_select_rerank_kernel(
+    const uint64_t* __restrict__ cand, const uint32_t* __restrict__ counts, uint32_t n_chunks,
+    uint32_t nq, uint32_t cap, const uint32_t* __restrict__ gthr, FlatSearchArgs a,
+    uint32_t lcap, const float* __restrict__ X32, const float* __restrict__ Qm,
     GpuSearchResult* __restrict__ out) {
     extern __shared__ uint64_t s_k[];
+    __shared__ uint32_t s_n;
+    __shared__ uint32_t s_g;
     const uint32_t q = blockIdx.x;
-    const uint32_t nl = next_pow2(n_chunks);
-    // Lists padded to lpow entries; missing lists are empty.
-    for (uint32_t i = threadIdx.x; i < nl * lpow; i += blockDim.x) {
-        uint32_t c = i / lpow, j = i % lpow;
-        s_k[i] = (c < n_chunks && j < keep) ? partial[((size_t)c * nq + q) * keep + j] : kEmptyKey;
+    uint32_t k, S;
+    query_k_s(a, q, k, S);
+    if (threadIdx.x == 0) {
+        s_n = 0;
+        s_g = gthr[q];
     }
     __syncthreads();
-    // Merge tree: at each level list 2m absorbs list 2m+1 (both ascending).
-    for (uint32_t width = 1; width < nl; width <<= 1) {
-        const uint32_t pairs = nl / (2 * width);
-        // Half-cleaner on A ++ reverse(B): keep min(A[i], B[lpow-1-i]) in A.
-        for (uint32_t i = threadIdx.x; i < pairs * lpow; i += blockDim.x) {
-            uint32_t p = i / lpow, e = i % lpow;
-            uint64_t* A = s_k + (size_t)(2 * p * width) * lpow;
-            uint64_t* B = s_k + (size_t)((2 * p + 1) * width) * lpow;
-            uint64_t a = A[e], bb = B[lpow - 1 - e];
-            A[e] = a < bb ? a : bb;
-        }
-        __syncthreads();
-        // A now holds the lpow smallest as a bitonic sequence: sort it.
-        for (uint32_t j = lpow >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = threadIdx.x; i < pairs * (lpow >> 1); i += blockDim.x) {
-                uint32_t p = i / (lpow >> 1), e = i % (lpow >> 1);
-                uint64_t* A = s_k + (size_t)(2 * p * width) * lpow;
-                uint32_t lo = ((e / j) * (2 * j)) + (e % j), hi = lo + j;
-                uint64_t a = A[lo], bb = A[hi];
-                if (a > bb) {
-                    A[lo] = bb;
-                    A[hi] = a;
-                }
+    for (uint32_t c = 0; c < n_chunks; ++c) {
+        const uint32_t cnt = counts[(size_t)c * nq + q];
+        const uint64_t* bq = cand + ((size_t)c * nq + q) * cap;
+        for (uint32_t e0 = 0; e0 < cnt; e0 += kSelectBatch) {
+            const uint32_t e = e0 + threadIdx.x;
+            const uint32_t g = s_g;
+            if (e < cnt) {
+                const uint64_t key = bq[e];
+                if ((uint32_t)(key >> 32) <= g) s_k[atomicAdd(&s_n, 1u)] = key;
             }
             __syncthreads();
-        }
-    }
-    // s_k[0..keep) = the keep best approximate candidates, ascending.
-    uint64_t* ex = s_k + lpow;  // reuse: exact keys of the shortlist
-    const uint32_t epow = lpow;
-    const float* qv = Qm + (size_t)q * dim;
-    for (uint32_t i = threadIdx.x; i < epow; i += blockDim.x) {
-        uint64_t k = i < keep ? s_k[i] : kEmptyKey;
-        uint64_t e = kEmptyKey;
-        if (k != kEmptyKey) {
-            uint32_t r = key_slot(k);
-            float d;
-            if (X32) {
-                d = exact_distance<METRIC>(qv, X32 + (size_t)r * dim, dim);
-            } else {
-                // No FP32 rows: report the reduced-precision score as a distance.
-                float s = key_dist(k);
-                if (METRIC == METRIC_L2) {
-                    float qn = 0.0f;
-                    for (uint32_t dd = 0; dd < dim; ++dd) qn = __builtin_fmaf(qv[dd], qv[dd], qn);
-                    d = fmaxf(0.0f, qn + 2.0f * s);
-                } else if (METRIC == METRIC_COSINE) {
-                    float qn = 0.0f;
-                    for (uint32_t dd = 0; dd < dim; ++dd) qn = __builtin_fmaf(qv[dd], qv[dd], qn);
-                    d = 1.0f + s / (sqrtf(qn) + 1e-8f);
-                } else {
-                    d = s;
+            if (s_n + kSelectBatch > lcap) {
+                const uint32_t n = s_n;
+                const uint32_t np = next_pow2(n);
+                for (uint32_t i = n + threadIdx.x; i < np; i += blockDim.x) s_k[i] = kEmptyKey;
+                __syncthreads();
+                block_bitonic_sort(s_k, np);
+                if (threadIdx.x == 0) {
+                    s_n = n < S ? n : S;
+                    if (n >= S) s_g = min(s_g, (uint32_t)(s_k[S - 1] >> 32));
                 }
+                __syncthreads();
             }
-            if (d < kAdmit) e = make_key(d, r);
         }
-        ex[i] = e;
     }
+    const uint32_t n = s_n;
+    const uint32_t np = next_pow2(n < 2u ? 2u : n);
+    for (uint32_t i = n + threadIdx.x; i < np; i += blockDim.x) s_k[i] = kEmptyKey;
     __syncthreads();
-    block_bitonic_sort(ex, epow);
-    for (uint32_t i = threadIdx.x; i < top_k; i += blockDim.x) {
-        uint64_t k = i < epow ? ex[i] : kEmptyKey;
+    block_bitonic_sort(s_k, np);
+    const uint32_t have = n < S ? n : S;
+    const float* qv = Qm + (size_t)q * a.dim;
+    for (uint32_t i = threadIdx.x; i < have; i += blockDim.x) {
+        const uint64_t key = s_k[i];
+        const uint32_t r = key_slot(key);
+        float d;
+        if (X32) {
+            d = exact_distance_vec<METRIC>(qv, X32 + (size_t)r * a.dim, a.dim);
+        } else {
+            // No FP32 rows: report the reduced-precision score as a distance.
+            const float s = key_dist(key);
+            if (METRIC == METRIC_INNER_PRODUCT) {
+                d = s;
+            } else {
+                float qn = 0.0f;
+                for (uint32_t dd = 0; dd < a.dim; ++dd) qn = __builtin_fmaf(qv[dd], qv[dd], qn);
+                d = METRIC == METRIC_L2 ? fmaxf(0.0f, qn + 2.0f * s) : 1.0f + s / (sqrtf(qn) + 1e-8f);
+            }
+        }
+        s_k[i] = d < kAdmit ? make_key(d, r) : kEmptyKey;
+    }
+    const uint32_t hp = next_pow2(have < 2u ? 2u : have);
+    for (uint32_t i = have + threadIdx.x; i < hp; i += blockDim.x) s_k[i] = kEmptyKey;
+    __syncthreads();
+    block_bitonic_sort(s_k, hp);
+    for (uint32_t i = threadIdx.x; i < a.top_k; i += blockDim.x) {
+        const uint64_t key = i < k && i < hp ? s_k[i] : kEmptyKey;
         GpuSearchResult r;
-        if (k == kEmptyKey) {
+        if (key == kEmptyKey) {
             r.id = FV_EMPTY_ID;
             r.distance = FV_EMPTY_DIST;
         } else {
-            r.id = key_slot(k);
-            r.distance = key_dist(k);
+            r.id = key_slot(key);
+            r.distance = key_dist(key);
         }
-        out[(size_t)q * out_stride + i] = r;
+        out[(size_t)q * a.top_k + i] = r;
     }
-    (void)norms;
 }
 
-FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, uint32_t nq, uint32_t top_k,
-                            uint32_t shortlist) {
+// Negated queries as 16-bit codes, rows of `pdq` elements, zero padded.
+template <int DT>
+__global__ void encode_queries_kernel(const float* __restrict__ Qm, uint32_t nq, uint32_t dim,
+                                      uint32_t pdq, uint32_t total_rows, uint16_t* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)total_rows * pdq) return;
+    const uint32_t q = (uint32_t)(i / pdq), d = (uint32_t)(i % pdq);
+    const float v = (q < nq && d < dim) ? -Qm[(size_t)q * dim + d] : 0.0f;
+    out[i] = MfmaOps<DT>::from_f32(v);
+}
+
+// ---- mask compaction: ascending list of the rows whose bit is set ---------
+
+__global__ void mask_block_counts_kernel(const uint32_t* __restrict__ mask, uint32_t nwords,
+                                         uint32_t n, uint32_t* __restrict__ block_counts) {
+    __shared__ uint32_t s;
+    if (threadIdx.x == 0) s = 0;
+    __syncthreads();
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t c = 0;
+    if (w < nwords) {
+        uint32_t bits = mask[w];
+        if ((w + 1) * 32u > n) bits &= (n - w * 32u) >= 32u ? 0xFFFFFFFFu : ((1u << (n - w * 32u)) - 1u);
+        c = __popc(bits);
+    }
+    atomicAdd(&s, c);
+    __syncthreads();
+    if (threadIdx.x == 0) block_counts[blockIdx.x] = s;
+}
+
+__global__ void mask_scan_kernel(uint32_t* __restrict__ block_counts, uint32_t nblocks,
+                                 uint32_t* __restrict__ total) {
+    // One workgroup: exclusive scan of the per-block counts, in place.
+    __shared__ uint32_t s_part[256];
+    const uint32_t per = (nblocks + 255) / 256;
+    const uint32_t b0 = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (uint32_t i = 0; i < per && b0 + i < nblocks; ++i) sum += block_counts[b0 + i];
+    s_part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int i = 0; i < 256; ++i) {
+            const uint32_t v = s_part[i];
+            s_part[i] = run;
+            run += v;
+        }
+        *total = run;
+    }
+    __syncthreads();
+    uint32_t run = s_part[threadIdx.x];
+    for (uint32_t i = 0; i < per && b0 + i < nblocks; ++i) {
+        const uint32_t v = block_counts[b0 + i];
+        block_counts[b0 + i] = run;
+        run += v;
+    }
+}
+
+__global__ void mask_scatter_kernel(const uint32_t* __restrict__ mask, uint32_t nwords,
+                                    uint32_t n, const uint32_t* __restrict__ block_offsets,
+                                    uint32_t* __restrict__ rows) {
+    __shared__ uint32_t s_c[256];
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t bits = 0;
+    if (w < nwords) {
+        bits = mask[w];
+        if ((w + 1) * 32u > n) bits &= (n - w * 32u) >= 32u ? 0xFFFFFFFFu : ((1u << (n - w * 32u)) - 1u);
+    }
+    s_c[threadIdx.x] = __popc(bits);
+    __syncthreads();
+    // Inclusive Hillis-Steele scan within the block.
+    for (uint32_t off = 1; off < 256; off <<= 1) {
+        const uint32_t v = threadIdx.x >= off ? s_c[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s_c[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t pos = block_offsets[blockIdx.x] + s_c[threadIdx.x] - __popc(bits);
+    while (bits) {
+        const int b = __builtin_ctz(bits);
+        bits &= bits - 1;
+        rows[pos++] = w * 32u + (uint32_t)b;
+    }
+}
+
+// ---- planning and launch -----------------------------------------------------
+
+static uint32_t deepest_shortlist(const FlatSearchArgs& a) {
+    uint32_t smax = a.factor ? a.factor * a.top_k : (a.shortlist ? a.shortlist : 8u * a.top_k);
+    if (smax < a.top_k) smax = a.top_k;
+    if (smax > FV_MAX_SHORTLIST) smax = FV_MAX_SHORTLIST;
+    return smax;
+}
+
+FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, uint32_t nq, const FlatSearchArgs& a,
+                            bool has_mask, uint32_t valid_count) {
     FlatMfmaPlan p{};
-    uint32_t S = shortlist ? shortlist : 8u * top_k;
-    if (S < top_k) S = top_k;
-    if (S > FV_MAX_SHORTLIST) S = FV_MAX_SHORTLIST;
-    if (S > n) S = n < top_k ? top_k : n;
-    p.keep = S;
-    p.lpow = next_pow2(S < 2u ? 2u : S);
-    p.cap = next_pow2(S + kRowsPerTile);
-    if (p.cap < 256u) p.cap = 256u;
-    p.ks = (padded_dim + 31) / 32;
+    const uint32_t smax = deepest_shortlist(a);
+    p.s_max = smax;
+    // Buffers hold S plus one tile of appends plus one wavefront of bisection slack.
+    if (smax + 2 * kRowsPerTile <= 256u) {
+        p.cap = 256u;
+        p.prune_regs = 4;
+    } else if (smax + 2 * kRowsPerTile <= 1024u) {
+        p.cap = 1024u;
+        p.prune_regs = 16;
+    } else {
+        p.cap = next_pow2(smax + 2 * kRowsPerTile);
+        p.prune_regs = 0;  // workgroup prunes through shared scratch
+    }
+    p.block_prune = p.prune_regs == 0;
+    p.lds_scan = (size_t)(p.block_prune ? 1 : 4) * p.cap * 8;
+    p.lcap = next_pow2(smax + kSelectBatch);
+    const uint32_t ks = (padded_dim + 31) / 32;
+    p.ks = ks <= 4 ? ks : (ks <= 6 ? 6 : 8);
     p.qblocks = (nq + kQPerBlock - 1) / kQPerBlock;
-    // Chunks: enough workgroups to fill 256 CUs about five times over, rows
-    // per chunk >= 2048, the merge tree within 64 KB of LDS, and a multiple
-    // of 8 (one slice per XCD) when there are at least 8.
-    uint32_t max_by_merge = 8192u / p.lpow;
-    if (max_by_merge < 1u) max_by_merge = 1u;
-    uint32_t max_by_rows = (n + 2047u) / 2048u;
-    uint32_t want = (2560u + p.qblocks - 1) / p.qblocks;
-    uint32_t c = want;
-    if (c > max_by_rows) c = max_by_rows;
-    if (c > max_by_merge) c = max_by_merge;
+    p.compact = has_mask && valid_count > 0 && (uint64_t)valid_count * 2 <= n;
+    const uint32_t rows_plan = p.compact ? valid_count : n;
+    // Chunks: enough workgroups to cover 256 CUs several times, >= 2048 rows
+    // each, a multiple of 8 (one slice per XCD) when there are at least 8.
+    const uint32_t want = (2560u + p.qblocks - 1) / p.qblocks;
+    const uint32_t max_by_rows = (rows_plan + 2047u) / 2048u;
+    uint32_t c = want < max_by_rows ? want : max_by_rows;
+    if (c > 256u) c = 256u;
     if (c >= 8u) c &= ~7u;
     if (c < 1u) c = 1u;
-    uint32_t rows = (n + c - 1) / c;
-    rows = (rows + kRowsPerTile - 1) / kRowsPerTile * kRowsPerTile;
-    p.rows_per_chunk = rows;
-    p.n_chunks = (n + rows - 1) / rows;
+    p.n_chunks = c;
     size_t off = 0;
-    p.off_gthr = off;
-    off += ((size_t)nq * 4 + 255) / 256 * 256;
-    p.off_cand = off;
-    off += (size_t)p.n_chunks * nq * p.cap * 8;
-    p.off_partial = off;
-    off += (size_t)p.n_chunks * nq * p.keep * 8;
-    p.workspace_bytes = (off + 255) / 256 * 256;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) / 256 * 256;
+        return o;
+    };
+    p.off_gthr = take((size_t)nq * 4);
+    p.off_counts = take((size_t)p.n_chunks * nq * 4);
+    p.off_cand = take((size_t)p.n_chunks * nq * p.cap * 8);
+    p.off_qh = take((size_t)p.qblocks * kQPerBlock * 32 * p.ks * 2);
+    if (p.compact) {
+        const uint32_t nwords = (n + 31) / 32;
+        p.mask_blocks = (nwords + 255) / 256;
+        p.off_rows = take((size_t)n * 4);
+        p.off_nrows = take(4);
+        p.off_blocks = take((size_t)p.mask_blocks * 4);
+    }
+    p.workspace_bytes = off;
     return p;
 }
 
@@ -437,104 +612,169 @@ __global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {
     if (i < n) p[i] = v;
 }
 
+struct ScanLaunch {
+    const FlatMfmaPlan* p;
+    const uint16_t* X;
+    uint32_t stride;
+    const float* norms;
+    const uint16_t* Qh;
+    uint32_t nq;
+    FlatSearchArgs a;
+    const uint32_t* rows;
+    const uint32_t* nrows;
+    uint32_t n;
+    uint32_t* gthr;
+    uint32_t* counts;
+    uint64_t* cand;
+    hipStream_t stream;
+};
+
+template <int DT, int METRIC, int KS, int R>
+static void launch_scan_r(const ScanLaunch& L) {
+    const FlatMfmaPlan& p = *L.p;
+    const dim3 g(p.qblocks * p.n_chunks);
+    flat_mfma_scan_kernel<DT, METRIC, KS, R><<<g, kMfmaThreads, p.lds_scan, L.stream>>>(
+        L.X, L.stride, L.norms, L.Qh, L.nq, L.a, p.n_chunks, p.qblocks, p.cap, L.rows, L.nrows,
+        L.n, L.gthr, L.counts, L.cand);
+}
+
 template <int DT, int METRIC, int KS>
-static void launch_scan(const FlatMfmaPlan& p, const uint16_t* X, uint32_t n, uint32_t stride,
-                        uint32_t dim, const float* norms, const float* Qm, uint32_t nq,
-                        const uint32_t* mask, uint32_t* gthr, uint64_t* cand, uint64_t* partial,
-                        hipStream_t stream) {
-    size_t lds = (size_t)4 * p.cap * sizeof(uint64_t);
-    flat_mfma_scan_kernel<DT, METRIC, KS><<<p.qblocks * p.n_chunks, kMfmaThreads, lds, stream>>>(
-        X, n, stride, dim, norms, Qm, nq, mask, p.keep, p.cap, p.rows_per_chunk, p.n_chunks,
-        p.qblocks, gthr, cand, partial);
+static void launch_scan(const ScanLaunch& L) {
+    if (L.p->prune_regs == 4) launch_scan_r<DT, METRIC, KS, 4>(L);
+    else if (L.p->prune_regs == 16) launch_scan_r<DT, METRIC, KS, 16>(L);
+    else launch_scan_r<DT, METRIC, KS, 0>(L);
 }
 
 template <int DT, int METRIC>
-static hipError_t dispatch_ks(const FlatMfmaPlan& p, const uint16_t* X, uint32_t n,
-                              uint32_t stride, uint32_t dim, const float* norms, const float* Qm,
-                              uint32_t nq, const uint32_t* mask, uint32_t* gthr, uint64_t* cand,
-                              uint64_t* partial, hipStream_t stream) {
-    switch (p.ks) {
-        case 1: launch_scan<DT, METRIC, 1>(p, X, n, stride, dim, norms, Qm, nq, mask, gthr, cand, partial, stream); break;
-        case 2: launch_scan<DT, METRIC, 2>(p, X, n, stride, dim, norms, Qm, nq, mask, gthr, cand, partial, stream); break;
-        case 3: launch_scan<DT, METRIC, 3>(p, X, n, stride, dim, norms, Qm, nq, mask, gthr, cand, partial, stream); break;
-        case 4: launch_scan<DT, METRIC, 4>(p, X, n, stride, dim, norms, Qm, nq, mask, gthr, cand, partial, stream); break;
-        case 5: launch_scan<DT, METRIC, 5>(p, X, n, stride, dim, norms, Qm, nq, mask, gthr, cand, partial, stream); break;
-        case 6: launch_scan<DT, METRIC, 6>(p, X, n, stride, dim, norms, Qm, nq, mask, gthr, cand, partial, stream); break;
-        case 7: launch_scan<DT, METRIC, 7>(p, X, n, stride, dim, norms, Qm, nq, mask, gthr, cand, partial, stream); break;
-        case 8: launch_scan<DT, METRIC, 8>(p, X, n, stride, dim, norms, Qm, nq, mask, gthr, cand, partial, stream); break;
+static hipError_t dispatch_ks(const ScanLaunch& L) {
+    switch (L.p->ks) {
+        case 1: launch_scan<DT, METRIC, 1>(L); break;
+        case 2: launch_scan<DT, METRIC, 2>(L); break;
+        case 3: launch_scan<DT, METRIC, 3>(L); break;
+        case 4: launch_scan<DT, METRIC, 4>(L); break;
+        case 6: launch_scan<DT, METRIC, 6>(L); break;
+        case 8: launch_scan<DT, METRIC, 8>(L); break;
         default: return hipErrorInvalidValue;
     }
     return hipSuccess;
 }
 
-hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq, uint32_t top_k,
-                            int metric, const uint32_t* mask, GpuSearchResult* out,
-                            uint32_t out_stride, void* workspace, const FlatMfmaPlan& p,
-                            hipStream_t stream) {
+template <int DT>
+static hipError_t dispatch_metric(const ScanLaunch& L, int metric) {
+    if (metric == METRIC_L2) return dispatch_ks<DT, METRIC_L2>(L);
+    if (metric == METRIC_INNER_PRODUCT) return dispatch_ks<DT, METRIC_INNER_PRODUCT>(L);
+    return dispatch_ks<DT, METRIC_COSINE>(L);
+}
+
+hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
+                            const FlatSearchArgs& a, GpuSearchResult* out, void* workspace,
+                            const FlatMfmaPlan& p, hipStream_t stream) {
     char* ws = static_cast<char*>(workspace);
     uint32_t* gthr = reinterpret_cast<uint32_t*>(ws + p.off_gthr);
+    uint32_t* counts = reinterpret_cast<uint32_t*>(ws + p.off_counts);
     uint64_t* cand = reinterpret_cast<uint64_t*>(ws + p.off_cand);
-    uint64_t* partial = reinterpret_cast<uint64_t*>(ws + p.off_partial);
-    const uint16_t* X = static_cast<const uint16_t*>(t->d_codes);
+    const uint32_t* rows = nullptr;
+    const uint32_t* nrows = nullptr;
     fill_u32_kernel<<<(nq + 255) / 256, 256, 0, stream>>>(gthr, nq, 0xFFFFFFFFu);
+    if (p.compact) {
+        uint32_t* r = reinterpret_cast<uint32_t*>(ws + p.off_rows);
+        uint32_t* nr = reinterpret_cast<uint32_t*>(ws + p.off_nrows);
+        uint32_t* blk = reinterpret_cast<uint32_t*>(ws + p.off_blocks);
+        const uint32_t nwords = (t->num_vectors + 31) / 32;
+        mask_block_counts_kernel<<<p.mask_blocks, 256, 0, stream>>>(a.mask, nwords, t->num_vectors, blk);
+        mask_scan_kernel<<<1, 256, 0, stream>>>(blk, p.mask_blocks, nr);
+        mask_scatter_kernel<<<p.mask_blocks, 256, 0, stream>>>(a.mask, nwords, t->num_vectors, blk, r);
+        rows = r;
+        nrows = nr;
+    }
+    uint16_t* Qh = reinterpret_cast<uint16_t*>(ws + p.off_qh);
+    {
+        const uint32_t qrows = p.qblocks * kQPerBlock, pdq = 32 * p.ks;
+        const size_t total = (size_t)qrows * pdq;
+        const unsigned blocks = (unsigned)((total + 255) / 256);
+        if (t->dtype == FV_DTYPE_F16)
+            encode_queries_kernel<FV_DTYPE_F16><<<blocks, 256, 0, stream>>>(Qm, nq, t->dim, pdq, qrows, Qh);
+        else
+            encode_queries_kernel<FV_DTYPE_BF16><<<blocks, 256, 0, stream>>>(Qm, nq, t->dim, pdq, qrows, Qh);
+    }
+    ScanLaunch L{&p, static_cast<const uint16_t*>(t->d_codes), t->padded_dim, t->d_norms_sq, Qh, nq,
+                 a, rows, nrows, t->num_vectors, gthr, counts, cand, stream};
     void* timer = timer_begin(stream);
     hipError_t e = hipErrorInvalidValue;
-    const uint32_t n = t->num_vectors, st = t->padded_dim, dim = t->dim;
-    const float* nr = t->d_norms_sq;
-    if (t->dtype == FV_DTYPE_F16) {
-        if (metric == METRIC_L2) e = dispatch_ks<FV_DTYPE_F16, METRIC_L2>(p, X, n, st, dim, nr, Qm, nq, mask, gthr, cand, partial, stream);
-        else if (metric == METRIC_INNER_PRODUCT) e = dispatch_ks<FV_DTYPE_F16, METRIC_INNER_PRODUCT>(p, X, n, st, dim, nr, Qm, nq, mask, gthr, cand, partial, stream);
-        else e = dispatch_ks<FV_DTYPE_F16, METRIC_COSINE>(p, X, n, st, dim, nr, Qm, nq, mask, gthr, cand, partial, stream);
-    } else if (t->dtype == FV_DTYPE_BF16) {
-        if (metric == METRIC_L2) e = dispatch_ks<FV_DTYPE_BF16, METRIC_L2>(p, X, n, st, dim, nr, Qm, nq, mask, gthr, cand, partial, stream);
-        else if (metric == METRIC_INNER_PRODUCT) e = dispatch_ks<FV_DTYPE_BF16, METRIC_INNER_PRODUCT>(p, X, n, st, dim, nr, Qm, nq, mask, gthr, cand, partial, stream);
-        else e = dispatch_ks<FV_DTYPE_BF16, METRIC_COSINE>(p, X, n, st, dim, nr, Qm, nq, mask, gthr, cand, partial, stream);
-    }
+    if (t->dtype == FV_DTYPE_F16) e = dispatch_metric<FV_DTYPE_F16>(L, a.metric);
+    else if (t->dtype == FV_DTYPE_BF16) e = dispatch_metric<FV_DTYPE_BF16>(L, a.metric);
     timer_end(timer, stream);
     if (e != hipSuccess) return e;
-    // The merge tree needs next_pow2(n_chunks) lists; the rerank reuses list 1.
-    const uint32_t nl = next_pow2(p.n_chunks) < 2u ? 2u : next_pow2(p.n_chunks);
-    const size_t lds = (size_t)nl * p.lpow * sizeof(uint64_t);
-    if (metric == METRIC_L2)
-        flat_rerank_kernel<METRIC_L2><<<nq, 256, lds, stream>>>(partial, p.n_chunks, nq, p.keep, p.lpow, t->d_vectors, dim, Qm, nr, top_k, out_stride, out);
-    else if (metric == METRIC_INNER_PRODUCT)
-        flat_rerank_kernel<METRIC_INNER_PRODUCT><<<nq, 256, lds, stream>>>(partial, p.n_chunks, nq, p.keep, p.lpow, t->d_vectors, dim, Qm, nr, top_k, out_stride, out);
+    const size_t lds = (size_t)p.lcap * sizeof(uint64_t);
+    if (a.metric == METRIC_L2)
+        flat_select_rerank_kernel<METRIC_L2><<<nq, 256, lds, stream>>>(cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.lcap, t->d_vectors, Qm, out);
+    else if (a.metric == METRIC_INNER_PRODUCT)
+        flat_select_rerank_kernel<METRIC_INNER_PRODUCT><<<nq, 256, lds, stream>>>(cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.lcap, t->d_vectors, Qm, out);
     else
-        flat_rerank_kernel<METRIC_COSINE><<<nq, 256, lds, stream>>>(partial, p.n_chunks, nq, p.keep, p.lpow, t->d_vectors, dim, Qm, nr, top_k, out_stride, out);
+        flat_select_rerank_kernel<METRIC_COSINE><<<nq, 256, lds, stream>>>(cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.lcap, t->d_vectors, Qm, out);
     return hipGetLastError();
+}
+
+static int check_tables(const FvFlatTables* t) {
+    if (!t->d_codes || !t->d_norms_sq) return -1;
+    if (t->padded_dim == 0 || t->padded_dim % 16 || t->padded_dim > 256 || t->dim == 0 ||
+        t->dim > t->padded_dim)
+        return (int)hipErrorInvalidValue;
+    if (t->dtype != FV_DTYPE_F16 && t->dtype != FV_DTYPE_BF16) return (int)hipErrorInvalidValue;
+    return 0;
+}
+
+static bool make_args(const FvFlatTables* t, const FvFlatSearchParams* sp, FlatSearchArgs& a) {
+    a = FlatSearchArgs{};
+    a.mask = sp->d_valid_mask;
+    a.d_top_k = sp->d_top_k;
+    a.top_k = sp->top_k;
+    a.factor = sp->shortlist_factor;
+    a.shortlist = sp->shortlist;
+    a.metric = (int)sp->metric;
+    a.dim = t->dim;
+    if (a.top_k == 0 || a.top_k > FV_MAX_TOPK) return false;
+    if (sp->metric > METRIC_INNER_PRODUCT || sp->reserved != 0) return false;
+    a.s_max = deepest_shortlist(a);
+    return true;
 }
 
 }  // namespace fv
 
 extern "C" {
 
-size_t fv_flat_workspace_size(const FvFlatTables* tables, uint32_t num_queries, uint32_t top_k,
-                              uint32_t shortlist) {
-    if (!tables || top_k == 0 || top_k > FV_MAX_TOPK || tables->num_vectors == 0) return 0;
-    return fv::plan_flat_mfma(tables->num_vectors, tables->padded_dim, num_queries, top_k,
-                              shortlist).workspace_bytes;
+size_t fv_flat_workspace_size(const FvFlatTables* tables, uint32_t num_queries,
+                              const FvFlatSearchParams* params) {
+    if (!tables || !params || tables->num_vectors == 0) return 0;
+    fv::FlatSearchArgs a;
+    if (!fv::make_args(tables, params, a)) return 0;
+    fv::FlatMfmaPlan p = fv::plan_flat_mfma(tables->num_vectors, tables->padded_dim, num_queries,
+                                            a, params->d_valid_mask != nullptr,
+                                            params->valid_count);
+    if (p.lcap > 8192u) return 0;
+    return p.workspace_bytes;
 }
 
 int fv_flat_search(const FvFlatTables* tables, const float* d_queries, uint32_t num_queries,
-                   uint32_t top_k, uint32_t shortlist, MetricType metric,
-                   const uint32_t* d_valid_mask, GpuSearchResult* d_out_results,
+                   const FvFlatSearchParams* params, GpuSearchResult* d_out_results,
                    void* d_workspace, size_t workspace_bytes, void* stream) {
-    if (!tables || !d_queries || !d_out_results || !tables->d_codes || !tables->d_norms_sq) return -1;
-    if (top_k > FV_MAX_TOPK || tables->padded_dim == 0 || tables->padded_dim % 16 ||
-        tables->padded_dim > 256 || tables->dim > tables->padded_dim)
-        return (int)hipErrorInvalidValue;
-    if (tables->dtype != FV_DTYPE_F16 && tables->dtype != FV_DTYPE_BF16)
-        return (int)hipErrorInvalidValue;
-    if (num_queries == 0 || top_k == 0) return 0;
+    if (!tables || !d_queries || !d_out_results || !params) return -1;
+    const int tc = fv::check_tables(tables);
+    if (tc) return tc;
+    fv::FlatSearchArgs a;
+    if (!fv::make_args(tables, params, a)) return (int)hipErrorInvalidValue;
+    if (num_queries == 0) return 0;
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (tables->num_vectors == 0)
-        return (int)fv::fill_results(d_out_results, (size_t)num_queries * top_k, FV_EMPTY_ID,
+        return (int)fv::fill_results(d_out_results, (size_t)num_queries * a.top_k, FV_EMPTY_ID,
                                      0x7149f2cau /* 1e30f */, s);
     fv::FlatMfmaPlan p = fv::plan_flat_mfma(tables->num_vectors, tables->padded_dim, num_queries,
-                                            top_k, shortlist);
+                                            a, params->d_valid_mask != nullptr,
+                                            params->valid_count);
+    if (p.lcap > 8192u) return (int)hipErrorInvalidValue;  // shortlist deeper than 7936
     if (!d_workspace || workspace_bytes < p.workspace_bytes) return (int)hipErrorInvalidValue;
-    hipError_t e = fv::launch_flat_mfma(tables, d_queries, num_queries, top_k, (int)metric,
-                                        d_valid_mask, d_out_results, top_k, d_workspace, p, s);
+    hipError_t e = fv::launch_flat_mfma(tables, d_queries, num_queries, a, d_out_results,
+                                        d_workspace, p, s);
     return e == hipSuccess ? 0 : (int)e;
 }
 
@@ -555,15 +795,21 @@ int cuda_fp16_tc_search_batch(const Fp16GpuTables* tables, const float* d_querie
     t.dim = tables->dim;
     t.padded_dim = tables->padded_dim;
     t.dtype = FV_DTYPE_F16;
-    if (!t.d_codes || !t.d_norms_sq) return -1;
-    if (t.padded_dim % 16 || t.padded_dim > 256) return (int)hipErrorInvalidValue;
+    const int tc = fv::check_tables(&t);
+    if (tc) return tc;
+    FvFlatSearchParams sp{};
+    sp.d_valid_mask = d_valid_mask;
+    sp.metric = (uint32_t)metric;
+    sp.top_k = top_k;
+    fv::FlatSearchArgs a;
+    if (!fv::make_args(&t, &sp, a)) return (int)hipErrorInvalidValue;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    fv::FlatMfmaPlan p = fv::plan_flat_mfma(t.num_vectors, t.padded_dim, num_queries, top_k, 0);
+    fv::FlatMfmaPlan p = fv::plan_flat_mfma(t.num_vectors, t.padded_dim, num_queries, a,
+                                            d_valid_mask != nullptr, 0);
     void* ws = nullptr;
     hipError_t e = hipMallocAsync(&ws, p.workspace_bytes, s);
     if (e != hipSuccess) return (int)e;
-    e = fv::launch_flat_mfma(&t, d_queries, num_queries, top_k, (int)metric, d_valid_mask,
-                             d_out_results, top_k, ws, p, s);
+    e = fv::launch_flat_mfma(&t, d_queries, num_queries, a, d_out_results, ws, p, s);
     hipError_t e2 = hipFreeAsync(ws, s);
     if (e != hipSuccess) return (int)e;
     return e2 == hipSuccess ? 0 : (int)e2;

@@ -5,24 +5,39 @@
 
 namespace fv {
 
+// Per-call arguments shared by the scan and the select/rerank kernels.
+struct FlatSearchArgs {
+    const uint32_t* mask;     // validity words or NULL
+    const uint32_t* d_top_k;  // per-query k or NULL
+    uint32_t top_k;           // uniform k and output row stride
+    uint32_t factor;          // shortlist = factor * k (0: fixed shortlist)
+    uint32_t shortlist;       // fixed shortlist (0: 8 * k)
+    uint32_t s_max;           // deepest shortlist of the call
+    uint32_t dim;
+    int metric;
+};
+
 struct FlatMfmaPlan {
-    uint32_t keep;            // shortlist depth S kept per (chunk, query)
-    uint32_t lpow;            // next power of two >= keep
+    uint32_t s_max;           // deepest per-query shortlist
     uint32_t cap;             // candidate buffer capacity per (chunk, query)
+    uint32_t prune_regs;      // keys per lane of a wave prune (4 or 16), 0 = workgroup prunes
+    bool block_prune;         // prune with the whole workgroup (deep shortlists)
+    size_t lds_scan;          // dynamic LDS of the scan kernel
+    uint32_t lcap;            // LDS keys of the select/rerank kernel
     uint32_t ks;              // 32-wide K steps per row
     uint32_t qblocks;         // 256-query blocks
-    uint32_t rows_per_chunk;  // multiple of the 64-row tile
     uint32_t n_chunks;
-    size_t off_gthr, off_cand, off_partial;
+    bool compact;             // scan the compacted list of valid rows
+    uint32_t mask_blocks;
+    size_t off_gthr, off_counts, off_cand, off_qh, off_rows, off_nrows, off_blocks;
     size_t workspace_bytes;
 };
 
-FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, uint32_t nq, uint32_t top_k,
-                            uint32_t shortlist);
+FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, uint32_t nq, const FlatSearchArgs& a,
+                            bool has_mask, uint32_t valid_count);
 
-hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq, uint32_t top_k,
-                            int metric, const uint32_t* mask, GpuSearchResult* out,
-                            uint32_t out_stride, void* workspace, const FlatMfmaPlan& p,
-                            hipStream_t stream);
+hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
+                            const FlatSearchArgs& a, GpuSearchResult* out, void* workspace,
+                            const FlatMfmaPlan& p, hipStream_t stream);
 
 }  // namespace fv

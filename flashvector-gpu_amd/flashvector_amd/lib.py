@@ -68,6 +68,21 @@ class FvFlatTables(ctypes.Structure):
     ]
 
 
+class FvFlatSearchParams(ctypes.Structure):
+    """include/flashvector/fv_ext.h FvFlatSearchParams."""
+
+    _fields_ = [
+        ("d_valid_mask", ctypes.c_void_p),
+        ("d_top_k", ctypes.c_void_p),
+        ("valid_count", ctypes.c_uint32),
+        ("metric", ctypes.c_uint32),
+        ("top_k", ctypes.c_uint32),
+        ("shortlist_factor", ctypes.c_uint32),
+        ("shortlist", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
 DTYPES = {"f32": 0, "f16": 1, "bf16": 2, "fp8": 3}
 
 P, U, I, SZ = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_size_t
@@ -100,8 +115,8 @@ SIGNATURES = {
     "fv_flat_exact_search": (I, [P, U, U, P, U, U, I, P, P, P, SZ, P]),
     "cuda_fp16_tc_search_batch": (I, [P, P, U, U, I, P, P, P]),
     "fv_encode_flat": (I, [P, U, U, U, U, P, P, P, P, P]),
-    "fv_flat_workspace_size": (SZ, [P, U, U, U]),
-    "fv_flat_search": (I, [P, P, U, U, U, I, P, P, P, SZ, P]),
+    "fv_flat_workspace_size": (SZ, [P, U, P]),
+    "fv_flat_search": (I, [P, P, U, P, P, P, SZ, P]),
     "cuda_ivf_pq_search_batch": (I, [P, P, U, U, U, U, I, P, P, P]),
     "fv_ivf_pq_list_order_codes": (I, [P, P, P]),
     "fv_ivf_pq_workspace_size": (SZ, [P, U, U, U, U]),

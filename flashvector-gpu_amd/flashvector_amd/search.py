@@ -122,20 +122,33 @@ class FlatIndex:
                                      n, dim, pd, L.DTYPES[dtype])
         self._ws = None
 
-    def workspace_size(self, nq, top_k, shortlist=0):
-        return L.load().fv_flat_workspace_size(ctypes.byref(self.tables), nq, top_k, shortlist)
+    def params(self, top_k, metric="l2", valid_mask=None, valid_count=0, shortlist=0,
+               shortlist_factor=0, top_k_per_query=None):
+        return L.FvFlatSearchParams(L.ptr(valid_mask), L.ptr(top_k_per_query), int(valid_count),
+                                    _metric(metric), int(top_k), int(shortlist_factor),
+                                    int(shortlist), 0)
 
-    def search(self, queries, top_k, metric="l2", valid_mask=None, shortlist=0, out=None):
+    def workspace_size(self, nq, params):
+        return L.load().fv_flat_workspace_size(ctypes.byref(self.tables), nq, ctypes.byref(params))
+
+    def search(self, queries, top_k, metric="l2", valid_mask=None, shortlist=0, out=None,
+               valid_count=0, shortlist_factor=0, top_k_per_query=None):
+        """fv_flat_search. `valid_count` (set bits of the mask, if known) lets
+        sparse masks be compacted; `top_k_per_query` is an optional int32
+        device tensor of per-query k (each <= top_k)."""
         torch = L._torch()
         lib = L.load()
         nq = queries.shape[0]
-        need = self.workspace_size(nq, top_k, shortlist)
+        prm = self.params(top_k, metric, valid_mask, valid_count, shortlist, shortlist_factor,
+                          top_k_per_query)
+        need = self.workspace_size(nq, prm)
+        if need == 0:
+            raise L.FvError("invalid flat search shape")
         if self._ws is None or self._ws.numel() < need:
-            self._ws = torch.empty(max(need, 1), dtype=torch.uint8, device=queries.device)
+            self._ws = torch.empty(need, dtype=torch.uint8, device=queries.device)
         out = L.result_tensor(nq, top_k, queries.device) if out is None else out
-        rc = lib.fv_flat_search(ctypes.byref(self.tables), L.ptr(queries), nq, top_k, shortlist,
-                                _metric(metric), L.ptr(valid_mask), L.ptr(out), L.ptr(self._ws),
-                                self._ws.numel(), L.stream_handle())
+        rc = lib.fv_flat_search(ctypes.byref(self.tables), L.ptr(queries), nq, ctypes.byref(prm),
+                                L.ptr(out), L.ptr(self._ws), self._ws.numel(), L.stream_handle())
         L.check(rc, "fv_flat_search")
         return out
 

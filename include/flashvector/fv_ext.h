@@ -64,11 +64,26 @@ int fv_encode_flat(const float* d_vectors, uint32_t num_vectors, uint32_t dim,
                    float* d_norms_out, double* d_norms_f64_out, float* d_row_scales_out,
                    void* stream);
 
-size_t fv_flat_workspace_size(const FvFlatTables* tables, uint32_t num_queries, uint32_t top_k,
-                              uint32_t shortlist);
+/* Search parameters of the flat scans. Shortlist of query i: factor * k_i
+ * when shortlist_factor != 0, else `shortlist` (0 means 8 * k_i), never below
+ * k_i; at most 7936 rows. valid_count is the number of set bits in the mask
+ * when the caller knows it (0 = unknown): a mask keeping at most half of the
+ * rows is then compacted on the device and only the valid rows are scored. */
+typedef struct {
+    const uint32_t* d_valid_mask; /* LSB-first u32 words, NULL = every row */
+    const uint32_t* d_top_k;      /* per-query k (each <= top_k), or NULL */
+    uint32_t valid_count;
+    uint32_t metric;              /* MetricType */
+    uint32_t top_k;               /* uniform k and output row stride, <= 1024 */
+    uint32_t shortlist_factor;
+    uint32_t shortlist;
+    uint32_t reserved;            /* must be 0 */
+} FvFlatSearchParams;
+
+size_t fv_flat_workspace_size(const FvFlatTables* tables, uint32_t num_queries,
+                              const FvFlatSearchParams* params);
 int fv_flat_search(const FvFlatTables* tables, const float* d_queries, uint32_t num_queries,
-                   uint32_t top_k, uint32_t shortlist, MetricType metric,
-                   const uint32_t* d_valid_mask, GpuSearchResult* d_out_results,
+                   const FvFlatSearchParams* params, GpuSearchResult* d_out_results,
                    void* d_workspace, size_t workspace_bytes, void* stream);
 
 /* ---- IVF-PQ scan with uneven batches ------------------------------------

@@ -119,3 +119,44 @@ def test_mfma_layout_probe_matches_cpu(torch):
     ids, d = run(torch, db, qs, 64, "ip", keep_fp32=False)
     ref = -(qs.astype(np.float64) @ db.astype(np.float64).T)
     np.testing.assert_array_equal(np.take_along_axis(ref, ids.astype(np.int64), 1), d)
+
+
+def test_sparse_mask_compaction(torch):
+    """A mask keeping 5% of rows with its popcount supplied: only valid rows
+    are scored (compacted on the device); results equal the oracle."""
+    db, qs = S.flat_workload(80000, 300, 128, 64, seeds=(61, 62, 63))
+    mask = S.random_valid_mask(80000, 0.05, 64)
+    vc = int(S.unpack_bits(mask, 80000).sum())
+    idx = F.FlatIndex(dev(torch, db), "f16")
+    for count in (vc, 0):                     # compacted, then dense masked scan
+        out = idx.search(dev(torch, qs), 10, "l2", dev(torch, mask), valid_count=count)
+        torch.cuda.synchronize()
+        assert_same(L.split_results(out), O.flat_topk(db, qs, 10, "l2", valid_mask=mask))
+    odd = S.random_valid_mask(80001, 0.2, 65)[: (80000 + 31) // 32]   # tail word bits past n
+    out = idx.search(dev(torch, qs), 10, "l2", dev(torch, odd), valid_count=1)
+    torch.cuda.synchronize()
+    assert_same(L.split_results(out), O.flat_topk(db, qs, 10, "l2", valid_mask=odd))
+
+
+def test_per_query_k_and_factor(torch):
+    db, qs = S.flat_workload(30000, 260, 128, 64, seeds=(71, 72, 73))
+    rng = np.random.default_rng(74)
+    ks = rng.integers(1, 65, size=260).astype(np.int32)
+    idx = F.FlatIndex(dev(torch, db), "bf16")
+    out = idx.search(dev(torch, qs), 64, "ip", shortlist_factor=4,
+                     top_k_per_query=dev(torch, ks))
+    torch.cuda.synchronize()
+    ids, d = L.split_results(out)
+    e_ids, e_d = O.flat_topk(db, qs, 64, "ip")
+    for i in range(260):
+        k = ks[i]
+        np.testing.assert_array_equal(ids[i, :k], e_ids[i, :k])
+        np.testing.assert_array_equal(d[i, :k].view(np.uint32), e_d[i, :k].view(np.uint32))
+        assert (ids[i, k:] == 0xFFFFFFFF).all()
+
+
+def test_deep_shortlist_workgroup_prunes(torch):
+    """Shortlists above 960 rows switch the scan to workgroup-wide prunes."""
+    db, qs = S.flat_workload(40000, 70, 64, 64, seeds=(81, 82, 83))
+    got = run(torch, db, qs, 200, "l2", "f16", shortlist=2000)
+    assert_same(got, O.flat_topk(db, qs, 200, "l2"))
