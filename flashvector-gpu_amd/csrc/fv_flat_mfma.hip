@@ -47,6 +47,7 @@ constexpr int kQPerWave = 64;
 constexpr int kQPerBlock = 4 * kQPerWave;
 constexpr int kRowsPerTile = 64;
 constexpr uint32_t kSelectBatch = 256;
+constexpr uint32_t kStage = 512;  // staged survivors per wave (two query tiles' worth)
 
 template <int DT>
 struct MfmaOps;
@@ -102,13 +103,18 @@ __device__ __forceinline__ void query_k_s(const FlatSearchArgs& a, uint32_t q, u
     if (S > a.s_max) S = a.s_max;
 }
 
-template <int DT, int METRIC, int KS, int R>
+// BOUND = true is the first pass: same tiles and MMAs, but the epilogue only
+// keeps a running minimum per (lane, element) — 16 disjoint row subsets per
+// query per chunk — written to `minima`; rows are limited to the first
+// `bound_rows` positions. No thresholds, buffers or prunes.
+template <int DT, int METRIC, int KS, int R, bool BOUND>
 __global__ __launch_bounds__(kMfmaThreads, 2) void flat_mfma_scan_kernel(
     const uint16_t* __restrict__ X, uint32_t row_stride, const float* __restrict__ norms,
     const uint16_t* __restrict__ Qh, uint32_t nq, FlatSearchArgs a, uint32_t n_chunks,
     uint32_t qblocks, uint32_t cap, const uint32_t* __restrict__ rows,
     const uint32_t* __restrict__ nrows, uint32_t n, uint32_t* __restrict__ gthr,
-    uint32_t* __restrict__ counts, uint64_t* __restrict__ cand) {
+    uint32_t* __restrict__ counts, uint64_t* __restrict__ cand, float* __restrict__ minima,
+    uint32_t bound_rows) {
     using Ops = MfmaOps<DT>;
     using vec = typename Ops::vec;
     constexpr int kRowBytes = KS * 64;          // 32*KS 16-bit elements
@@ -146,7 +152,8 @@ __global__ __launch_bounds__(kMfmaThreads, 2) void flat_mfma_scan_kernel(
         qblock = b % qblocks;
     }
     // Chunk bounds in the (possibly compacted) row space.
-    const uint32_t n_eff = nrows ? *nrows : n;
+    uint32_t n_eff = nrows ? *nrows : n;
+    if (BOUND && n_eff > bound_rows) n_eff = bound_rows;
     uint32_t rpc = (n_eff + n_chunks - 1) / n_chunks;
     rpc = (rpc + kRowsPerTile - 1) / kRowsPerTile * kRowsPerTile;
     const uint32_t r_begin = min(n_eff, chunk * rpc);
@@ -236,6 +243,25 @@ __global__ __launch_bounds__(kMfmaThreads, 2) void flat_mfma_scan_kernel(
         }
     };
 
+    // Wave-private staging list of survivors, flushed into the per-query
+    // buffers by all 64 lanes at once.
+    // Wave mode: the staging list aliases the wave's prune scratch (never live
+    // at the same time). Workgroup mode: it follows the shared scratch.
+    uint64_t* st_key = kBlockPrune ? s_scratch + cap + (size_t)wave * (kStage + kStage / 8)
+                                   : my_scratch;
+    uint8_t* st_q = reinterpret_cast<uint8_t*>(st_key + kStage);
+    uint32_t nstage = 0;
+    auto flush_stage = [&]() {
+        wave_lds_sync();
+        for (uint32_t e = lane; e < nstage; e += kWave) {
+            const uint32_t ql = wave * kQPerWave + st_q[e];
+            const uint32_t pos = atomicAdd(&s_cnt[ql], 1u);
+            cand[((size_t)chunk * nq + qb0 + ql) * cap + pos] = st_key[e];
+        }
+        nstage = 0;
+        wave_lds_sync();
+    };
+
     if (ntiles > 0) {
         fetch_rid(r_begin);
         load_tile(r_begin);
@@ -243,6 +269,10 @@ __global__ __launch_bounds__(kMfmaThreads, 2) void flat_mfma_scan_kernel(
         store_tile(0);
     }
     __syncthreads();
+
+    floatx4 runmin[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) runmin[j] = floatx4{kAdmit, kAdmit, kAdmit, kAdmit};
 
     for (uint32_t t = 0; t < ntiles; ++t) {
         const int buf = t & 1;
@@ -279,27 +309,40 @@ __global__ __launch_bounds__(kMfmaThreads, 2) void flat_mfma_scan_kernel(
 #pragma unroll
                 for (int j = 0; j < 4; ++j) acc[j] = acc[j] * scl;
             }
+            if constexpr (BOUND) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) runmin[j][i] = fminf(runmin[j][i], acc[j][i]);
+                continue;
+            }
+            // Survivors are compacted wave-wide into the staging list (ballot +
+            // mbcnt), so a slot costs one compare unless some lane inserts.
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const float m = fminf(fminf(acc[j][0], acc[j][1]), fminf(acc[j][2], acc[j][3]));
-                if (m < thr[j]) {
-                    const uint32_t ql = wave * kQPerWave + 16 * j + l16;
-                    const uint32_t qg = qb0 + ql;
-                    if (qg < nq) {
-                        uint64_t* bq = cand + ((size_t)chunk * nq + qg) * cap;
+                const bool live_q = qw0 + 16 * j + l16 < nq;
 #pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            if (acc[j][i] < thr[j]) {
-                                const uint32_t pos = atomicAdd(&s_cnt[ql], 1u);
-                                bq[pos] = make_key(acc[j][i], s_rowid[buf][rt * 16 + 4 * grp + i]);
-                            }
+                for (int i = 0; i < 4; ++i) {
+                    const bool f = live_q && acc[j][i] < thr[j];
+                    const uint64_t bm = __ballot(f);
+                    if (bm) {
+                        const uint32_t idx = nstage + __builtin_amdgcn_mbcnt_hi(
+                            (uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
+                        if (f) {
+                            st_key[idx] = make_key(acc[j][i], s_rowid[buf][rt * 16 + 4 * grp + i]);
+                            st_q[idx] = (uint8_t)(16 * j + l16);
                         }
+                        nstage += (uint32_t)__popcll(bm);
                     }
                 }
+                // One query tile stages at most 4 * 64 survivors.
+                if (nstage > kStage - 4 * kWave) flush_stage();
             }
         }
+        if (!BOUND) flush_stage();
         if (t + 1 < ntiles) store_tile(buf ^ 1);
         __syncthreads();
+        if (BOUND) continue;
 
         if (!kBlockPrune) {
             // Wave-private prunes of buffers that could overflow on the next tile.
@@ -367,7 +410,47 @@ __global__ __launch_bounds__(kMfmaThreads, 2) void flat_mfma_scan_kernel(
         if (my_live) s_thr[tid] = fminf(s_thr[tid], bound_from_ord(gpre));
         wave_lds_sync();
     }
+    if (BOUND) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t qg = qw0 + 16 * j + l16;
+            if (qg < nq) {
+                float* mo = minima + ((size_t)chunk * nq + qg) * 16 + 4 * grp;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) mo[i] = runmin[j][i];
+            }
+        }
+        return;
+    }
     if (my_live) counts[(size_t)chunk * nq + my_q] = s_cnt[tid];
+}
+
+// Per query: the S-th smallest of the first pass's subset minima. Each minimum
+// is the score of a distinct row, so at least S rows score at or below it:
+// it bounds the global S-th best and seeds the shared bound of the main pass.
+__global__ __launch_bounds__(256) void bound_select_kernel(const float* __restrict__ minima,
+                                                           uint32_t m, uint32_t nq,
+                                                           FlatSearchArgs a,
+                                                           uint32_t* __restrict__ gthr) {
+    extern __shared__ uint64_t s_b[];
+    const uint32_t q = blockIdx.x;
+    const uint32_t np = next_pow2(m);
+    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) {
+        uint64_t key = kEmptyKey;
+        if (i < m) {
+            const uint32_t c = i >> 4, e = i & 15u;
+            const float v = minima[((size_t)c * nq + q) * 16 + e];
+            if (v < kAdmit) key = make_key(v, i);
+        }
+        s_b[i] = key;
+    }
+    __syncthreads();
+    block_bitonic_sort(s_b, np);
+    if (threadIdx.x == 0) {
+        uint32_t k, S;
+        query_k_s(a, q, k, S);
+        if (S <= m && s_b[S - 1] != kEmptyKey) atomicMin(&gthr[q], (uint32_t)(s_b[S - 1] >> 32));
+    }
 }
 
 // One workgroup per query: streams the candidates of every chunk that are
@@ -559,10 +642,9 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, uint32_t nq, const 
     const uint32_t smax = deepest_shortlist(a);
     p.s_max = smax;
     // Buffers hold S plus one tile of appends plus one wavefront of bisection slack.
-    if (smax + 2 * kRowsPerTile <= 256u) {
-        p.cap = 256u;
-        p.prune_regs = 4;
-    } else if (smax + 2 * kRowsPerTile <= 1024u) {
+    // A 1024-key buffer absorbs the whole warm-up of a typical chunk, so
+    // most (query, chunk) pairs prune at most once.
+    if (smax + 2 * kRowsPerTile <= 1024u) {
         p.cap = 1024u;
         p.prune_regs = 16;
     } else {
@@ -570,22 +652,33 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, uint32_t nq, const 
         p.prune_regs = 0;  // workgroup prunes through shared scratch
     }
     p.block_prune = p.prune_regs == 0;
-    p.lds_scan = (size_t)(p.block_prune ? 1 : 4) * p.cap * 8;
+    // Per wave: prune scratch and staging list share one region; workgroup
+    // prunes use one shared scratch followed by four staging lists.
+    const size_t stage_bytes = (size_t)kStage * 9;
+    p.lds_scan = p.block_prune ? (size_t)p.cap * 8 + 4 * (kStage + kStage / 8) * 8
+                               : 4 * ((size_t)p.cap * 8 > stage_bytes ? (size_t)p.cap * 8 : stage_bytes);
     p.lcap = next_pow2(smax + kSelectBatch);
     const uint32_t ks = (padded_dim + 31) / 32;
     p.ks = ks <= 4 ? ks : (ks <= 6 ? 6 : 8);
     p.qblocks = (nq + kQPerBlock - 1) / kQPerBlock;
     p.compact = has_mask && valid_count > 0 && (uint64_t)valid_count * 2 <= n;
     const uint32_t rows_plan = p.compact ? valid_count : n;
-    // Chunks: enough workgroups to cover 256 CUs several times, >= 2048 rows
-    // each, a multiple of 8 (one slice per XCD) when there are at least 8.
+    // Chunks: enough workgroups to cover 256 CUs several times, but >= 8192
+    // rows each (every chunk pays a cold start of about S*ln(rows/S) inserts
+    // before its threshold settles), a multiple of 8 (one slice per XCD) when
+    // there are at least 8.
     const uint32_t want = (2560u + p.qblocks - 1) / p.qblocks;
-    const uint32_t max_by_rows = (rows_plan + 2047u) / 2048u;
+    const uint32_t max_by_rows = (rows_plan + 8191u) / 8192u;
     uint32_t c = want < max_by_rows ? want : max_by_rows;
     if (c > 256u) c = 256u;
     if (c >= 8u) c &= ~7u;
     if (c < 1u) c = 1u;
     p.n_chunks = c;
+    // First pass: subset minima over the first bound_rows positions, 64 chunks
+    // (1024 minima per query) so the S-th smallest is a tight bound.
+    const uint32_t brows = rows_plan / 4 > 262144u ? rows_plan / 4 : (rows_plan < 262144u ? rows_plan : 262144u);
+    p.bound_rows = brows;
+    p.bound_chunks = (brows >= 64u * kRowsPerTile && 16u * 64u >= smax) ? 64u : 0u;
     size_t off = 0;
     auto take = [&](size_t bytes) {
         size_t o = off;
@@ -596,6 +689,7 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, uint32_t nq, const 
     p.off_counts = take((size_t)p.n_chunks * nq * 4);
     p.off_cand = take((size_t)p.n_chunks * nq * p.cap * 8);
     p.off_qh = take((size_t)p.qblocks * kQPerBlock * 32 * p.ks * 2);
+    if (p.bound_chunks) p.off_minima = take((size_t)p.bound_chunks * nq * 16 * 4);
     if (p.compact) {
         const uint32_t nwords = (n + 31) / 32;
         p.mask_blocks = (nwords + 255) / 256;
@@ -626,6 +720,7 @@ struct ScanLaunch {
     uint32_t* gthr;
     uint32_t* counts;
     uint64_t* cand;
+    float* minima;
     hipStream_t stream;
 };
 
@@ -633,15 +728,24 @@ template <int DT, int METRIC, int KS, int R>
 static void launch_scan_r(const ScanLaunch& L) {
     const FlatMfmaPlan& p = *L.p;
     const dim3 g(p.qblocks * p.n_chunks);
-    flat_mfma_scan_kernel<DT, METRIC, KS, R><<<g, kMfmaThreads, p.lds_scan, L.stream>>>(
+    flat_mfma_scan_kernel<DT, METRIC, KS, R, false><<<g, kMfmaThreads, p.lds_scan, L.stream>>>(
         L.X, L.stride, L.norms, L.Qh, L.nq, L.a, p.n_chunks, p.qblocks, p.cap, L.rows, L.nrows,
-        L.n, L.gthr, L.counts, L.cand);
+        L.n, L.gthr, L.counts, L.cand, nullptr, 0u);
 }
 
 template <int DT, int METRIC, int KS>
 static void launch_scan(const ScanLaunch& L) {
-    if (L.p->prune_regs == 4) launch_scan_r<DT, METRIC, KS, 4>(L);
-    else if (L.p->prune_regs == 16) launch_scan_r<DT, METRIC, KS, 16>(L);
+    const FlatMfmaPlan& p = *L.p;
+    if (p.bound_chunks) {
+        const dim3 g(p.qblocks * p.bound_chunks);
+        flat_mfma_scan_kernel<DT, METRIC, KS, 16, true><<<g, kMfmaThreads, 0, L.stream>>>(
+            L.X, L.stride, L.norms, L.Qh, L.nq, L.a, p.bound_chunks, p.qblocks, p.cap, L.rows,
+            L.nrows, L.n, L.gthr, L.counts, L.cand, L.minima, p.bound_rows);
+        const uint32_t m = p.bound_chunks * 16;
+        bound_select_kernel<<<L.nq, 256, (size_t)next_pow2(m) * 8, L.stream>>>(
+            L.minima, m, L.nq, L.a, L.gthr);
+    }
+    if (L.p->prune_regs == 16) launch_scan_r<DT, METRIC, KS, 16>(L);
     else launch_scan_r<DT, METRIC, KS, 0>(L);
 }
 
@@ -697,8 +801,9 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
         else
             encode_queries_kernel<FV_DTYPE_BF16><<<blocks, 256, 0, stream>>>(Qm, nq, t->dim, pdq, qrows, Qh);
     }
+    float* minima = p.bound_chunks ? reinterpret_cast<float*>(ws + p.off_minima) : nullptr;
     ScanLaunch L{&p, static_cast<const uint16_t*>(t->d_codes), t->padded_dim, t->d_norms_sq, Qh, nq,
-                 a, rows, nrows, t->num_vectors, gthr, counts, cand, stream};
+                 a, rows, nrows, t->num_vectors, gthr, counts, cand, minima, stream};
     void* timer = timer_begin(stream);
     hipError_t e = hipErrorInvalidValue;
     if (t->dtype == FV_DTYPE_F16) e = dispatch_metric<FV_DTYPE_F16>(L, a.metric);

@@ -28,8 +28,10 @@ struct FlatMfmaPlan {
     uint32_t qblocks;         // 256-query blocks
     uint32_t n_chunks;
     bool compact;             // scan the compacted list of valid rows
+    uint32_t bound_chunks;    // first-pass chunks (0 = no first pass)
+    uint32_t bound_rows;      // rows covered by the first pass
     uint32_t mask_blocks;
-    size_t off_gthr, off_counts, off_cand, off_qh, off_rows, off_nrows, off_blocks;
+    size_t off_gthr, off_counts, off_cand, off_qh, off_rows, off_nrows, off_blocks, off_minima;
     size_t workspace_bytes;
 };
 
