@@ -26,6 +26,16 @@ MFMA_PEAK_TFLOPS = {"f16": 2500.0, "bf16": 2500.0}
 FP32_PEAK_TFLOPS = 157.3
 
 
+def measured_traffic(key):
+    """HBM bytes per launch of a kernel/workload from the committed rocprofv3
+    counter summaries (profiles/traffic.json), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            return json.load(f).get(key, {}).get("hbm_bytes_per_launch")
+    except OSError:
+        return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -223,7 +233,8 @@ def main():
         "plain_fp32_scan": fp32,
         "second_mask_reading": alt,
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak,
-                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+                     "traffic": measured_traffic("flat_mfma_scan_kernel/flat_l2_mfma_rerank"),
                      "kernel": "flat_mfma_scan_kernel", "kernel_avg_ms": round(scan_avg_ms, 4),
                      "algorithmic_flop_per_launch": flops},
         "cpu_baseline": cpu,
@@ -324,7 +335,8 @@ def bench_ivfpq(args):
         "recall_at_k_vs_exact": round(hits / tot, 5), "recall_sample_queries": rs,
         "ids_bit_exact_vs_cpu_adc": ids_match,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-                     "frac": round(achieved / 8000.0, 4), "traffic": None,
+                     "frac": round(achieved / 8000.0, 4),
+                     "traffic": measured_traffic("ivf_scan_kernel/ivfpq_uneven_10m"),
                      "kernel": "ivf_scan_kernel", "kernel_avg_ms": round(scan_avg, 4),
                      "algorithmic_bytes_per_launch": nbytes},
         "cpu_baseline": cpu,

@@ -34,7 +34,10 @@ __global__ void row_norms_kernel(const float* __restrict__ x, uint32_t n, uint32
     out[i] = acc;
 }
 
-template <int METRIC>
+// FMA = true accumulates L2 as fmaf(diff, diff, acc), the order of the IVF
+// coarse quantizer (kernels/src/ivf_rerank.cuh:81-84); false is the reference
+// CPU path's non-fused order.
+template <int METRIC, bool FMA>
 __global__ __launch_bounds__(kScanThreads) void flat_exact_scan_kernel(
     const float* __restrict__ X, uint32_t n, uint32_t dim, const float* __restrict__ Qm,
     uint32_t nq, const float* __restrict__ xnorm, const float* __restrict__ qnorm,
@@ -101,7 +104,10 @@ __global__ __launch_bounds__(kScanThreads) void flat_exact_scan_kernel(
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        if (METRIC == METRIC_L2) {
+                        if (METRIC == METRIC_L2 && FMA) {
+                            float diff = qa[i] - xa[j];
+                            acc[i][j] = __builtin_fmaf(diff, diff, acc[i][j]);
+                        } else if (METRIC == METRIC_L2) {
                             float diff = qa[i] - xa[j];
                             float s = diff * diff;
                             acc[i][j] = acc[i][j] + s;
@@ -236,7 +242,8 @@ FlatExactPlan plan_flat_exact(uint32_t n, uint32_t dim, uint32_t nq, uint32_t to
 hipError_t launch_flat_exact(const float* X, const uint32_t* vec_ids, uint32_t n, uint32_t dim,
                              const float* Qm, uint32_t nq, uint32_t top_k, int metric,
                              const uint32_t* mask, GpuSearchResult* out, uint32_t out_stride,
-                             void* workspace, const FlatExactPlan& p, hipStream_t stream) {
+                             void* workspace, const FlatExactPlan& p, hipStream_t stream,
+                             bool fma_l2) {
     char* ws = static_cast<char*>(workspace);
     uint64_t* cand = reinterpret_cast<uint64_t*>(ws + p.off_cand);
     uint64_t* partial = reinterpret_cast<uint64_t*>(ws + p.off_partial);
@@ -250,15 +257,18 @@ hipError_t launch_flat_exact(const float* X, const uint32_t* vec_ids, uint32_t n
     }
     dim3 grid(p.qblocks, p.n_chunks);
     size_t lds = (size_t)4 * p.cap * sizeof(uint64_t);
-    void* timer = timer_begin(stream);
+    void* timer = fma_l2 ? nullptr : timer_begin(stream);
     if (metric == METRIC_INNER_PRODUCT)
-        flat_exact_scan_kernel<METRIC_INNER_PRODUCT><<<grid, kScanThreads, lds, stream>>>(
+        flat_exact_scan_kernel<METRIC_INNER_PRODUCT, false><<<grid, kScanThreads, lds, stream>>>(
             X, n, dim, Qm, nq, xnorm, qnorm, mask, p.keep, p.cap, p.rows_per_chunk, cand, partial);
     else if (metric == METRIC_COSINE)
-        flat_exact_scan_kernel<METRIC_COSINE><<<grid, kScanThreads, lds, stream>>>(
+        flat_exact_scan_kernel<METRIC_COSINE, false><<<grid, kScanThreads, lds, stream>>>(
+            X, n, dim, Qm, nq, xnorm, qnorm, mask, p.keep, p.cap, p.rows_per_chunk, cand, partial);
+    else if (fma_l2)
+        flat_exact_scan_kernel<METRIC_L2, true><<<grid, kScanThreads, lds, stream>>>(
             X, n, dim, Qm, nq, xnorm, qnorm, mask, p.keep, p.cap, p.rows_per_chunk, cand, partial);
     else
-        flat_exact_scan_kernel<METRIC_L2><<<grid, kScanThreads, lds, stream>>>(
+        flat_exact_scan_kernel<METRIC_L2, false><<<grid, kScanThreads, lds, stream>>>(
             X, n, dim, Qm, nq, xnorm, qnorm, mask, p.keep, p.cap, p.rows_per_chunk, cand, partial);
     timer_end(timer, stream);
     uint32_t total = next_pow2(p.n_chunks * p.keep);
@@ -293,7 +303,7 @@ int cuda_flat_scan_batch(const float* d_vectors, const uint32_t* d_vec_ids, uint
     hipError_t e = hipMallocAsync(&ws, p.workspace_bytes, s);
     if (e != hipSuccess) return (int)e;
     e = fv::launch_flat_exact(d_vectors, d_vec_ids, num_vectors, dim, d_queries, num_queries,
-                              top_k, (int)metric, d_valid_mask, d_out_results, top_k, ws, p, s);
+                              top_k, (int)metric, d_valid_mask, d_out_results, top_k, ws, p, s, false);
     hipError_t e2 = hipFreeAsync(ws, s);
     if (e != hipSuccess) return (int)e;
     return e2 == hipSuccess ? 0 : (int)e2;
@@ -325,7 +335,7 @@ int fv_flat_exact_search(const float* d_vectors, uint32_t num_vectors, uint32_t 
     if (!d_workspace || workspace_bytes < p.workspace_bytes) return (int)hipErrorInvalidValue;
     hipError_t e = fv::launch_flat_exact(d_vectors, nullptr, num_vectors, dim, d_queries,
                                          num_queries, top_k, (int)metric, d_valid_mask,
-                                         d_out_results, top_k, d_workspace, p, s);
+                                         d_out_results, top_k, d_workspace, p, s, false);
     return e == hipSuccess ? 0 : (int)e;
 }
 

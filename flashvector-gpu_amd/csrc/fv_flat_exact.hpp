@@ -22,11 +22,15 @@ FlatExactPlan plan_flat_exact(uint32_t n, uint32_t dim, uint32_t nq, uint32_t to
 hipError_t launch_flat_exact(const float* X, const uint32_t* vec_ids, uint32_t n, uint32_t dim,
                              const float* Qm, uint32_t nq, uint32_t top_k, int metric,
                              const uint32_t* mask, GpuSearchResult* out, uint32_t out_stride,
-                             void* workspace, const FlatExactPlan& p, hipStream_t stream);
+                             void* workspace, const FlatExactPlan& p, hipStream_t stream,
+                             bool fma_l2 = false);
 
 // Kernel timer hooks (fv_kernel_timer_enable): begin returns a token or NULL.
 void* timer_begin(hipStream_t stream);
 void timer_end(void* token, hipStream_t stream);
+
+// Device fill of n words (a kernel, so calls stay graph-capturable).
+void fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t stream);
 
 hipError_t fill_results(GpuSearchResult* out, size_t count, uint32_t id_bits, uint32_t dist_bits,
                         hipStream_t stream);

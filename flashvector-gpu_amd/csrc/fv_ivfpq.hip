@@ -80,9 +80,13 @@ __device__ __forceinline__ uint64_t block_prune(uint64_t* buf, uint32_t* s_cnt, 
     return thr;
 }
 
+// Persistent: each workgroup repeatedly takes the next query from `order`
+// (heaviest first) through a global atomic counter, so queries of very uneven
+// cost (Zipf lists, per-query nprobe) are balanced dynamically.
 template <bool LISTORDER, bool GBUF, int METRIC>
 __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(
     IvfPqGpuTables t, const uint8_t* __restrict__ codes_lo, const float* __restrict__ Qm,
+    uint32_t nq, const uint32_t* __restrict__ order, uint32_t* __restrict__ next_q,
     const uint32_t* __restrict__ probes, uint32_t max_nprobe, const uint32_t* __restrict__ k_q,
     uint32_t k_u, uint32_t shortlist_u, uint32_t cap, int do_rerank,
     const uint32_t* __restrict__ mask, uint64_t* __restrict__ gbuf,
@@ -93,8 +97,14 @@ __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(
                          : reinterpret_cast<uint64_t*>(s_dyn + (size_t)t.m_pq * 256 * sizeof(float));
     __shared__ uint32_t s_cnt;
     __shared__ uint64_t s_thr;
+    __shared__ uint32_t s_item;
 
-    const uint32_t q = blockIdx.x;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_item = atomicAdd(next_q, 1u);
+    __syncthreads();
+    if (s_item >= nq) return;
+    const uint32_t q = order[s_item];
     const uint32_t M = t.m_pq, sub = t.sub_dim, dim = t.dim;
     const float* qv = Qm + (size_t)q * dim;
     uint32_t k = k_q ? k_q[q] : k_u;
@@ -127,32 +137,62 @@ __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(
         const uint32_t beg = t.d_ivf_offsets[c], end = t.d_ivf_offsets[c + 1];
         for (uint32_t base = beg; base < end; base += kEntriesPerCheck) {
             const uint64_t thr = s_thr;
-#pragma unroll 1
-            for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
-                const uint32_t e = base + pass * kIvfThreads + threadIdx.x;
-                if (e >= end) break;
-                const uint32_t id = t.d_ivf_vec_ids[e];
-                if (!mask_allows(mask, id)) continue;
-                const uint8_t* cp = LISTORDER ? codes_lo + (size_t)e * M : t.d_pq_codes + (size_t)id * M;
-                float adc = 0.0f;
-                if ((M & 15u) == 0u && (reinterpret_cast<uintptr_t>(cp) & 15u) == 0u) {
-                    for (uint32_t m0 = 0; m0 < M; m0 += 16) {
-                        const uint4 w = *reinterpret_cast<const uint4*>(cp + m0);
-                        const uint32_t ww[4] = {w.x, w.y, w.z, w.w};
+            if (M == 16u) {
+                // Issue the id and code loads of all passes first so that each
+                // wave keeps kPassesPerCheck 128-bit loads in flight.
+                uint32_t ids[kPassesPerCheck];
+                uint4 cw[kPassesPerCheck];
 #pragma unroll
-                        for (int b = 0; b < 16; ++b) {
-                            const uint32_t code = (ww[b >> 2] >> (8 * (b & 3))) & 255u;
-                            adc = adc + lut[(m0 + b) * 256u + code];
+                for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
+                    const uint32_t e = base + pass * kIvfThreads + threadIdx.x;
+                    const uint32_t ec = e < end ? e : end - 1;
+                    ids[pass] = t.d_ivf_vec_ids[ec];
+                    if (LISTORDER) cw[pass] = *reinterpret_cast<const uint4*>(codes_lo + (size_t)ec * 16u);
+                }
+                if (!LISTORDER) {
+#pragma unroll
+                    for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
+                        const uint8_t* cp = t.d_pq_codes + (size_t)ids[pass] * 16u;
+                        if ((reinterpret_cast<uintptr_t>(cp) & 15u) == 0u) {
+                            cw[pass] = *reinterpret_cast<const uint4*>(cp);
+                        } else {
+                            uint32_t w4[4];
+                            for (int b = 0; b < 4; ++b)
+                                w4[b] = cp[4 * b] | (cp[4 * b + 1] << 8) | (cp[4 * b + 2] << 16) | ((uint32_t)cp[4 * b + 3] << 24);
+                            cw[pass] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
                         }
                     }
-                } else {
-                    for (uint32_t m = 0; m < M; ++m) adc = adc + lut[m * 256u + cp[m]];
                 }
-                if (adc < kAdmit) {
-                    const uint64_t key = make_key(adc, id);
-                    if (key < thr) {
-                        const uint32_t pos = atomicAdd(&s_cnt, 1u);
-                        buf[pos] = key;
+#pragma unroll
+                for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
+                    const uint32_t e = base + pass * kIvfThreads + threadIdx.x;
+                    const uint32_t id = ids[pass];
+                    if (e >= end || !mask_allows(mask, id)) continue;
+                    const uint32_t ww[4] = {cw[pass].x, cw[pass].y, cw[pass].z, cw[pass].w};
+                    float adc = 0.0f;
+#pragma unroll
+                    for (int b = 0; b < 16; ++b) {
+                        const uint32_t code = (ww[b >> 2] >> (8 * (b & 3))) & 255u;
+                        adc = adc + lut[b * 256u + code];
+                    }
+                    if (adc < kAdmit) {
+                        const uint64_t key = make_key(adc, id);
+                        if (key < thr) buf[atomicAdd(&s_cnt, 1u)] = key;
+                    }
+                }
+            } else {
+#pragma unroll 1
+                for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
+                    const uint32_t e = base + pass * kIvfThreads + threadIdx.x;
+                    if (e >= end) break;
+                    const uint32_t id = t.d_ivf_vec_ids[e];
+                    if (!mask_allows(mask, id)) continue;
+                    const uint8_t* cp = LISTORDER ? codes_lo + (size_t)e * M : t.d_pq_codes + (size_t)id * M;
+                    float adc = 0.0f;
+                    for (uint32_t m = 0; m < M; ++m) adc = adc + lut[m * 256u + cp[m]];
+                    if (adc < kAdmit) {
+                        const uint64_t key = make_key(adc, id);
+                        if (key < thr) buf[atomicAdd(&s_cnt, 1u)] = key;
                     }
                 }
             }
@@ -193,6 +233,60 @@ __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(
         }
         out[(size_t)q * out_stride + i] = r;
     }
+  }
+}
+
+// Scan cost of each query (entries over its probed lists), bucketed by
+// log2 so that the heaviest queries are dispatched first.
+__global__ void ivf_cost_kernel(const uint32_t* __restrict__ probes, uint32_t max_nprobe,
+                                const uint32_t* __restrict__ offsets, uint32_t nlist,
+                                uint32_t nq, uint32_t* __restrict__ bucket,
+                                uint32_t* __restrict__ bucket_count) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    uint64_t cost = 0;
+    for (uint32_t p = 0; p < max_nprobe; ++p) {
+        const uint32_t c = probes[(size_t)q * max_nprobe + p];
+        if (c >= nlist) break;
+        cost += offsets[c + 1] - offsets[c];
+    }
+    const uint32_t c32 = cost > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)cost;
+    const uint32_t b = (uint32_t)__clz(c32 | 1u);  // 0 = heaviest
+    bucket[q] = b;
+    atomicAdd(&bucket_count[b], 1u);
+}
+
+__global__ void ivf_order_kernel(const uint32_t* __restrict__ bucket,
+                                 const uint32_t* __restrict__ bucket_count, uint32_t nq,
+                                 uint32_t* __restrict__ bucket_fill, uint32_t* __restrict__ order) {
+    __shared__ uint32_t s_off[33];
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int b = 0; b < 33; ++b) {
+            s_off[b] = run;
+            run += bucket_count[b];
+        }
+    }
+    __syncthreads();
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+        const uint32_t b = bucket[q];
+        order[s_off[b] + atomicAdd(&bucket_fill[b], 1u)] = q;
+    }
+}
+
+// Probe list of each query from the coarse top-k results: the first np_q
+// cluster ids, empty past np_q.
+__global__ void probes_from_results_kernel(const GpuSearchResult* __restrict__ res, uint32_t nq,
+                                           uint32_t max_nprobe, const uint32_t* __restrict__ nprobe_q,
+                                           uint32_t nprobe_u, uint32_t nlist,
+                                           uint32_t* __restrict__ probes) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nq * max_nprobe) return;
+    const uint32_t q = (uint32_t)(i / max_nprobe), p = (uint32_t)(i % max_nprobe);
+    uint32_t np = nprobe_q ? nprobe_q[q] : nprobe_u;
+    if (np == 0) np = 1;
+    if (np > nlist) np = nlist;
+    probes[i] = p < np ? res[i].id : FV_EMPTY_ID;
 }
 
 __global__ void list_order_codes_kernel(const uint8_t* __restrict__ codes,
@@ -215,27 +309,41 @@ IvfPqPlan plan_ivfpq(const IvfPqGpuTables* t, uint32_t nq, uint32_t max_nprobe,
     const size_t lut_bytes = (size_t)t->m_pq * 256 * sizeof(float);
     p.gbuf = lut_bytes + (size_t)p.cap * 8 > 64 * 1024;
     p.lds_scan = lut_bytes + (p.gbuf ? 0 : (size_t)p.cap * 8);
+    p.grid = nq < kMaxScanGrid ? nq : kMaxScanGrid;
     size_t off = 0;
-    p.off_probes = off;
-    off += ((size_t)nq * max_nprobe * 4 + 255) / 256 * 256;
-    p.off_gbuf = off;
-    if (p.gbuf) off += (size_t)nq * p.cap * 8;
-    p.workspace_bytes = (off + 255) / 256 * 256;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) / 256 * 256;
+        return o;
+    };
+    p.off_probes = take((size_t)nq * max_nprobe * 4);
+    p.off_bucket = take((size_t)nq * 4);
+    p.off_order = take((size_t)nq * 4);
+    p.off_counters = take(68 * 4);  // 33 counts, 33 fills, next query, pad
+    p.off_gbuf = take(p.gbuf ? (size_t)p.grid * p.cap * 8 : 0);
+    p.tiled_coarse = max_nprobe <= 256u && t->nlist >= 64u;
+    if (p.tiled_coarse) {
+        p.coarse_plan = plan_flat_exact(t->nlist, t->dim, nq, max_nprobe, METRIC_L2);
+        p.off_coarse_res = take((size_t)nq * max_nprobe * sizeof(GpuSearchResult));
+        p.off_coarse_ws = take(p.coarse_plan.workspace_bytes);
+    }
+    p.workspace_bytes = off;
     return p;
 }
 
 template <bool LO, bool GB>
 static void launch_scan_m(int metric, dim3 g, size_t lds, hipStream_t s, const IvfPqGpuTables& t,
-                          const uint8_t* lo, const float* Qm, const uint32_t* probes, uint32_t mnp,
+                          const uint8_t* lo, const float* Qm, uint32_t nq, const uint32_t* order,
+                          uint32_t* next_q, const uint32_t* probes, uint32_t mnp,
                           const uint32_t* k_q, uint32_t k_u, uint32_t S, uint32_t cap, int rr,
                           const uint32_t* mask, uint64_t* gbuf, GpuSearchResult* out,
                           uint32_t stride) {
     if (metric == METRIC_INNER_PRODUCT)
-        ivf_scan_kernel<LO, GB, METRIC_INNER_PRODUCT><<<g, kIvfThreads, lds, s>>>(t, lo, Qm, probes, mnp, k_q, k_u, S, cap, rr, mask, gbuf, out, stride);
+        ivf_scan_kernel<LO, GB, METRIC_INNER_PRODUCT><<<g, kIvfThreads, lds, s>>>(t, lo, Qm, nq, order, next_q, probes, mnp, k_q, k_u, S, cap, rr, mask, gbuf, out, stride);
     else if (metric == METRIC_COSINE)
-        ivf_scan_kernel<LO, GB, METRIC_COSINE><<<g, kIvfThreads, lds, s>>>(t, lo, Qm, probes, mnp, k_q, k_u, S, cap, rr, mask, gbuf, out, stride);
+        ivf_scan_kernel<LO, GB, METRIC_COSINE><<<g, kIvfThreads, lds, s>>>(t, lo, Qm, nq, order, next_q, probes, mnp, k_q, k_u, S, cap, rr, mask, gbuf, out, stride);
     else
-        ivf_scan_kernel<LO, GB, METRIC_L2><<<g, kIvfThreads, lds, s>>>(t, lo, Qm, probes, mnp, k_q, k_u, S, cap, rr, mask, gbuf, out, stride);
+        ivf_scan_kernel<LO, GB, METRIC_L2><<<g, kIvfThreads, lds, s>>>(t, lo, Qm, nq, order, next_q, probes, mnp, k_q, k_u, S, cap, rr, mask, gbuf, out, stride);
 }
 
 hipError_t launch_ivfpq(const IvfPqGpuTables* t, const uint8_t* codes_lo, const float* Qm,
@@ -246,19 +354,40 @@ hipError_t launch_ivfpq(const IvfPqGpuTables* t, const uint8_t* codes_lo, const 
     char* ws = static_cast<char*>(workspace);
     uint32_t* probes = reinterpret_cast<uint32_t*>(ws + p.off_probes);
     uint64_t* gbuf = p.gbuf ? reinterpret_cast<uint64_t*>(ws + p.off_gbuf) : nullptr;
-    ivf_coarse_kernel<<<nq, 256, (size_t)p.npow * 8, stream>>>(
-        t->d_centroids, t->nlist, t->dim, Qm, nprobe_q, nprobe_u, p.max_nprobe, p.npow, probes);
+    uint32_t* bucket = reinterpret_cast<uint32_t*>(ws + p.off_bucket);
+    uint32_t* order = reinterpret_cast<uint32_t*>(ws + p.off_order);
+    uint32_t* ctr = reinterpret_cast<uint32_t*>(ws + p.off_counters);
+    if (p.tiled_coarse) {
+        // Tiled exact scan with fmaf accumulation: same values and (distance,
+        // cluster) order as the per-query coarse kernel, at GEMM-like reuse.
+        GpuSearchResult* cres = reinterpret_cast<GpuSearchResult*>(ws + p.off_coarse_res);
+        launch_flat_exact(t->d_centroids, nullptr, t->nlist, t->dim, Qm, nq, p.max_nprobe,
+                          METRIC_L2, nullptr, cres, p.max_nprobe, ws + p.off_coarse_ws,
+                          p.coarse_plan, stream, true);
+        const size_t tot = (size_t)nq * p.max_nprobe;
+        probes_from_results_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, stream>>>(
+            cres, nq, p.max_nprobe, nprobe_q, nprobe_u, t->nlist, probes);
+    } else {
+        ivf_coarse_kernel<<<nq, 256, (size_t)p.npow * 8, stream>>>(
+            t->d_centroids, t->nlist, t->dim, Qm, nprobe_q, nprobe_u, p.max_nprobe, p.npow, probes);
+    }
+    fill_u32(ctr, 68, 0u, stream);
+    ivf_cost_kernel<<<(nq + 255) / 256, 256, 0, stream>>>(probes, p.max_nprobe, t->d_ivf_offsets,
+                                                         t->nlist, nq, bucket, ctr);
+    ivf_order_kernel<<<(nq + 255) / 256 < 64 ? (nq + 255) / 256 : 64, 256, 0, stream>>>(
+        bucket, ctr, nq, ctr + 33, order);
+    uint32_t* next_q = ctr + 66;
     void* timer = timer_begin(stream);
-    const dim3 g(nq);
+    const dim3 g(p.grid);
     const bool lo = codes_lo != nullptr;
     if (lo && p.gbuf)
-        launch_scan_m<true, true>(metric, g, p.lds_scan, stream, *t, codes_lo, Qm, probes, p.max_nprobe, k_q, k_u, p.shortlist, p.cap, do_rerank, mask, gbuf, out, out_stride);
+        launch_scan_m<true, true>(metric, g, p.lds_scan, stream, *t, codes_lo, Qm, nq, order, next_q, probes, p.max_nprobe, k_q, k_u, p.shortlist, p.cap, do_rerank, mask, gbuf, out, out_stride);
     else if (lo)
-        launch_scan_m<true, false>(metric, g, p.lds_scan, stream, *t, codes_lo, Qm, probes, p.max_nprobe, k_q, k_u, p.shortlist, p.cap, do_rerank, mask, gbuf, out, out_stride);
+        launch_scan_m<true, false>(metric, g, p.lds_scan, stream, *t, codes_lo, Qm, nq, order, next_q, probes, p.max_nprobe, k_q, k_u, p.shortlist, p.cap, do_rerank, mask, gbuf, out, out_stride);
     else if (p.gbuf)
-        launch_scan_m<false, true>(metric, g, p.lds_scan, stream, *t, codes_lo, Qm, probes, p.max_nprobe, k_q, k_u, p.shortlist, p.cap, do_rerank, mask, gbuf, out, out_stride);
+        launch_scan_m<false, true>(metric, g, p.lds_scan, stream, *t, codes_lo, Qm, nq, order, next_q, probes, p.max_nprobe, k_q, k_u, p.shortlist, p.cap, do_rerank, mask, gbuf, out, out_stride);
     else
-        launch_scan_m<false, false>(metric, g, p.lds_scan, stream, *t, codes_lo, Qm, probes, p.max_nprobe, k_q, k_u, p.shortlist, p.cap, do_rerank, mask, gbuf, out, out_stride);
+        launch_scan_m<false, false>(metric, g, p.lds_scan, stream, *t, codes_lo, Qm, nq, order, next_q, probes, p.max_nprobe, k_q, k_u, p.shortlist, p.cap, do_rerank, mask, gbuf, out, out_stride);
     timer_end(timer, stream);
     return hipGetLastError();
 }

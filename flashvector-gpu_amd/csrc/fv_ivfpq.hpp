@@ -6,6 +6,7 @@
 namespace fv {
 
 constexpr uint32_t kMaxNlist = 8192;  // coarse selection sorts all centroids in 64 KB of LDS
+constexpr uint32_t kMaxScanGrid = 2048; // persistent scan workgroups (8 per CU on 256 CUs)
 
 struct IvfPqPlan {
     uint32_t shortlist;   // S, the ADC shortlist depth (>= every k)
@@ -14,7 +15,11 @@ struct IvfPqPlan {
     uint32_t npow;        // next power of two >= nlist
     bool gbuf;            // candidate buffer in global memory (deep shortlists)
     size_t lds_scan;      // dynamic LDS of the scan kernel
-    size_t off_probes, off_gbuf;
+    uint32_t grid;        // persistent scan workgroups
+    bool tiled_coarse;    // coarse probe through the tiled exact scan (nprobe <= 256)
+    FlatExactPlan coarse_plan;
+    size_t off_coarse_res, off_coarse_ws;
+    size_t off_probes, off_bucket, off_order, off_counters, off_gbuf;
     size_t workspace_bytes;
 };
 

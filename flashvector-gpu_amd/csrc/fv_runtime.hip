@@ -20,6 +20,15 @@ __global__ void fill_results_kernel(GpuSearchResult* out, size_t count, uint32_t
     }
 }
 
+__global__ void fill_words_kernel(uint32_t* p, size_t n, uint32_t v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t stream) {
+    if (n) fill_words_kernel<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(p, n, v);
+}
+
 hipError_t fill_results(GpuSearchResult* out, size_t count, uint32_t id_bits, uint32_t dist_bits,
                         hipStream_t stream) {
     if (count == 0) return hipSuccess;
