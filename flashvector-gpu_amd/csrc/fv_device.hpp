@@ -243,4 +243,79 @@ __device__ __forceinline__ float exact_distance_rt(const float* q, const float* 
     return exact_distance<METRIC_L2>(q, x, dim);
 }
 
+// Bound on |fp32 result - exact distance| for the sums of exact_distance
+// (standard forward error of a recursive sum of rounded terms, gamma_n ~ n*u
+// with u = 2^-24), doubled for margin. `d32` is the FP32 distance.
+template <int METRIC>
+__device__ __forceinline__ float exact_distance_err(const float* __restrict__ q,
+                                                    const float* __restrict__ x, uint32_t dim,
+                                                    float d32) {
+    constexpr float u = 5.9604645e-8f;
+    if (METRIC == METRIC_L2) return 2.0f * (float)(dim + 3) * u * d32;
+    float sabs = 0.0f, dot = 0.0f, nq = 0.0f, nx = 0.0f;
+    for (uint32_t d = 0; d < dim; ++d) {
+        const float p = q[d] * x[d];
+        sabs += fabsf(p);
+        if (METRIC == METRIC_COSINE) {
+            dot += p;
+            nq += q[d] * q[d];
+            nx += x[d] * x[d];
+        }
+    }
+    if (METRIC == METRIC_INNER_PRODUCT) return 2.0f * (float)(dim + 2) * u * sabs;
+    const float denom = sqrtf(nq) * sqrtf(nx) + 1e-8f;
+    return 2.0f * ((float)(dim + 5) * u * (sabs + fabsf(dot)) / denom + 2.0f * u);
+}
+
+// The same distances evaluated in FP64 (products of floats are exact there).
+template <int METRIC>
+__device__ __forceinline__ double exact_distance_f64(const float* __restrict__ q,
+                                                     const float* __restrict__ x, uint32_t dim) {
+    double a = 0.0, nq = 0.0, nx = 0.0;
+    for (uint32_t d = 0; d < dim; ++d) {
+        const double qd = q[d], xd = x[d];
+        if (METRIC == METRIC_L2) {
+            const double t = qd - xd;
+            a = __builtin_fma(t, t, a);
+        } else {
+            a = __builtin_fma(qd, xd, a);
+            if (METRIC == METRIC_COSINE) {
+                nq = __builtin_fma(qd, qd, nq);
+                nx = __builtin_fma(xd, xd, nx);
+            }
+        }
+    }
+    if (METRIC == METRIC_L2) return a;
+    if (METRIC == METRIC_INNER_PRODUCT) return -a;
+    return 1.0 - a / (sqrt(nq) * sqrt(nx) + 1e-8);
+}
+
+// OCP FP8 E4M3 (bias 7, no infinities, largest finite 448): round to nearest
+// even, saturating to +-448; NaN maps to the NaN code 0x7f.
+__host__ __device__ __forceinline__ uint8_t f32_to_e4m3(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    const uint8_t sign = (uint8_t)((u >> 24) & 0x80u);
+    u &= 0x7fffffffu;
+    if (u > 0x7f800000u) return (uint8_t)(sign | 0x7fu);
+    const float a = __builtin_bit_cast(float, u);
+    if (a >= 448.0f) return (uint8_t)(sign | 0x7eu);
+    const int e = (int)(u >> 23) - 127;
+    if (e < -6) {
+        // Subnormal range (step 2^-9); m == 8 is the code of 2^-6.
+        const uint32_t m = (uint32_t)rintf(a * 512.0f);
+        return (uint8_t)(sign | m);
+    }
+    const uint32_t mant = u & 0x7fffffu;
+    uint32_t r = mant >> 20;
+    const uint32_t rem = mant & 0xfffffu;
+    if (rem > 0x80000u || (rem == 0x80000u && (r & 1u))) ++r;
+    int ef = e + 7;
+    if (r == 8u) {
+        r = 0u;
+        ++ef;
+    }
+    if (ef > 15 || (ef == 15 && r == 7u)) return (uint8_t)(sign | 0x7eu);
+    return (uint8_t)(sign | (uint32_t)(ef << 3) | r);
+}
+
 }  // namespace fv

@@ -4,8 +4,8 @@
 // padded to a multiple of 16 halves, |x|^2 of the FP32 originals). Here the
 // encoder runs on the GPU, rounds to nearest even (the reference rounds half
 // up on magnitude; the two differ only on exact binary16 ties) and also emits
-// BF16 codes and FP64 norms.
-#include "fv_flat_exact.hpp"
+// BF16 codes, FP8 e4m3 codes with per-row scales, and FP64 norms.
+#include "fv_flat_mfma.hpp"
 
 namespace fv {
 
@@ -54,7 +54,11 @@ extern "C" int fv_encode_flat(const float* d_vectors, uint32_t num_vectors, uint
     if (padded_dim < dim || dim == 0) return (int)hipErrorInvalidValue;
     if (num_vectors == 0) return 0;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (d_codes_out) {
+    if (d_codes_out && dtype == FV_DTYPE_FP8_E4M3) {
+        if (padded_dim % 16) return (int)hipErrorInvalidValue;
+        fv::launch_encode_fp8_rows(d_vectors, num_vectors, dim, padded_dim, num_vectors, 1.0f,
+                                   static_cast<uint8_t*>(d_codes_out), d_row_scales_out, s);
+    } else if (d_codes_out) {
         if (dtype != FV_DTYPE_F16 && dtype != FV_DTYPE_BF16) return (int)hipErrorInvalidValue;
         size_t total = (size_t)num_vectors * padded_dim;
         fv::encode_16bit_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(
@@ -63,7 +67,6 @@ extern "C" int fv_encode_flat(const float* d_vectors, uint32_t num_vectors, uint
     if (d_norms_out || d_norms_f64_out)
         fv::norms_kernel<<<(num_vectors + 255) / 256, 256, 0, s>>>(d_vectors, num_vectors, dim,
                                                                   d_norms_out, d_norms_f64_out);
-    (void)d_row_scales_out;
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : (int)e;
 }

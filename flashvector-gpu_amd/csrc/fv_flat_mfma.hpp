@@ -15,31 +15,74 @@ struct FlatSearchArgs {
     uint32_t s_max;           // deepest shortlist of the call
     uint32_t dim;
     int metric;
+    uint32_t flags;           // FV_SEARCH_* bits
 };
 
 struct FlatMfmaPlan {
     uint32_t s_max;           // deepest per-query shortlist
     uint32_t cap;             // candidate buffer capacity per (chunk, query)
-    uint32_t prune_regs;      // keys per lane of a wave prune (4 or 16), 0 = workgroup prunes
+    uint32_t prune_regs;      // keys per lane of a wave prune (16), 0 = workgroup prunes
     bool block_prune;         // prune with the whole workgroup (deep shortlists)
     size_t lds_scan;          // dynamic LDS of the scan kernel
     uint32_t lcap;            // LDS keys of the select/rerank kernel
-    uint32_t ks;              // 32-wide K steps per row
-    uint32_t qblocks;         // 256-query blocks
+    size_t lds_select;        // dynamic LDS of the select/rerank kernel
+    uint32_t ks;              // K steps per row (32 elements for 16-bit, 128 for FP8)
+    uint32_t qw;              // queries per wave
+    uint32_t qb;              // queries per workgroup (4 * qw)
+    uint32_t qrow_bytes;      // bytes of one encoded query row
+    uint32_t qblocks;         // query blocks
     uint32_t n_chunks;
     bool compact;             // scan the compacted list of valid rows
     uint32_t bound_chunks;    // first-pass chunks (0 = no first pass)
     uint32_t bound_rows;      // rows covered by the first pass
     uint32_t mask_blocks;
-    size_t off_gthr, off_counts, off_cand, off_qh, off_rows, off_nrows, off_blocks, off_minima;
+    size_t off_gthr, off_counts, off_cand, off_qh, off_qscale, off_rows, off_nrows, off_blocks,
+        off_minima;
     size_t workspace_bytes;
 };
 
-FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, uint32_t nq, const FlatSearchArgs& a,
-                            bool has_mask, uint32_t valid_count);
+// Operands of one scan launch (main pass and optional first pass).
+struct ScanLaunch {
+    const FlatMfmaPlan* p;
+    const uint8_t* X;          // row codes
+    uint32_t row_bytes;        // stride of X in bytes
+    const float* norms;
+    const float* row_scales;   // FP8 only
+    const uint8_t* Qh;         // encoded negated queries
+    const float* qscale;       // FP8 only
+    uint32_t nq;
+    FlatSearchArgs a;
+    const uint32_t* rows;
+    const uint32_t* nrows;
+    uint32_t n;
+    uint32_t* gthr;
+    uint32_t* counts;
+    uint64_t* cand;
+    float* minima;
+    hipStream_t stream;
+};
+
+FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t nq,
+                            const FlatSearchArgs& a, bool has_mask, uint32_t valid_count);
 
 hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
                             const FlatSearchArgs& a, GpuSearchResult* out, void* workspace,
                             const FlatMfmaPlan& p, hipStream_t stream);
+
+// FP8 e4m3 rows with per-row scales (s = max|x|/448, code = e4m3(sign*x/s)),
+// rows >= n written as zeros with scale 1 (fv_flat_mfma.hip).
+void launch_encode_fp8_rows(const float* x, uint32_t n, uint32_t dim, uint32_t out_stride,
+                            uint32_t total_rows, float sign, uint8_t* out, float* scales,
+                            hipStream_t stream);
+
+// First-pass bound selection (fv_flat_mfma.hip), called by the scan launchers.
+void launch_bound_select(const ScanLaunch& L);
+
+// Scan launchers, one per dtype and K-step range (fv_flat_mfma_*.hip).
+hipError_t scan_f16_small(const ScanLaunch& L);   // ks in {1,2,3,4,6,8}
+hipError_t scan_f16_large(const ScanLaunch& L);   // ks in {12,16,24}
+hipError_t scan_bf16_small(const ScanLaunch& L);
+hipError_t scan_bf16_large(const ScanLaunch& L);
+hipError_t scan_fp8(const ScanLaunch& L);         // ks in {1,2,4,6}
 
 }  // namespace fv

@@ -1,0 +1,542 @@
+// Matrix-core scan kernel of the reduced-precision flat search, shared by the
+// per-dtype translation units (fv_flat_mfma_f16.hip, _bf16.hip, _fp8.hip),
+// which instantiate it for their K-step counts.
+//
+// Reference: kernels/src/ivf_quant_lookup.cu:887-1058 (wmma 16x16x16, one
+// block per 16 queries, FP32->FP16 query tiles reloaded per k-slice, serial
+// insertion epilogue, no rerank).
+//
+// Design (gfx950, wave64):
+//  * 16-bit codes: v_mfma_f32_16x16x32_{f16,bf16}, A = 16 database rows,
+//    B = 16 negated queries, C = 0.5*|x|^2 of each row, so D = 0.5|x|^2 - q.x
+//    and the L2 distance is |q|^2 + 2D: the |x|^2 term costs nothing.
+//  * FP8 e4m3 codes with one FP32 scale per row and per query:
+//    v_mfma_f32_16x16x128_f8f6f4 accumulates code products from zero and the
+//    epilogue applies D = c_row + (s_row * s_query) * acc.
+//  * Masked and out-of-range rows get c_row = +inf and never pass a threshold.
+//  * A workgroup owns 4 waves x QW queries (QW = 64, or 32 when 64 queries'
+//    fragments would not fit the register file) and one chunk of the row
+//    space. The queries' B fragments live in registers for the whole kernel;
+//    row tiles are double-buffered through LDS (rows padded by 16 bytes so the
+//    fragment reads are conflict-free), one barrier per tile.
+//  * The row space is either every row (the mask, if any, is tested per row)
+//    or a compacted ascending list of the valid rows (sparse masks). Chunk
+//    bounds are computed on the device from the actual row count.
+//  * Selection: survivors of the per-query threshold are compacted wave-wide
+//    (ballot + mbcnt) into an LDS staging list and flushed into per-(chunk,
+//    query) buffers in global memory. Near-full buffers are pruned to the
+//    query's shortlist depth S, by the owning wave (register bisection) or,
+//    for deep shortlists, by the workgroup. Thresholds combine the chunk's
+//    own S-th best (strict: rows arrive in ascending order) with a per-query
+//    bound shared by all chunks through global memory (non-strict: the S-th
+//    best of any chunk bounds the global S-th best), prefetched every tile.
+//  * Workgroup ids are mapped XCD-first so that the workgroups of one XCD
+//    sweep the same chunk together and share it through that XCD's L2.
+#pragma once
+
+#include "fv_flat_mfma.hpp"
+
+namespace fv {
+
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef int int32x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMfmaThreads = 256;
+constexpr uint32_t kStage = 512;  // staged survivors per wave (two query tiles' worth)
+
+template <int DT>
+struct MfmaOps;
+
+template <>
+struct MfmaOps<FV_DTYPE_F16> {
+    using vec = half8_t;
+    static constexpr int kLaneBytes = 16;  // 8 halves of K per lane per step
+    static __device__ __forceinline__ floatx4 mma(vec a, vec b, floatx4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct MfmaOps<FV_DTYPE_BF16> {
+    using vec = bf16x8_t;
+    static constexpr int kLaneBytes = 16;
+    static __device__ __forceinline__ floatx4 mma(vec a, vec b, floatx4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct MfmaOps<FV_DTYPE_FP8_E4M3> {
+    using vec = int32x8_t;
+    static constexpr int kLaneBytes = 32;  // 32 e4m3 codes of K per lane per step
+    static __device__ __forceinline__ floatx4 mma(vec a, vec b, floatx4 c) {
+        // Formats 0/0 = e4m3 for both operands; zero scale operands select
+        // the unscaled instruction.
+        return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, c, 0, 0, 0, 0, 0, 0);
+    }
+};
+
+// Compile-time shape of one instantiation.
+template <int DT, int KS>
+struct ScanShape {
+    static constexpr int kLaneBytes = MfmaOps<DT>::kLaneBytes;
+    static constexpr int kStepBytes = 4 * kLaneBytes;  // bytes of one row per K step
+    static constexpr int kRowBytes = KS * kStepBytes;
+    // Queries per wave: 64 while their fragments take at most 192 VGPRs.
+    static constexpr int kQW = 4 * KS * (kLaneBytes / 4) <= 192 ? 64 : 32;
+    static constexpr int kQB = 4 * kQW;
+    static constexpr int kBRegs = (kQW / 16) * KS * (kLaneBytes / 4);
+    static constexpr int kOcc = kBRegs < 128 ? 2 : 1;
+    // Rows per LDS tile: 64, or 32 when two 64-row tiles would exceed 100 KB.
+    static constexpr int kRT = 2 * 64 * (kRowBytes + 16) <= 102400 ? 64 : 32;
+    // Workgroup-prune instantiations (deep shortlists) need up to 82 KB of
+    // prune scratch, so their double-buffered tiles stay within 50 KB.
+    static constexpr int kRTDeep = 2 * 64 * (kRowBytes + 16) <= 51200
+                                       ? 64
+                                       : (2 * 32 * (kRowBytes + 16) <= 51200 ? 32 : 16);
+};
+
+// Host-side mirror of ScanShape (queries per wave and rows per tile).
+inline int scan_qw(int dt, uint32_t ks) {
+    const int lane = dt == FV_DTYPE_FP8_E4M3 ? 32 : 16;
+    return 4 * (int)ks * (lane / 4) <= 192 ? 64 : 32;
+}
+inline int scan_row_bytes(int dt, uint32_t ks) {
+    return (int)ks * 4 * (dt == FV_DTYPE_FP8_E4M3 ? 32 : 16);
+}
+
+template <typename V>
+__device__ __forceinline__ V ld_vec(const uint8_t* p) {
+    if constexpr (sizeof(V) == 16) {
+        return __builtin_bit_cast(V, *reinterpret_cast<const uint4*>(p));
+    } else {
+        struct Two {
+            uint4 a, b;
+        } t;
+        t.a = reinterpret_cast<const uint4*>(p)[0];
+        t.b = reinterpret_cast<const uint4*>(p)[1];
+        return __builtin_bit_cast(V, t);
+    }
+}
+
+// Smallest float above f (finite f): a non-strict bound tested with the same
+// strict compare as the chunk's own threshold.
+__device__ __forceinline__ float next_up(float f) {
+    uint32_t u = __float_as_uint(f);
+    if (f == 0.0f) return __uint_as_float(1u);
+    return __uint_as_float((u & 0x80000000u) ? u - 1u : u + 1u);
+}
+
+__device__ __forceinline__ float bound_from_ord(uint32_t g) {
+    if (g == 0xFFFFFFFFu) return kAdmit;
+    float f = unord_f32(g);
+    return f < kAdmit ? next_up(f) : kAdmit;
+}
+
+// Shortlist depth of one query: factor * k (or the fixed shortlist), at least
+// k, at most s_max.
+__device__ __forceinline__ void query_k_s(const FlatSearchArgs& a, uint32_t q, uint32_t& k,
+                                          uint32_t& S) {
+    k = a.d_top_k ? a.d_top_k[q] : a.top_k;
+    if (k > a.top_k) k = a.top_k;
+    if (k == 0) k = 1;
+    S = a.factor ? a.factor * k : (a.shortlist ? a.shortlist : 8u * k);
+    if (S < k) S = k;
+    if (S > a.s_max) S = a.s_max;
+}
+
+// BOUND = true is the first pass: same tiles and MMAs, but the epilogue only
+// keeps a running minimum per (lane, element) — 16 disjoint row subsets per
+// query per chunk — written to `minima`; rows are limited to the first
+// `bound_rows` positions. No thresholds, buffers or prunes.
+template <int DT, int METRIC, int KS, int R, bool BOUND>
+__global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_mfma_scan_kernel(
+    const uint8_t* __restrict__ X, uint32_t row_bytes, const float* __restrict__ norms,
+    const float* __restrict__ row_scales, const uint8_t* __restrict__ Qh,
+    const float* __restrict__ qscale, uint32_t nq, FlatSearchArgs a, uint32_t n_chunks,
+    uint32_t qblocks, uint32_t cap, const uint32_t* __restrict__ rows,
+    const uint32_t* __restrict__ nrows, uint32_t n, uint32_t* __restrict__ gthr,
+    uint32_t* __restrict__ counts, uint64_t* __restrict__ cand, float* __restrict__ minima,
+    uint32_t bound_rows) {
+    using Ops = MfmaOps<DT>;
+    using vec = typename Ops::vec;
+    using Sh = ScanShape<DT, KS>;
+    constexpr bool kFP8 = DT == FV_DTYPE_FP8_E4M3;
+    constexpr int QW = Sh::kQW;
+    constexpr int QB = Sh::kQB;
+    constexpr int NJ = QW / 16;                  // 16-query column blocks per wave
+    constexpr int RT = R == 0 ? Sh::kRTDeep : Sh::kRT;  // rows per tile
+    constexpr int LB = Sh::kLaneBytes;
+    constexpr int kRowBytes = Sh::kRowBytes;
+    constexpr int kLdsRow = kRowBytes + 16;      // padded LDS row
+    constexpr int kThrPerRow = kMfmaThreads / RT;
+    constexpr int kSegBytes = kRowBytes / kThrPerRow;  // bytes one thread stages per row
+    constexpr int NPRE = kSegBytes / 16;
+    static_assert(kSegBytes % 16 == 0, "staging segments are whole 16-byte loads");
+
+    __shared__ __attribute__((aligned(16))) uint8_t s_tile[2][RT * kLdsRow];
+    __shared__ __attribute__((aligned(16))) float s_cinit[2][RT];
+    __shared__ __attribute__((aligned(16))) float s_scale[2][RT];
+    __shared__ uint32_t s_rowid[2][RT];
+    __shared__ uint32_t s_cnt[QB];
+    __shared__ uint32_t s_keep[QB];
+    __shared__ float s_thr[QB];
+    __shared__ uint32_t s_need[QB / 32];
+    extern __shared__ uint64_t s_scratch[];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    const int l16 = lane & 15;
+    const int grp = lane >> 4;
+
+    // XCD-first work mapping: consecutive block ids land on different XCDs;
+    // within an XCD the chunk varies slowest.
+    const uint32_t b = blockIdx.x;
+    uint32_t chunk, qblock;
+    if ((n_chunks & 7u) == 0u) {
+        const uint32_t cpx = n_chunks >> 3;
+        const uint32_t xcd = b & 7u, i = b >> 3;
+        chunk = xcd * cpx + i / qblocks;
+        qblock = i % qblocks;
+    } else {
+        chunk = b / qblocks;
+        qblock = b % qblocks;
+    }
+    // Chunk bounds in the (possibly compacted) row space.
+    uint32_t n_eff = nrows ? *nrows : n;
+    if (BOUND && n_eff > bound_rows) n_eff = bound_rows;
+    uint32_t rpc = (n_eff + n_chunks - 1) / n_chunks;
+    rpc = (rpc + RT - 1) / RT * RT;
+    const uint32_t r_begin = min(n_eff, chunk * rpc);
+    const uint32_t r_end = min(n_eff, r_begin + rpc);
+    const uint32_t qb0 = qblock * QB;
+    const uint32_t qw0 = qb0 + wave * QW;
+    constexpr bool kBlockPrune = R == 0;
+    uint64_t* my_scratch = kBlockPrune ? s_scratch : s_scratch + (size_t)wave * cap;
+
+    // B fragments: lane (l16, grp) holds query qw0 + 16j + l16, bytes
+    // LB*KS*grp + LB*s .. +LB of the encoded row for step s (the same
+    // permutation of K is used for the A fragments, so every dim is paired
+    // exactly once). Queries arrive pre-encoded (negated, rows of kRowBytes,
+    // padded to whole query blocks).
+    vec bfrag[NJ][KS];
+    float qsc[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const uint8_t* qrow = Qh + (size_t)(qw0 + 16 * j + l16) * kRowBytes + LB * KS * grp;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) bfrag[j][s] = ld_vec<vec>(qrow + LB * s);
+        qsc[j] = kFP8 ? qscale[qw0 + 16 * j + l16] : 1.0f;
+    }
+    // Retire the fragment loads here: otherwise the wait-count tracking keeps
+    // them "in flight" around the tile loop and every tile waits for its own
+    // prefetches (vmcnt(0)) before its last MMAs. 0x0F70 = vmcnt(0).
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    // Per-query state; lane l < QW of wave w owns local query w*QW + l.
+    const bool owner = lane < QW;
+    const uint32_t my_ql = (uint32_t)(wave * QW + lane);
+    const uint32_t my_q = qb0 + my_ql;
+    const bool my_live = owner && my_q < nq;
+    if (owner) {
+        uint32_t k, S;
+        if (my_live) query_k_s(a, my_q, k, S);
+        else S = 1;
+        s_keep[my_ql] = S;
+        s_cnt[my_ql] = 0;
+        s_thr[my_ql] = my_live
+            ? bound_from_ord(__hip_atomic_load(&gthr[my_q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            : kAdmit;
+    }
+
+    const uint32_t ntiles = (r_end - r_begin + RT - 1) / RT;
+    const int st_row = tid / kThrPerRow;
+    const int st_seg = tid % kThrPerRow;
+    // Staging registers of the next tile. Nothing loaded here is used until
+    // store_tile, after this tile's MMAs, so the loads overlap the compute.
+    uint4 pre[NPRE];
+    uint32_t pre_row = 0, pre_mword = 0xFFFFFFFFu;
+    float pre_norm = 0.0f, pre_rs = 1.0f;
+    bool pre_in = false;
+    // Compacted row space: row ids are fetched one tile ahead of the data.
+    // All prefetch loads are unconditional from clamped, valid addresses (rows
+    // past the chunk end get C = +inf, so their data never matters): no
+    // exec-masked loads, so no conservative waits around them.
+    uint32_t rid_next = 0;
+    auto fetch_rid = [&](uint32_t p0) {
+        const uint32_t p = p0 + st_row;
+        rid_next = rows ? rows[p < r_end ? p : r_end - 1] : p;
+    };
+    auto load_tile = [&](uint32_t p0) {
+        const uint32_t p = p0 + st_row;
+        pre_in = p < r_end;
+        pre_row = pre_in ? rid_next : (rows ? rows[r_begin] : r_begin);
+        const uint8_t* src = X + (size_t)pre_row * row_bytes;
+#pragma unroll
+        for (int i = 0; i < NPRE; ++i) {
+            const uint32_t e = (uint32_t)(st_seg * kSegBytes + 16 * i);
+            if (e + 16 <= row_bytes) pre[i] = *reinterpret_cast<const uint4*>(src + e);
+            else pre[i] = make_uint4(0, 0, 0, 0);
+        }
+        if (METRIC != METRIC_INNER_PRODUCT) pre_norm = norms[pre_row];
+        if (kFP8) pre_rs = row_scales[pre_row];
+        pre_mword = (a.mask && !rows) ? a.mask[pre_row >> 5] : 0xFFFFFFFFu;
+    };
+    auto store_tile = [&](int buf) {
+        uint8_t* dst = &s_tile[buf][st_row * kLdsRow + st_seg * kSegBytes];
+#pragma unroll
+        for (int i = 0; i < NPRE; ++i) *reinterpret_cast<uint4*>(dst + 16 * i) = pre[i];
+        if (st_seg == 0) {
+            const bool ok = pre_in && ((pre_mword >> (pre_row & 31u)) & 1u);
+            float c = __builtin_inff(), sc = 1.0f;
+            if (ok) {
+                if (METRIC == METRIC_L2) c = 0.5f * pre_norm;
+                else c = 0.0f;
+                if (METRIC == METRIC_COSINE) sc = pre_norm > 0.0f ? 1.0f / sqrtf(pre_norm) : 0.0f;
+                if (kFP8) sc = sc * pre_rs;
+            }
+            s_cinit[buf][st_row] = c;
+            s_scale[buf][st_row] = sc;
+            s_rowid[buf][st_row] = pre_row;
+        }
+    };
+
+    // Wave-private staging list of survivors, flushed into the per-query
+    // buffers by all 64 lanes at once.
+    // Wave mode: the staging list aliases the wave's prune scratch (never live
+    // at the same time). Workgroup mode: it follows the shared scratch.
+    uint64_t* st_key = kBlockPrune ? s_scratch + cap + (size_t)wave * (kStage + kStage / 8)
+                                   : my_scratch;
+    uint8_t* st_q = reinterpret_cast<uint8_t*>(st_key + kStage);
+    uint32_t nstage = 0;
+    auto flush_stage = [&]() {
+        wave_lds_sync();
+        for (uint32_t e = lane; e < nstage; e += kWave) {
+            const uint32_t ql = wave * QW + st_q[e];
+            const uint32_t pos = atomicAdd(&s_cnt[ql], 1u);
+            cand[((size_t)chunk * nq + qb0 + ql) * cap + pos] = st_key[e];
+        }
+        nstage = 0;
+        wave_lds_sync();
+    };
+
+    if (ntiles > 0) {
+        fetch_rid(r_begin);
+        load_tile(r_begin);
+        if (ntiles > 1) fetch_rid(r_begin + RT);
+        store_tile(0);
+    }
+    __syncthreads();
+
+    floatx4 runmin[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) runmin[j] = floatx4{kAdmit, kAdmit, kAdmit, kAdmit};
+
+    for (uint32_t t = 0; t < ntiles; ++t) {
+        const int buf = t & 1;
+        const uint32_t p0 = r_begin + t * RT;
+        if (t + 1 < ntiles) {
+            load_tile(p0 + RT);
+            if (t + 2 < ntiles) fetch_rid(p0 + 2 * RT);
+        }
+        // Prefetch the shared bound; it is consumed after this tile's MMAs.
+        const uint32_t gpre =
+            __hip_atomic_load(&gthr[my_live ? my_q : 0u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+
+        float thr[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) thr[j] = s_thr[wave * QW + 16 * j + l16];
+
+#pragma unroll
+        for (int rt = 0; rt < RT / 16; ++rt) {
+            const uint8_t* arow = &s_tile[buf][(rt * 16 + l16) * kLdsRow + grp * (LB * KS)];
+            vec afrag[KS];
+#pragma unroll
+            for (int s = 0; s < KS; ++s) afrag[s] = ld_vec<vec>(arow + LB * s);
+            const floatx4 cin = *reinterpret_cast<const floatx4*>(&s_cinit[buf][rt * 16 + 4 * grp]);
+            floatx4 acc[NJ];
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) acc[j] = kFP8 ? floatx4{0.0f, 0.0f, 0.0f, 0.0f} : cin;
+            // NJ independent accumulation chains, interleaved.
+#pragma unroll
+            for (int s = 0; s < KS; ++s)
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) acc[j] = Ops::mma(afrag[s], bfrag[j][s], acc[j]);
+            if (kFP8 || METRIC == METRIC_COSINE) {
+                const floatx4 scl = *reinterpret_cast<const floatx4*>(&s_scale[buf][rt * 16 + 4 * grp]);
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) {
+                    if (kFP8) acc[j] = cin + acc[j] * (scl * qsc[j]);
+                    else acc[j] = acc[j] * scl;
+                }
+            }
+            if constexpr (BOUND) {
+#pragma unroll
+                for (int j = 0; j < NJ; ++j)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) runmin[j][i] = fminf(runmin[j][i], acc[j][i]);
+                continue;
+            }
+            // Survivors are compacted wave-wide into the staging list (ballot +
+            // mbcnt), so a slot costs one compare unless some lane inserts.
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                const bool live_q = qw0 + 16 * j + l16 < nq;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const bool f = live_q && acc[j][i] < thr[j];
+                    const uint64_t bm = __ballot(f);
+                    if (bm) {
+                        const uint32_t idx = nstage + __builtin_amdgcn_mbcnt_hi(
+                            (uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
+                        if (f) {
+                            st_key[idx] = make_key(acc[j][i], s_rowid[buf][rt * 16 + 4 * grp + i]);
+                            st_q[idx] = (uint8_t)(16 * j + l16);
+                        }
+                        nstage += (uint32_t)__popcll(bm);
+                    }
+                }
+                // One query column block stages at most 4 * 64 survivors.
+                if (nstage > kStage - 4 * kWave) flush_stage();
+            }
+        }
+        if (!BOUND) flush_stage();
+        if (t + 1 < ntiles) store_tile(buf ^ 1);
+        __syncthreads();
+        if (BOUND) continue;
+
+        if (!kBlockPrune) {
+            // Wave-private prunes of buffers that could overflow on the next tile.
+            const uint32_t c = my_live ? s_cnt[my_ql] : 0u;
+            uint64_t need = __ballot(my_live && c + RT > cap);
+            while (need) {
+                const int bl = __builtin_ctzll(need);
+                need &= need - 1;
+                const uint32_t qlb = wave * QW + bl;
+                const uint32_t qgb = qb0 + qlb;
+                const uint32_t keep = s_keep[qlb];
+                uint64_t* bq = cand + ((size_t)chunk * nq + qgb) * cap;
+                uint32_t th;
+                uint32_t kept = wave_bisect_prune<(R > 0 ? R : 1)>(bq, s_cnt[qlb], keep, lane, &th);
+                if (kept + RT > cap) {
+                    // Heavy ties at the bound: exact (distance, row) selection.
+                    kept = wave_prune(bq, kept, keep, my_scratch, lane);
+                    th = kept == keep ? (uint32_t)(my_scratch[keep - 1] >> 32) : 0xFFFFFFFFu;
+                }
+                if (lane == 0) {
+                    s_cnt[qlb] = kept;
+                    if (th != 0xFFFFFFFFu) {
+                        s_thr[qlb] = fminf(s_thr[qlb], unord_f32(th));
+                        atomicMin(&gthr[qgb], th);
+                    }
+                }
+                wave_lds_sync();
+            }
+        } else {
+            // Workgroup prunes, one query at a time, through shared scratch.
+            if (tid < QB / 32) s_need[tid] = 0;
+            __syncthreads();
+            if (my_live && s_cnt[my_ql] + RT > cap) atomicOr(&s_need[my_ql >> 5], 1u << (my_ql & 31));
+            __syncthreads();
+            for (int w = 0; w < QB / 32; ++w) {
+                uint32_t bits = s_need[w];
+                while (bits) {
+                    const int bl = __builtin_ctz(bits);
+                    bits &= bits - 1;
+                    const uint32_t ql = w * 32 + bl;
+                    const uint32_t qg = qb0 + ql;
+                    const uint32_t c = s_cnt[ql];
+                    const uint32_t keep = s_keep[ql];
+                    uint64_t* bq = cand + ((size_t)chunk * nq + qg) * cap;
+                    const uint32_t np = next_pow2(c < 2u ? 2u : c);
+                    for (uint32_t i = tid; i < np; i += kMfmaThreads) my_scratch[i] = i < c ? bq[i] : kEmptyKey;
+                    __syncthreads();
+                    block_bitonic_sort(my_scratch, np);
+                    const uint32_t kept = c < keep ? c : keep;
+                    for (uint32_t i = tid; i < kept; i += kMfmaThreads) bq[i] = my_scratch[i];
+                    if (tid == 0) {
+                        s_cnt[ql] = kept;
+                        if (kept == keep) {
+                            const float th = key_dist(my_scratch[keep - 1]);
+                            s_thr[ql] = fminf(s_thr[ql], th);
+                            atomicMin(&gthr[qg], ord_f32(th));
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+        // Tighten thresholds with the prefetched shared bound. Each wave only
+        // reads its own queries' thresholds, so a wave-level sync suffices.
+        if (my_live) s_thr[my_ql] = fminf(s_thr[my_ql], bound_from_ord(gpre));
+        wave_lds_sync();
+    }
+    if (BOUND) {
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const uint32_t qg = qw0 + 16 * j + l16;
+            if (qg < nq) {
+                float* mo = minima + ((size_t)chunk * nq + qg) * 16 + 4 * grp;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) mo[i] = runmin[j][i];
+            }
+        }
+        return;
+    }
+    if (my_live) counts[(size_t)chunk * nq + my_q] = s_cnt[my_ql];
+}
+
+// Static LDS of one main-pass instantiation (queried once); with the plan's
+// dynamic LDS it must fit the 160 KB of a CU.
+template <int DT, int METRIC, int KS, int R>
+static size_t scan_static_lds() {
+    static const size_t v = [] {
+        hipFuncAttributes attr{};
+        const void* f = reinterpret_cast<const void*>(flat_mfma_scan_kernel<DT, METRIC, KS, R, false>);
+        return hipFuncGetAttributes(&attr, f) == hipSuccess ? attr.sharedSizeBytes : (size_t)1 << 30;
+    }();
+    return v;
+}
+
+template <int DT, int METRIC, int KS, int R>
+static hipError_t launch_scan_r(const ScanLaunch& L) {
+    const FlatMfmaPlan& p = *L.p;
+    const dim3 g(p.qblocks * p.n_chunks);
+    flat_mfma_scan_kernel<DT, METRIC, KS, R, false><<<g, kMfmaThreads, p.lds_scan, L.stream>>>(
+        L.X, L.row_bytes, L.norms, L.row_scales, L.Qh, L.qscale, L.nq, L.a, p.n_chunks, p.qblocks,
+        p.cap, L.rows, L.nrows, L.n, L.gthr, L.counts, L.cand, nullptr, 0u);
+    return hipSuccess;
+}
+
+template <int DT, int METRIC, int KS>
+static hipError_t launch_scan(const ScanLaunch& L) {
+    const FlatMfmaPlan& p = *L.p;
+    const size_t st = p.prune_regs == 16 ? scan_static_lds<DT, METRIC, KS, 16>()
+                                         : scan_static_lds<DT, METRIC, KS, 0>();
+    if (st + p.lds_scan > 160u * 1024u) return hipErrorInvalidValue;
+    if (p.bound_chunks) {
+        const dim3 g(p.qblocks * p.bound_chunks);
+        flat_mfma_scan_kernel<DT, METRIC, KS, 16, true><<<g, kMfmaThreads, 0, L.stream>>>(
+            L.X, L.row_bytes, L.norms, L.row_scales, L.Qh, L.qscale, L.nq, L.a, p.bound_chunks,
+            p.qblocks, p.cap, L.rows, L.nrows, L.n, L.gthr, L.counts, L.cand, L.minima,
+            p.bound_rows);
+        launch_bound_select(L);
+    }
+    // The kernel timer brackets the main pass only.
+    void* timer = timer_begin(L.stream);
+    const hipError_t e = p.prune_regs == 16 ? launch_scan_r<DT, METRIC, KS, 16>(L)
+                                            : launch_scan_r<DT, METRIC, KS, 0>(L);
+    timer_end(timer, L.stream);
+    return e;
+}
+
+template <int DT, int KS>
+static hipError_t launch_scan_metric(const ScanLaunch& L) {
+    if (L.a.metric == METRIC_L2) return launch_scan<DT, METRIC_L2, KS>(L);
+    if (L.a.metric == METRIC_INNER_PRODUCT) return launch_scan<DT, METRIC_INNER_PRODUCT, KS>(L);
+    return launch_scan<DT, METRIC_COSINE, KS>(L);
+}
+
+}  // namespace fv

@@ -79,7 +79,7 @@ class FvFlatSearchParams(ctypes.Structure):
         ("top_k", ctypes.c_uint32),
         ("shortlist_factor", ctypes.c_uint32),
         ("shortlist", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
     ]
 
 

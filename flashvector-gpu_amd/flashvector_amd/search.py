@@ -92,11 +92,15 @@ def device_count() -> int:
     return c.value
 
 
+FV_SEARCH_F64_BOUNDARY = 1
+
+
 class FlatIndex:
     """Device-resident flat tables for the matrix-core scan (fv_flat_search).
 
-    Holds the FP32 rows (for the exact rerank), their 16-bit codes padded to a
-    multiple of 32 dims, FP32 and FP64 norms, all encoded on the GPU.
+    Holds the FP32 rows (for the exact rerank), their codes (FP16/BF16 padded
+    to a multiple of 32 dims, or FP8 e4m3 with per-row scales padded to a
+    multiple of 128), FP32 and FP64 norms, all encoded on the GPU.
     """
 
     def __init__(self, vectors, dtype="f16", keep_fp32=True, padded_dim=None):
@@ -106,41 +110,46 @@ class FlatIndex:
         n, dim = self.vectors.shape
         self.dim, self.n = dim, n
         self.dtype = dtype
-        pd = padded_dim or (dim + 31) // 32 * 32
+        fp8 = dtype == "fp8"
+        step = 128 if fp8 else 32
+        pd = padded_dim or (dim + step - 1) // step * step
         self.padded_dim = pd
         dev = self.vectors.device
-        self.codes = torch.empty((n, pd), dtype=torch.int16, device=dev)
+        self.codes = torch.empty((n, pd), dtype=torch.uint8 if fp8 else torch.int16, device=dev)
+        self.row_scales = torch.empty(n, dtype=torch.float32, device=dev) if fp8 else None
         self.norms = torch.empty(n, dtype=torch.float32, device=dev)
         self.norms64 = torch.empty(n, dtype=torch.float64, device=dev)
         L.check(lib.fv_encode_flat(L.ptr(self.vectors), n, dim, pd, L.DTYPES[dtype],
                                    L.ptr(self.codes), L.ptr(self.norms), L.ptr(self.norms64),
-                                   None, L.stream_handle()), "fv_encode_flat")
+                                   L.ptr(self.row_scales), L.stream_handle()), "fv_encode_flat")
         self.keep_fp32 = keep_fp32
-        self.tables = L.FvFlatTables(L.ptr(self.codes), None, L.ptr(self.norms),
+        self.tables = L.FvFlatTables(L.ptr(self.codes), L.ptr(self.row_scales), L.ptr(self.norms),
                                      L.ptr(self.norms64),
                                      L.ptr(self.vectors) if keep_fp32 else None,
                                      n, dim, pd, L.DTYPES[dtype])
         self._ws = None
 
     def params(self, top_k, metric="l2", valid_mask=None, valid_count=0, shortlist=0,
-               shortlist_factor=0, top_k_per_query=None):
+               shortlist_factor=0, top_k_per_query=None, f64_boundary=False):
         return L.FvFlatSearchParams(L.ptr(valid_mask), L.ptr(top_k_per_query), int(valid_count),
                                     _metric(metric), int(top_k), int(shortlist_factor),
-                                    int(shortlist), 0)
+                                    int(shortlist),
+                                    FV_SEARCH_F64_BOUNDARY if f64_boundary else 0)
 
     def workspace_size(self, nq, params):
         return L.load().fv_flat_workspace_size(ctypes.byref(self.tables), nq, ctypes.byref(params))
 
     def search(self, queries, top_k, metric="l2", valid_mask=None, shortlist=0, out=None,
-               valid_count=0, shortlist_factor=0, top_k_per_query=None):
+               valid_count=0, shortlist_factor=0, top_k_per_query=None, f64_boundary=False):
         """fv_flat_search. `valid_count` (set bits of the mask, if known) lets
         sparse masks be compacted; `top_k_per_query` is an optional int32
-        device tensor of per-query k (each <= top_k)."""
+        device tensor of per-query k (each <= top_k); `f64_boundary` sets
+        FV_SEARCH_F64_BOUNDARY."""
         torch = L._torch()
         lib = L.load()
         nq = queries.shape[0]
         prm = self.params(top_k, metric, valid_mask, valid_count, shortlist, shortlist_factor,
-                          top_k_per_query)
+                          top_k_per_query, f64_boundary)
         need = self.workspace_size(nq, prm)
         if need == 0:
             raise L.FvError("invalid flat search shape")

@@ -31,12 +31,14 @@ int fv_flat_exact_search(const float* d_vectors, uint32_t num_vectors, uint32_t 
                          size_t workspace_bytes, void* stream);
 
 /* ---- reduced-precision flat scan on the matrix cores --------------------
- * Rows are stored as 16-bit codes (FP16 or BF16, RNE from FP32) padded to
- * padded_dim (a multiple of 16, at most 256). The scan scores every row on
- * MFMA with FP32 accumulation, keeps a shortlist of `shortlist` rows per query
- * (0 = 8 * top_k), reranks it exactly in FP32 against d_vectors (the reference
- * CPU path's arithmetic) and returns the top_k by (distance, row). With
- * d_vectors NULL the reduced-precision scores are returned instead. */
+ * Rows are stored as 16-bit codes (FP16 or BF16, RNE from FP32) or as FP8
+ * e4m3 codes with one FP32 scale per row (s = max|x|/448, code =
+ * RNE-saturating e4m3(x/s)), padded with zeros to padded_dim (a multiple of
+ * 16, at most 768). The scan scores every row on MFMA with FP32 accumulation
+ * (queries are encoded the same way per call, FP8 with one scale per query),
+ * keeps a shortlist per query, reranks it exactly in FP32 against d_vectors
+ * (the reference CPU path's arithmetic) and returns the top_k by (distance,
+ * row). With d_vectors NULL the reduced-precision scores are returned. */
 typedef enum {
     FV_DTYPE_F32 = 0,
     FV_DTYPE_F16 = 1,
@@ -57,8 +59,8 @@ typedef struct {
 } FvFlatTables;
 
 /* Encodes FP32 rows into FvFlatTables storage on the device: codes (padded
- * with zeros), FP32 norms as sequential non-fused sums, FP64 norms. Any output
- * pointer may be NULL to skip it. */
+ * with zeros; FP8 also writes d_row_scales_out), FP32 norms as sequential
+ * non-fused sums, FP64 norms. Any output pointer may be NULL to skip it. */
 int fv_encode_flat(const float* d_vectors, uint32_t num_vectors, uint32_t dim,
                    uint32_t padded_dim, uint32_t dtype, void* d_codes_out,
                    float* d_norms_out, double* d_norms_f64_out, float* d_row_scales_out,
@@ -68,7 +70,13 @@ int fv_encode_flat(const float* d_vectors, uint32_t num_vectors, uint32_t dim,
  * when shortlist_factor != 0, else `shortlist` (0 means 8 * k_i), never below
  * k_i; at most 7936 rows. valid_count is the number of set bits in the mask
  * when the caller knows it (0 = unknown): a mask keeping at most half of the
- * rows is then compacted on the device and only the valid rows are scored. */
+ * rows is then compacted on the device and only the valid rows are scored.
+ * FV_SEARCH_F64_BOUNDARY (needs d_vectors): reranked candidates whose FP32
+ * distances are within twice their rounding-error bound of a neighbour are
+ * re-scored in FP64 and ordered by (FP64 distance, row), so membership and
+ * order of the returned top k follow the exact distances; those rows report
+ * the FP64 distance rounded to FP32. */
+#define FV_SEARCH_F64_BOUNDARY 1u
 typedef struct {
     const uint32_t* d_valid_mask; /* LSB-first u32 words, NULL = every row */
     const uint32_t* d_top_k;      /* per-query k (each <= top_k), or NULL */
@@ -77,7 +85,7 @@ typedef struct {
     uint32_t top_k;               /* uniform k and output row stride, <= 1024 */
     uint32_t shortlist_factor;
     uint32_t shortlist;
-    uint32_t reserved;            /* must be 0 */
+    uint32_t flags;               /* FV_SEARCH_* bits, others must be 0 */
 } FvFlatSearchParams;
 
 size_t fv_flat_workspace_size(const FvFlatTables* tables, uint32_t num_queries,

@@ -461,3 +461,111 @@ uint32_t fvo_encode_fp16_padded(const float* vectors, uint32_t n, uint32_t dim,
     }
     return pd;
 }
+
+/* ------------------------------------------------------------------------ */
+/* FP8 e4m3 and the FP64 top-k                                               */
+/* ------------------------------------------------------------------------ */
+
+uint8_t fvo_f32_to_e4m3(float x) {
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    const uint8_t sign = (uint8_t)((u >> 24) & 0x80u);
+    u &= 0x7fffffffu;
+    if (u > 0x7f800000u) return (uint8_t)(sign | 0x7fu);
+    float a;
+    memcpy(&a, &u, 4);
+    if (a >= 448.0f) return (uint8_t)(sign | 0x7eu);
+    const int e = (int)(u >> 23) - 127;
+    if (e < -6) return (uint8_t)(sign | (uint32_t)rintf(a * 512.0f)); /* step 2^-9 */
+    const uint32_t mant = u & 0x7fffffu;
+    uint32_t r = mant >> 20;
+    const uint32_t rem = mant & 0xfffffu;
+    if (rem > 0x80000u || (rem == 0x80000u && (r & 1u))) ++r;
+    int ef = e + 7;
+    if (r == 8u) {
+        r = 0u;
+        ++ef;
+    }
+    if (ef > 15 || (ef == 15 && r == 7u)) return (uint8_t)(sign | 0x7eu);
+    return (uint8_t)(sign | (uint32_t)(ef << 3) | r);
+}
+
+float fvo_e4m3_to_f32(uint8_t c) {
+    const uint32_t e = (c >> 3) & 15u, m = c & 7u;
+    float v;
+    if (e == 15u && m == 7u) v = NAN;
+    else if (e == 0u) v = (float)m / 512.0f;
+    else v = ldexpf(1.0f + (float)m / 8.0f, (int)e - 7);
+    return (c & 0x80u) ? -v : v;
+}
+
+void fvo_encode_fp8_rows(const float* x, uint32_t n, uint32_t dim, uint32_t pdim, float sign,
+                         uint8_t* codes, float* scales) {
+    for (uint32_t r = 0; r < n; ++r) {
+        const float* row = x + (size_t)r * dim;
+        float m = 0.0f;
+        for (uint32_t d = 0; d < dim; ++d) m = fmaxf(m, fabsf(row[d]));
+        const float s = m > 0.0f ? m / 448.0f : 1.0f;
+        for (uint32_t d = 0; d < pdim; ++d)
+            codes[(size_t)r * pdim + d] = d < dim ? fvo_f32_to_e4m3(sign * row[d] / s) : 0u;
+        if (scales) scales[r] = s;
+    }
+}
+
+typedef struct {
+    double d;
+    uint32_t id;
+} D64Res;
+
+static int cmp_d64(const void* a, const void* b) {
+    const D64Res* x = (const D64Res*)a;
+    const D64Res* y = (const D64Res*)b;
+    if (x->d < y->d) return -1;
+    if (x->d > y->d) return 1;
+    return x->id < y->id ? -1 : (x->id > y->id ? 1 : 0);
+}
+
+static double dist_f64(const float* q, const float* x, uint32_t dim, int metric) {
+    double a = 0.0, nq = 0.0, nx = 0.0;
+    for (uint32_t d = 0; d < dim; ++d) {
+        const double qd = q[d], xd = x[d];
+        if (metric == METRIC_L2) {
+            const double t = qd - xd;
+            a = fma(t, t, a);
+        } else {
+            a = fma(qd, xd, a);
+            nq = fma(qd, qd, nq);
+            nx = fma(xd, xd, nx);
+        }
+    }
+    if (metric == METRIC_L2) return a;
+    if (metric == METRIC_INNER_PRODUCT) return -a;
+    return 1.0 - a / (sqrt(nq) * sqrt(nx) + 1e-8);
+}
+
+int fvo_flat_topk_f64(const float* vectors, uint32_t n, uint32_t dim, const float* queries,
+                      uint32_t nq, uint32_t top_k, int metric, const uint32_t* valid_mask,
+                      uint32_t* out_ids, double* out_dist, int nthreads) {
+    if (!vectors || !queries || !out_ids || !out_dist) return -1;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (uint32_t i = 0; i < nq; ++i) {
+        D64Res* all = (D64Res*)malloc(sizeof(D64Res) * (n ? n : 1));
+        uint32_t c = 0;
+        for (uint32_t r = 0; r < n; ++r) {
+            if (valid_mask && !((valid_mask[r >> 5] >> (r & 31u)) & 1u)) continue;
+            all[c].d = dist_f64(queries + (size_t)i * dim, vectors + (size_t)r * dim, dim, metric);
+            all[c].id = r;
+            ++c;
+        }
+        qsort(all, c, sizeof(D64Res), cmp_d64);
+        for (uint32_t j = 0; j < top_k; ++j) {
+            out_ids[(size_t)i * top_k + j] = j < c ? all[j].id : 0xFFFFFFFFu;
+            out_dist[(size_t)i * top_k + j] = j < c ? all[j].d : INFINITY;
+        }
+        free(all);
+    }
+    return 0;
+}

@@ -28,6 +28,8 @@
  * (crates/engine/src/quant.rs:362-368) plus an exhaustive binary16 sweep
  * against numpy, and IVF-PQ invariants (exhaustive probing equals the flat
  * scan; integer codebooks make ADC equal a decode-and-brute-force).
+ * FP8 e4m3 conversion is pinned exhaustively against torch's float8_e4m3fn
+ * and a brute-force nearest-value search; the FP64 top-k against numpy.
  * Functions without such a check are marked "parity unpinned" below.
  */
 #ifndef FV_ORACLE_H
@@ -122,6 +124,25 @@ uint32_t fvo_encode_fp16_padded(const float* vectors, uint32_t n, uint32_t dim,
  * the dense [nq x n] matrix of fvo_distance. */
 void fvo_distance_matrix(const float* queries, const float* dataset, uint32_t nq,
                          uint32_t n, uint32_t dim, int metric, float* out);
+
+/* OCP FP8 E4M3 (bias 7, no infinities, largest finite 448): round to nearest
+ * even, saturating to +-448 (NaN -> 0x7f), and its exact decode. The format
+ * is the one of v_mfma_f32_16x16x128_f8f6f4 with format code 0. */
+uint8_t fvo_f32_to_e4m3(float x);
+float fvo_e4m3_to_f32(uint8_t c);
+
+/* FP8 rows with one FP32 scale per row: s = max|x| / 448 (1 for an all-zero
+ * row), code = e4m3(sign * x / s), zero padded to pdim codes. */
+void fvo_encode_fp8_rows(const float* x, uint32_t n, uint32_t dim, uint32_t pdim, float sign,
+                         uint8_t* codes, float* scales);
+
+/* Exact top-k with distances evaluated in FP64 (float products are exact in
+ * FP64; L2 = sum (q-x)^2, IP = -sum q*x, cosine = 1 - dot/(|q||x| + 1e-8)),
+ * ties to the lower id. The FP64 oracle of recall and distance-error reports
+ * and of the FP64-boundary search mode. */
+int fvo_flat_topk_f64(const float* vectors, uint32_t n, uint32_t dim, const float* queries,
+                      uint32_t nq, uint32_t top_k, int metric, const uint32_t* valid_mask,
+                      uint32_t* out_ids, double* out_dist, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -76,6 +76,12 @@ def lib():
         L.fvo_encode_fp16_padded.argtypes = [P, U, U, P, P]
         L.fvo_encode_fp16_padded.restype = U
         L.fvo_distance_matrix.argtypes = [P, P, U, U, U, I, P]
+        L.fvo_f32_to_e4m3.argtypes = [ctypes.c_float]
+        L.fvo_f32_to_e4m3.restype = ctypes.c_uint8
+        L.fvo_e4m3_to_f32.argtypes = [ctypes.c_uint8]
+        L.fvo_e4m3_to_f32.restype = ctypes.c_float
+        L.fvo_encode_fp8_rows.argtypes = [P, U, U, U, ctypes.c_float, P, P]
+        L.fvo_flat_topk_f64.argtypes = [P, U, U, P, U, U, I, P, P, P, I]
         _lib = L
     return _lib
 
@@ -230,3 +236,61 @@ def encode_fp16_padded(vectors):
     norms = np.zeros(n, np.float32)
     lib().fvo_encode_fp16_padded(_p(x), n, d, _p(out), _p(norms))
     return out, norms
+
+
+def f32_to_e4m3(x: float) -> int:
+    return int(lib().fvo_f32_to_e4m3(float(x)))
+
+
+def e4m3_to_f32(c: int) -> float:
+    return float(lib().fvo_e4m3_to_f32(int(c)))
+
+
+def e4m3_table() -> np.ndarray:
+    """Decoded value of each of the 256 e4m3 codes (NaN for 0x7f/0xff)."""
+    return np.array([e4m3_to_f32(c) for c in range(256)], np.float32)
+
+
+def encode_fp8_rows(x, pdim=None, sign=1.0):
+    """FP8 rows with per-row scales (fvo_encode_fp8_rows): (codes u8, scales f32)."""
+    x = _f32(x)
+    n, d = x.shape
+    pdim = d if pdim is None else pdim
+    codes = np.zeros((n, pdim), np.uint8)
+    scales = np.zeros(n, np.float32)
+    lib().fvo_encode_fp8_rows(_p(x), n, d, pdim, float(sign), _p(codes), _p(scales))
+    return codes, scales
+
+
+def fp8_scores(db, queries, metric="l2"):
+    """Scores of the FP8 scan without rerank: c_row + (s_row * s_q) * acc,
+    acc = the exact sum of code products (exact in FP64) rounded to FP32;
+    c_row = 0.5|x|^2 (L2) else 0; cosine multiplies s_row by 1/|x|. Matches
+    the device bit for bit whenever acc is exact in FP32."""
+    t = e4m3_table().astype(np.float64)
+    xc, xs = encode_fp8_rows(db)
+    qc, qs = encode_fp8_rows(queries, sign=-1.0)
+    acc = (t[qc] @ t[xc].T).astype(np.float32)
+    x32 = _f32(db)
+    norms = np.array([distance(np.zeros_like(r), r, "l2") for r in x32], np.float32)
+    sx = xs.copy()
+    if metric == "cosine":
+        inv = np.where(norms > 0, (np.float32(1.0) / np.sqrt(norms)).astype(np.float32), np.float32(0))
+        sx = (inv * xs).astype(np.float32)
+    c = (np.float32(0.5) * norms) if metric == "l2" else np.zeros_like(norms)
+    scale = (sx[None, :] * qs[:, None]).astype(np.float32)
+    return (c[None, :] + acc * scale).astype(np.float32)
+
+
+def flat_topk_f64(vectors, queries, top_k, metric="l2", valid_mask=None, nthreads=0):
+    """Exact top-k by FP64 distance, ties to the lower id: (ids u32, dist f64)."""
+    v, q = _f32(vectors), _f32(queries)
+    n, d = v.shape
+    nq = q.shape[0]
+    ids = np.zeros((nq, top_k), np.uint32)
+    dist = np.zeros((nq, top_k), np.float64)
+    m = _u32(valid_mask)
+    rc = lib().fvo_flat_topk_f64(_p(v), n, d, _p(q), nq, top_k, METRICS[metric], _p(m),
+                                 _p(ids), _p(dist), nthreads)
+    assert rc == 0
+    return ids, dist

@@ -1,0 +1,184 @@
+"""GPU parity of the FP8 e4m3 matrix-core scan, of the 16-bit scan at large
+dims (32 queries per wave) and of the FP64 boundary re-scoring.
+
+FP8 rows with integer entries in [-14, 14] and one |x| = 14 per row have
+scale 1/32 and encode exactly, so their MFMA scores are exact and the
+reranked output must equal the oracle's exact FP32 top-k bit for bit. On
+real-valued unit vectors the bar is recall against the FP64 oracle (FP8
+keeps 3 mantissa bits) plus exact FP32 distances for every returned row."""
+import numpy as np
+import pytest
+
+import oracle as O
+from flashvector_amd import lib as L
+from flashvector_amd import search as F
+from flashvector_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    assert t.cuda.is_available(), "GPU tests need a ROCm device"
+    L.load()
+    O.build()
+    return t
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def assert_same(got, exp):
+    np.testing.assert_array_equal(got[0], exp[0])
+    np.testing.assert_array_equal(got[1].view(np.uint32), exp[1].view(np.uint32))
+
+
+def int14(n, dim, seed, sign=1.0):
+    """Integer rows in [-14, 14] with a +-14 in every row (exact FP8 at scale 1/32)."""
+    rng = np.random.default_rng(seed)
+    x = rng.integers(-14, 15, size=(n, dim)).astype(np.float32)
+    x[np.arange(n), rng.integers(0, dim, size=n)] = 14.0 * sign
+    return x
+
+
+def search(torch, db, qs, k, metric, dtype="fp8", keep_fp32=True, **kw):
+    idx = F.FlatIndex(dev(torch, db), dtype, keep_fp32=keep_fp32)
+    out = idx.search(dev(torch, qs), k, metric, **kw)
+    torch.cuda.synchronize()
+    return L.split_results(out)
+
+
+def test_fp8_encoder_matches_oracle(torch):
+    rng = np.random.default_rng(1)
+    x = (rng.normal(size=(300, 200)) * rng.uniform(0.01, 50, size=(300, 1))).astype(np.float32)
+    x[7] = 0.0
+    idx = F.FlatIndex(dev(torch, x), "fp8")
+    torch.cuda.synchronize()
+    codes, scales = O.encode_fp8_rows(x, pdim=idx.padded_dim)
+    np.testing.assert_array_equal(idx.codes.cpu().numpy(), codes)
+    np.testing.assert_array_equal(idx.row_scales.cpu().numpy().view(np.uint32), scales.view(np.uint32))
+
+
+@pytest.mark.parametrize("n,nq,dim,k,metric", [
+    (6000, 70, 64, 10, "l2"),       # one K step of 128 (padded)
+    (6000, 70, 200, 10, "ip"),      # two K steps
+    (5000, 40, 384, 25, "l2"),      # four K steps
+    (5000, 300, 768, 10, "ip"),     # six K steps, several query blocks
+    (5000, 33, 768, 100, "l2"),     # deep shortlist
+])
+def test_fp8_integer_data_exact(torch, n, nq, dim, k, metric):
+    db = int14(n, dim, n + dim)
+    qs = int14(nq, dim, nq + 7)
+    assert_same(search(torch, db, qs, k, metric), O.flat_topk(db, qs, k, metric))
+
+
+def test_fp8_scores_without_rerank(torch):
+    """keep_fp32=False returns the FP8 scores: equal to the oracle's
+    c_row + (s_row * s_q) * acc on exactly representable data."""
+    db = int14(500, 256, 3)
+    qs = int14(20, 256, 4)
+    ids, d = search(torch, db, qs, 500, "ip", keep_fp32=False)
+    s = O.fp8_scores(db, qs, "ip")
+    for i in range(20):
+        order = np.lexsort((np.arange(500), s[i]))
+        np.testing.assert_array_equal(ids[i], order)
+        np.testing.assert_array_equal(d[i], s[i][order])
+
+
+def test_fp8_mixed_k_masked(torch):
+    db = int14(20000, 768, 11)
+    qs = int14(260, 768, 12)
+    ks = S.log_uniform_k(260, 128, 13)
+    mask = S.random_valid_mask(20000, 0.5, 14)
+    idx = F.FlatIndex(dev(torch, db), "fp8")
+    out = idx.search(dev(torch, qs), 128, "ip", dev(torch, mask), shortlist_factor=4,
+                     top_k_per_query=dev(torch, ks))
+    torch.cuda.synchronize()
+    ids, d = L.split_results(out)
+    e_ids, e_d = O.flat_topk(db, qs, 128, "ip", valid_mask=mask)
+    for i in range(260):
+        k = ks[i]
+        np.testing.assert_array_equal(ids[i, :k], e_ids[i, :k])
+        np.testing.assert_array_equal(d[i, :k].view(np.uint32), e_d[i, :k].view(np.uint32))
+        assert (ids[i, k:] == 0xFFFFFFFF).all()
+
+
+def test_fp8_vmf_recall_and_exact_distances(torch):
+    """Unit vMF rows at d=768: every returned distance is the exact FP32
+    distance of its row; recall@10 against the FP64 oracle with an 8x
+    shortlist is high (FP8 noise only reorders near-ties)."""
+    db, qs = S.vmf_workload(30000, 100, 768, 64, 50.0, device="cuda")
+    dbn, qsn = db.cpu().numpy(), qs.cpu().numpy()
+    idx = F.FlatIndex(db, "fp8")
+    out = idx.search(qs, 10, "ip", shortlist_factor=8)
+    torch.cuda.synchronize()
+    ids, d = L.split_results(out)
+    e_ids, _ = O.flat_topk_f64(dbn, qsn, 10, "ip")
+    hits = sum(len(set(ids[i]) & set(e_ids[i])) for i in range(100))
+    assert hits / 1000 >= 0.97
+    exact = O.distance_matrix(qsn, dbn, "ip")
+    np.testing.assert_array_equal(d, np.take_along_axis(exact, ids.astype(np.int64), 1))
+
+
+@pytest.mark.parametrize("dtype", ["f16", "bf16"])
+@pytest.mark.parametrize("dim", [384, 500, 768])
+def test_16bit_large_dims_exact(torch, dtype, dim):
+    """Widths of 12, 16 and 24 K steps (32 queries per wave)."""
+    rng = np.random.default_rng(dim)
+    db = rng.integers(0, 32, size=(4000, dim)).astype(np.float32)
+    qs = rng.integers(0, 32, size=(150, dim)).astype(np.float32)
+    assert_same(search(torch, db, qs, 20, "l2", dtype), O.flat_topk(db, qs, 20, "l2"))
+
+
+def _tie_case(metric, dim=64):
+    """Rows 0 and 1 tie in FP32 but not in exact arithmetic (row 1 is nearer),
+    among 2000 far-away rows."""
+    rng = np.random.default_rng(7)
+    db = (rng.normal(size=(2002, dim)) * 1e-3).astype(np.float32)
+    q = np.zeros((1, dim), np.float32)
+    if metric == "ip":
+        q[0, :2] = 1.0
+        db[0] = 0.0
+        db[0, 0] = 1.0                      # dot 1
+        db[1] = 0.0
+        db[1, 0], db[1, 1] = 1.0, 2.0 ** -30  # dot 1 + 2^-30, rounds to 1 in FP32
+    else:
+        q[0, 0] = 1.0
+        db[0] = 0.0
+        db[0, 1] = 2.0 ** -13                 # |q-x|^2 = 1 + 2^-26, rounds to 1
+        db[1] = 0.0                           # exactly 1
+        db[2:, 0] += 10.0
+    return db, q
+
+
+@pytest.mark.parametrize("dtype", ["f16", "fp8"])
+@pytest.mark.parametrize("metric", ["ip", "l2"])
+def test_f64_boundary_breaks_fp32_ties(torch, dtype, metric):
+    db, q = _tie_case(metric)
+    ids, d = search(torch, db, q, 1, metric, dtype, shortlist=64)
+    assert ids[0, 0] == 0                     # FP32 tie -> lower id
+    ids, d = search(torch, db, q, 2, metric, dtype, shortlist=64, f64_boundary=True)
+    np.testing.assert_array_equal(ids[0], [1, 0])
+    e_ids, e_d = O.flat_topk_f64(db, q, 2, metric)
+    np.testing.assert_array_equal(ids[0], e_ids[0])
+    np.testing.assert_array_equal(d[0], e_d[0].astype(np.float32))
+
+
+@pytest.mark.parametrize("metric", ["ip", "l2", "cosine"])
+def test_f64_boundary_equals_fp64_oracle(torch, metric):
+    """Shortlist >= N: every row is reranked, so with FP64 boundary
+    re-scoring the output is the FP64 top-k exactly; without it, the exact
+    FP32 top-k."""
+    db, qs = S.vmf_workload(3000, 50, 128, 16, 50.0, device="cuda")
+    dbn, qsn = db.cpu().numpy(), qs.cpu().numpy()
+    idx = F.FlatIndex(db, "bf16")
+    out = idx.search(qs, 40, metric, shortlist=3072, f64_boundary=True)
+    torch.cuda.synchronize()
+    ids, d = L.split_results(out)
+    e_ids, e_d = O.flat_topk_f64(dbn, qsn, 40, metric)
+    np.testing.assert_array_equal(ids, e_ids)
+    out = idx.search(qs, 40, metric, shortlist=3072)
+    torch.cuda.synchronize()
+    assert_same(L.split_results(out), O.flat_topk(dbn, qsn, 40, metric))

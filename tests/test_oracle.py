@@ -288,3 +288,93 @@ def test_merge_dedup_keeps_best():
     m_ids, m_d = O.merge_topk(ids, d, 4)
     np.testing.assert_array_equal(m_ids[0], [5, 9, 2, 0xFFFFFFFF])
     np.testing.assert_array_equal(m_d[0], np.array([1.0, 2.0, 5.0, 1e30], np.float32))
+
+
+# --------------------------------------------------------------------------
+# FP8 e4m3 and the FP64 top-k
+# --------------------------------------------------------------------------
+
+def test_e4m3_decode_matches_torch_float8():
+    import torch
+    ref = torch.arange(256, dtype=torch.uint8).view(torch.float8_e4m3fn).float().numpy()
+    got = O.e4m3_table()
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(ref))
+    np.testing.assert_array_equal(got[~np.isnan(got)], ref[~np.isnan(ref)])
+
+
+def test_e4m3_encode_matches_torch_and_brute_force():
+    """Round to nearest even inside +-448 (torch's float8_e4m3fn cast), on
+    random values, every code, and every midpoint between adjacent codes."""
+    import torch
+    tab = O.e4m3_table()
+    pos = np.unique(tab[np.isfinite(tab) & (tab >= 0)])
+    mid = ((pos[1:].astype(np.float64) + pos[:-1]) / 2).astype(np.float32)
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(-448, 448, 20000), rng.normal(0, 1e-2, 5000),
+                        rng.normal(0, 1e-3, 5000), pos, -pos, mid, -mid]).astype(np.float32)
+    ref = torch.from_numpy(x).to(torch.float8_e4m3fn).view(torch.uint8).numpy()
+    got = np.array([O.f32_to_e4m3(v) for v in x], np.uint8)
+    np.testing.assert_array_equal(got, ref)
+    # Brute force: the nearest finite code, ties to the even mantissa.
+    fin = np.nonzero(np.isfinite(tab))[0]
+    for v in x[:3000]:
+        d = np.abs(tab[fin].astype(np.float64) - float(v))
+        best = fin[d == d.min()]
+        if len(best) > 1:
+            best = best[(best & 1) == 0]
+        assert tab[O.f32_to_e4m3(v)] == tab[best[0]], v
+
+
+def test_e4m3_saturates_and_keeps_nan():
+    assert O.f32_to_e4m3(1e6) == 0x7E and O.f32_to_e4m3(-1e6) == 0xFE
+    assert O.f32_to_e4m3(460.0) == 0x7E
+    assert O.f32_to_e4m3(float("inf")) == 0x7E
+    assert O.f32_to_e4m3(float("nan")) & 0x7F == 0x7F
+
+
+def test_fp8_rows_scale_and_exact_integers():
+    """Rows with integer entries in [-14, 14] and a 14 in every row encode
+    exactly (scale 1/32): decode * scale reproduces the row."""
+    rng = np.random.default_rng(4)
+    x = rng.integers(-14, 15, size=(50, 40)).astype(np.float32)
+    x[:, 0] = 14.0
+    codes, scales = O.encode_fp8_rows(x, pdim=48)
+    np.testing.assert_array_equal(scales, np.float32(14.0) / np.float32(448.0))
+    tab = O.e4m3_table()
+    np.testing.assert_array_equal(tab[codes[:, :40]] * scales[:, None], x)
+    assert (codes[:, 40:] == 0).all()
+    z, zs = O.encode_fp8_rows(np.zeros((2, 8), np.float32))
+    assert (z == 0).all() and (zs == 1.0).all()
+
+
+def test_fp8_scores_closed_form_on_integers():
+    rng = np.random.default_rng(5)
+    db = rng.integers(-14, 15, size=(60, 64)).astype(np.float32)
+    qs = rng.integers(-14, 15, size=(7, 64)).astype(np.float32)
+    db[:, 3] = 14.0
+    qs[:, 5] = -14.0
+    s = O.fp8_scores(db, qs, "ip")
+    np.testing.assert_array_equal(s, -(qs.astype(np.float64) @ db.T.astype(np.float64)))
+    s2 = O.fp8_scores(db, qs, "l2")
+    ref = 0.5 * (db.astype(np.float64) ** 2).sum(1)[None, :] - qs.astype(np.float64) @ db.T
+    np.testing.assert_array_equal(s2, ref)
+
+
+def test_flat_topk_f64_matches_numpy():
+    rng = np.random.default_rng(6)
+    db = rng.normal(size=(700, 24)).astype(np.float32)
+    qs = rng.normal(size=(9, 24)).astype(np.float32)
+    db[10] = db[3]                              # exact tie: lower id first
+    for metric in ("l2", "ip", "cosine"):
+        ids, d = O.flat_topk_f64(db, qs, 15, metric)
+        x, q = db.astype(np.float64), qs.astype(np.float64)
+        if metric == "l2":
+            full = ((q[:, None, :] - x[None, :, :]) ** 2).sum(-1)
+        elif metric == "ip":
+            full = -(q @ x.T)
+        else:
+            full = 1.0 - (q @ x.T) / (np.linalg.norm(q, axis=1)[:, None] * np.linalg.norm(x, axis=1)[None, :] + 1e-8)
+        for i in range(9):
+            order = np.lexsort((np.arange(700), full[i]))[:15]
+            np.testing.assert_array_equal(ids[i], order)
+            np.testing.assert_allclose(d[i], full[i][order], rtol=1e-12, atol=1e-12)
