@@ -66,8 +66,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--dtype", choices=["f16", "bf16"], default="f16")
     ap.add_argument("--k", type=int, default=10)
-    ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--nq", type=int, default=10_000)
+    ap.add_argument("--n", type=int, default=None, help="rows (default per workload)")
+    ap.add_argument("--nq", type=int, default=None, help="queries (default per workload)")
     ap.add_argument("--valid-frac", type=float, default=0.05)
     ap.add_argument("--fp32-steps", type=int, default=2)
     ap.add_argument("--recall-sample", type=int, default=100)
@@ -75,7 +75,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--alt-valid-frac", type=float, default=0.95,
                     help="second mask density reported beside the headline (0 = skip)")
-    ap.add_argument("--workload", choices=["flat", "ivfpq"], default="flat",
+    ap.add_argument("--workload", choices=["flat", "ivfpq", "fp8"], default="flat",
                     help="flat: the headline matrix-core scan; ivfpq: 10M-row IVF-PQ uneven batch")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy first (reference point for traffic counters)")
@@ -89,6 +89,10 @@ def main():
 
     if args.workload == "ivfpq":
         return bench_ivfpq(args)
+    if args.workload == "fp8":
+        return bench_fp8(args)
+    args.n = args.n or 1_000_000
+    args.nq = args.nq or 10_000
 
     import numpy as np
     import torch
@@ -340,6 +344,145 @@ def bench_ivfpq(args):
                      "kernel": "ivf_scan_kernel", "kernel_avg_ms": round(scan_avg, 4),
                      "algorithmic_bytes_per_launch": nbytes},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+
+
+def bench_fp8(args):
+    """FP8 flat inner-product scan with mixed k: N=5,000,000 unit rows at
+    d=768 from a von Mises-Fisher mixture (1024 clusters, kappa 50), stored as
+    FP8 e4m3 with per-row FP32 scales and scored on FP8 MFMA with FP32
+    accumulation; Q=4096 queries with k_i log-uniform on [1, 1024]; shortlist
+    4 k_i reranked exactly in FP32 with FP64 re-scoring at the boundary. One
+    step is one replay of the whole batch captured as a HIP graph."""
+    import numpy as np
+    import torch
+
+    from flashvector_amd import lib as L
+    from flashvector_amd import search as F
+    from flashvector_amd import synthetic as S
+
+    torch.cuda.set_device(0)
+    n, nq, dim, kmax = args.n or 5_000_000, args.nq or 4096, 768, 1024
+    sd = S.VMF_SEEDS
+    log(f"generating vMF workload n={n} nq={nq} d={dim} on the GPU")
+    t0 = time.perf_counter()
+    mu = S.vmf_mean_directions(1024, dim, sd[0], "cuda")
+    db = S.vmf_points(mu, n, 50.0, sd[1])
+    qs = S.vmf_points(mu, nq, 50.0, sd[2])
+    ks = S.log_uniform_k(nq, kmax, sd[3])
+    d_k = torch.from_numpy(ks).cuda()
+    idx = F.FlatIndex(db, "fp8")
+    torch.cuda.synchronize()
+    log(f"generated and encoded in {time.perf_counter() - t0:.1f} s")
+    out = L.result_tensor(nq, kmax, qs.device)
+
+    def step():
+        idx.search(qs, kmax, "ip", shortlist_factor=4, top_k_per_query=d_k, f64_boundary=True,
+                   out=out)
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    torch.cuda.synchronize()
+    eager = out.clone()
+    # Main-pass kernel time from eager calls (events around the kernel).
+    ksteps = max(1, args.steps // 4)
+    with L.KernelTimer() as kt:
+        for _ in range(ksteps):
+            step()
+        torch.cuda.synchronize()
+        scan_ms, launches = kt.read()
+    scan_avg = scan_ms / max(launches, 1)
+    log("capturing the batch as one HIP graph")
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        step()
+    torch.cuda.current_stream().wait_stream(side)
+    with torch.cuda.graph(g):
+        step()
+    out.zero_()
+    g.replay()
+    torch.cuda.synchronize()
+    graph_matches_eager = bool(torch.equal(out, eager))
+    steps = max(1, args.steps // 2)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    step_ms = e0.elapsed_time(e1) / steps
+    graph_matches_eager = graph_matches_eager and bool(torch.equal(out, eager))
+    ids, dists = L.split_results(out)
+
+    rs = min(args.recall_sample, nq)
+    log(f"FP64 oracle on {rs} queries (GPU FP64 products)")
+    q64 = qs[:rs].double()
+    sc = torch.empty((rs, n), dtype=torch.float64, device="cuda")
+    for c0 in range(0, n, 1 << 19):
+        c1 = min(n, c0 + (1 << 19))
+        sc[:, c0:c1] = q64 @ db[c0:c1].double().T
+    top = torch.topk(sc, kmax, dim=1).indices.cpu().numpy()
+    hits = tot = 0
+    rec = []
+    err = 0.0
+    for i in range(rs):
+        k = int(ks[i])
+        h = len(set(ids[i, :k].tolist()) & set(top[i, :k].tolist()))
+        hits, tot = hits + h, tot + k
+        rec.append(h / k)
+        got = torch.from_numpy(ids[i, :k].astype(np.int64)).cuda()
+        ref = -sc[i, got].cpu().numpy()
+        err = max(err, float(np.max(np.abs(dists[i, :k].astype(np.float64) - ref))))
+    small = [r for r, k in zip(rec, ks[:rs]) if k <= 10]
+    del sc
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        O.build()
+        cores = len(os.sched_getaffinity(0))
+        cs = min(cores, nq)
+        log(f"CPU oracle (exact FP32 scan) on {cs} queries, {cores} threads")
+        dbn = db.cpu().numpy()
+        qn = qs[:cs].cpu().numpy()
+        t0 = time.perf_counter()
+        O.flat_topk(dbn, qn, int(ks[:cs].max()), "ip", nthreads=cores)
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": round(cs / cpu_s, 3), "unit": "queries/s", "cores": cores, "kind": "port",
+               "sample": f"{cs} of the {nq} queries against all {n} rows, exact FP32 inner "
+                         f"product (oracle/fv_oracle.c, OpenMP over queries), {cpu_s:.2f} s"}
+        del dbn
+
+    flops = 2.0 * nq * n * dim
+    achieved = flops / (scan_avg / 1e3) / 1e12
+    peak = 5000.0
+    res = {
+        "metric": "FP8 flat IP QPS, 5M x 768 vMF, mixed k in [1,1024], 4k shortlist + FP32 rerank "
+                  "+ FP64 boundary, one HIP graph; recall vs FP64",
+        "value": round(nq / (step_ms / 1e3), 1), "unit": "queries/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": round(step_ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp8_e4m3",
+        "data": "synthetic (seeded vMF mixture, 1024 clusters, kappa 50, unit rows)",
+        "config": {"workload": "flat_ip_fp8_mixed_k", "n": n, "dim": dim, "num_queries": nq,
+                   "top_k": "per query, log-uniform on [1,1024]", "shortlist": "4 k_i",
+                   "f64_boundary": True, "hip_graph": True},
+        "recall_at_k": round(hits / tot, 5), "recall_at_k_mean_per_query": round(float(np.mean(rec)), 5),
+        "recall_at_k_queries_k_le_10": round(float(np.mean(small)), 5) if small else None,
+        "recall_sample_queries": rs, "max_abs_distance_error_vs_fp64": err,
+        "graph_replay_matches_eager": graph_matches_eager,
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak,
+                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+                     "traffic": measured_traffic("flat_mfma_scan_kernel/flat_ip_fp8_mixed_k"),
+                     "kernel": "flat_mfma_scan_kernel", "kernel_avg_ms": round(scan_avg, 4),
+                     "algorithmic_flop_per_launch": flops},
+        "cpu_baseline": cpu,
+        "wall_s_timed": round(wall, 3),
     }
     print(json.dumps(res), flush=True)
 

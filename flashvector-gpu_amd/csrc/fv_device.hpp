@@ -95,6 +95,29 @@ __device__ __forceinline__ void block_bitonic_sort(uint64_t* keys, uint32_t n) {
     }
 }
 
+// Same network on (key, value) pairs ordered lexicographically.
+__device__ __forceinline__ void block_bitonic_sort_pairs(uint64_t* keys, uint32_t* vals, uint32_t n) {
+    for (uint32_t k = 2; k <= n; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t p = threadIdx.x; p < (n >> 1); p += blockDim.x) {
+                uint32_t i = ((p & ~(j - 1u)) << 1) | (p & (j - 1u));
+                uint32_t m = i + j;
+                const uint64_t a = keys[i], b = keys[m];
+                const uint32_t va = vals[i], vb = vals[m];
+                const bool gt = a > b || (a == b && va > vb);
+                bool up = (i & k) == 0;
+                if (gt == up) {
+                    keys[i] = b;
+                    keys[m] = a;
+                    vals[i] = vb;
+                    vals[m] = va;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // Prune of one query's candidate buffer by one wavefront: sorts the `count`
 // buffered keys through LDS scratch (capacity >= next_pow2(count)), writes the
 // `keep` smallest back in ascending order and returns how many remain.
@@ -241,30 +264,6 @@ __device__ __forceinline__ float exact_distance_rt(const float* q, const float* 
     if (metric == METRIC_INNER_PRODUCT) return exact_distance<METRIC_INNER_PRODUCT>(q, x, dim);
     if (metric == METRIC_COSINE) return exact_distance<METRIC_COSINE>(q, x, dim);
     return exact_distance<METRIC_L2>(q, x, dim);
-}
-
-// Bound on |fp32 result - exact distance| for the sums of exact_distance
-// (standard forward error of a recursive sum of rounded terms, gamma_n ~ n*u
-// with u = 2^-24), doubled for margin. `d32` is the FP32 distance.
-template <int METRIC>
-__device__ __forceinline__ float exact_distance_err(const float* __restrict__ q,
-                                                    const float* __restrict__ x, uint32_t dim,
-                                                    float d32) {
-    constexpr float u = 5.9604645e-8f;
-    if (METRIC == METRIC_L2) return 2.0f * (float)(dim + 3) * u * d32;
-    float sabs = 0.0f, dot = 0.0f, nq = 0.0f, nx = 0.0f;
-    for (uint32_t d = 0; d < dim; ++d) {
-        const float p = q[d] * x[d];
-        sabs += fabsf(p);
-        if (METRIC == METRIC_COSINE) {
-            dot += p;
-            nq += q[d] * q[d];
-            nx += x[d] * x[d];
-        }
-    }
-    if (METRIC == METRIC_INNER_PRODUCT) return 2.0f * (float)(dim + 2) * u * sabs;
-    const float denom = sqrtf(nq) * sqrtf(nx) + 1e-8f;
-    return 2.0f * ((float)(dim + 5) * u * (sabs + fabsf(dot)) / denom + 2.0f * u);
 }
 
 // The same distances evaluated in FP64 (products of floats are exact there).

@@ -5,57 +5,79 @@
 //
 // Reference: kernels/src/ivf_quant_lookup.cu:887-1058, 1106-1123.
 //
-//  * flat_select_rerank_kernel: per query, the candidates of every chunk that
-//    are within the final shared bound are streamed into an LDS top-S, which
-//    is reranked exactly in FP32 (reference CPU path order) and sorted by
-//    (distance, row); the first k are written. With FV_SEARCH_F64_BOUNDARY,
-//    candidates whose FP32 distance lies within twice the FP32 rounding-error
-//    bound of a neighbour in that order are re-scored in FP64 and reordered by
-//    (FP64 distance, row), so the order of the returned rows, and which rows
-//    make the top k, follow the exact distances.
+//  * flat_select_kernel: per query, the candidates of every chunk that are
+//    within the final shared bound are streamed into an LDS top-S.
+//  * flat_rerank_kernel: per (query, 256 selected rows), exact FP32 distances
+//    in the reference CPU path's order, rows staged through LDS.
+//  * flat_finalize_kernel: per query, sort by (distance, row) and write the
+//    first k. With FV_SEARCH_F64_BOUNDARY, candidates whose FP32 distance
+//    lies within twice the FP32 rounding-error bound of a neighbour in that
+//    order are re-scored in FP64 and reordered by (FP64 distance, row), so the
+//    order of the returned rows, and which rows make the top k, follow the
+//    exact distances.
 #include "fv_flat_mfma_scan.hpp"
 
 namespace fv {
 
 constexpr uint32_t kSelectBatch = 256;
 
-// Per query: the S-th smallest of the first pass's subset minima. Each minimum
-// is the score of a distinct row, so at least S rows score at or below it:
-// it bounds the global S-th best and seeds the shared bound of the main pass.
+__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* s_red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) s_red[threadIdx.x / kWave] = v;
+    __syncthreads();
+    uint32_t t = 0;
+    for (uint32_t w = 0; w < blockDim.x / kWave; ++w) t += s_red[w];
+    return t;
+}
+
+// Per query: a value with at least S of the first pass's subset minima at or
+// below it (the S-th smallest, or slightly above it), found by bisection over
+// the ordered distance held in LDS. Each minimum is the score of a distinct
+// row, so at least S rows score at or below the result: it bounds the global
+// S-th best and seeds the shared bound of the main pass.
 __global__ __launch_bounds__(256) void bound_select_kernel(const float* __restrict__ minima,
                                                            uint32_t m, uint32_t nq,
                                                            FlatSearchArgs a,
                                                            uint32_t* __restrict__ gthr) {
-    extern __shared__ uint64_t s_b[];
+    extern __shared__ uint32_t s_o[];
+    __shared__ uint32_t s_red[256 / kWave];
     const uint32_t q = blockIdx.x;
-    const uint32_t np = next_pow2(m);
-    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) {
-        uint64_t key = kEmptyKey;
-        if (i < m) {
-            const uint32_t c = i >> 4, e = i & 15u;
-            const float v = minima[((size_t)c * nq + q) * 16 + e];
-            if (v < kAdmit) key = make_key(v, i);
+    uint32_t k, S;
+    query_k_s(a, q, k, S);
+    uint32_t valid = 0;
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+        const uint32_t c = i >> 4, e = i & 15u;
+        const float v = minima[((size_t)c * nq + q) * 16 + e];
+        const uint32_t o = v < kAdmit ? ord_f32(v) : 0xFFFFFFFFu;
+        s_o[i] = o;
+        valid += o != 0xFFFFFFFFu;
+    }
+    if (block_sum_u32(valid, s_red) < S) return;
+    uint32_t lo = 0u, hi = 0xFFFFFFFEu;
+    while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        uint32_t c = 0;
+        for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) c += s_o[i] <= mid;
+        c = block_sum_u32(c, s_red);
+        if (c >= S) {
+            hi = mid;
+            if (c <= S + (S >> 3) + 16u) break;
+        } else {
+            lo = mid + 1u;
         }
-        s_b[i] = key;
     }
-    __syncthreads();
-    block_bitonic_sort(s_b, np);
-    if (threadIdx.x == 0) {
-        uint32_t k, S;
-        query_k_s(a, q, k, S);
-        if (S <= m && s_b[S - 1] != kEmptyKey) atomicMin(&gthr[q], (uint32_t)(s_b[S - 1] >> 32));
-    }
+    if (threadIdx.x == 0) atomicMin(&gthr[q], hi);
 }
 
 // One workgroup per query: streams the candidates of every chunk that are
-// within the final shared bound into an LDS top-S, reranks them exactly in
-// FP32 and writes the top k by (distance, row).
-template <int METRIC>
-__global__ __launch_bounds__(256) void flat_select_rerank_kernel(
+// within the final shared bound into an LDS top-S (sorted only when more
+// than S arrive) and writes the S best keys (approximate score, row) to
+// sel[q][0..have) and have to sel_n[q].
+__global__ __launch_bounds__(256) void flat_select_kernel(
     const uint64_t* __restrict__ cand, const uint32_t* __restrict__ counts, uint32_t n_chunks,
     uint32_t nq, uint32_t cap, const uint32_t* __restrict__ gthr, FlatSearchArgs a,
-    uint32_t lcap, const float* __restrict__ X32, const float* __restrict__ Qm,
-    GpuSearchResult* __restrict__ out) {
+    uint32_t lcap, uint64_t* __restrict__ sel, uint32_t* __restrict__ sel_n) {
     extern __shared__ uint64_t s_k[];
     __shared__ uint32_t s_n;
     __shared__ uint32_t s_g;
@@ -93,99 +115,229 @@ __global__ __launch_bounds__(256) void flat_select_rerank_kernel(
         }
     }
     const uint32_t n = s_n;
-    const uint32_t np = next_pow2(n < 2u ? 2u : n);
-    for (uint32_t i = n + threadIdx.x; i < np; i += blockDim.x) s_k[i] = kEmptyKey;
-    __syncthreads();
-    block_bitonic_sort(s_k, np);
-    const uint32_t have = n < S ? n : S;
-    const float* qv = Qm + (size_t)q * a.dim;
-    for (uint32_t i = threadIdx.x; i < have; i += blockDim.x) {
-        const uint64_t key = s_k[i];
-        const uint32_t r = key_slot(key);
-        float d;
-        if (X32) {
-            d = exact_distance_vec<METRIC>(qv, X32 + (size_t)r * a.dim, a.dim);
-        } else {
-            // No FP32 rows: report the reduced-precision score as a distance.
-            const float s = key_dist(key);
-            if (METRIC == METRIC_INNER_PRODUCT) {
-                d = s;
-            } else {
-                float qn = 0.0f;
-                for (uint32_t dd = 0; dd < a.dim; ++dd) qn = __builtin_fmaf(qv[dd], qv[dd], qn);
-                d = METRIC == METRIC_L2 ? fmaxf(0.0f, qn + 2.0f * s) : 1.0f + s / (sqrtf(qn) + 1e-8f);
-            }
-        }
-        s_k[i] = d < kAdmit ? make_key(d, r) : kEmptyKey;
+    if (n > S) {
+        const uint32_t np = next_pow2(n);
+        for (uint32_t i = n + threadIdx.x; i < np; i += blockDim.x) s_k[i] = kEmptyKey;
+        __syncthreads();
+        block_bitonic_sort(s_k, np);
     }
-    __shared__ uint32_t s_emax;
-    const bool f64b = (a.flags & FV_SEARCH_F64_BOUNDARY) && X32 != nullptr;
-    if (f64b) {
-        // Largest FP32 error bound among the reranked candidates (positive
-        // floats order like their bit patterns).
-        float e = 0.0f;
-        for (uint32_t i = threadIdx.x; i < have; i += blockDim.x) {
-            const uint64_t key = s_k[i];
-            if (key == kEmptyKey) continue;
-            e = fmaxf(e, exact_distance_err<METRIC>(qv, X32 + (size_t)key_slot(key) * a.dim,
-                                                    a.dim, key_dist(key)));
+    const uint32_t have = n < S ? n : S;
+    uint64_t* o = sel + (size_t)q * a.s_max;
+    for (uint32_t i = threadIdx.x; i < have; i += blockDim.x) o[i] = s_k[i];
+    if (threadIdx.x == 0) sel_n[q] = have;
+}
+
+// Exact FP32 rerank of the selected rows: one workgroup per (query, group of
+// 256 selected rows), one row per thread. Rows are staged through LDS in
+// slabs of 32 dims with coalesced loads; each thread keeps its row's sums in
+// the reference CPU path's order (sequential over d, non-fused), so the
+// distance equals exact_distance<METRIC> bit for bit. Writes (distance, row)
+// keys in place and, when err != NULL, the FP32 error bound of each distance.
+// Without FP32 rows the approximate score is turned into a distance instead.
+constexpr uint32_t kRerankRows = 256;
+constexpr uint32_t kRerankSlab = 32;
+
+template <int METRIC>
+__global__ __launch_bounds__(256) void flat_rerank_kernel(
+    const float* __restrict__ X32, const float* __restrict__ Qm, uint32_t dim, uint32_t smax,
+    uint32_t groups, const uint32_t* __restrict__ sel_n, uint64_t* __restrict__ sel,
+    float* __restrict__ err) {
+    __shared__ float s_x[kRerankRows * (kRerankSlab + 1)];
+    __shared__ float s_q[768];
+    __shared__ uint32_t s_row[kRerankRows];
+    const uint32_t q = blockIdx.x / groups, g = blockIdx.x % groups;
+    const uint32_t have = sel_n[q];
+    const uint32_t i0 = g * kRerankRows;
+    if (i0 >= have) return;
+    const uint32_t rows_here = min(kRerankRows, have - i0);
+    const uint32_t tid = threadIdx.x;
+    uint64_t* keys = sel + (size_t)q * smax + i0;
+    const uint64_t my_key = tid < rows_here ? keys[tid] : kEmptyKey;
+    s_row[tid] = key_slot(tid < rows_here ? my_key : keys[0]);
+    const float* qv = Qm + (size_t)q * dim;
+    for (uint32_t d = tid; d < dim; d += blockDim.x) s_q[d] = qv[d];
+    __syncthreads();
+    float d32;
+    float e = 0.0f;
+    if (X32) {
+        // Sums in the reference order plus the running error bound of each
+        // recursive sum (Higham, Accuracy and Stability of Numerical
+        // Algorithms, 2002, sec. 4.2): |error| <= u * sum_i |partial sum_i|
+        // + c * u * sum_i |term_i| to first order, c = 1 for products and 3
+        // for squared differences.
+        float acc = 0.0f, nqa = 0.0f, nxa = 0.0f, sabs = 0.0f;
+        float run = 0.0f, run_q = 0.0f, run_x = 0.0f;
+        const bool vec = (dim & 3u) == 0u && (reinterpret_cast<uintptr_t>(X32) & 15u) == 0u;
+        for (uint32_t d0 = 0; d0 < dim; d0 += kRerankSlab) {
+            const uint32_t w = min(kRerankSlab, dim - d0);
+            // 256 rows x 32 dims: 8 float4 (or 32 scalars) per thread.
+            if (vec) {
+#pragma unroll
+                for (uint32_t m = 0; m < 8; ++m) {
+                    const uint32_t j = tid + 256u * m;
+                    const uint32_t row = j >> 3, part = j & 7u;
+                    if (row < rows_here && part * 4u < w) {
+                        const float4 v = *reinterpret_cast<const float4*>(
+                            X32 + (size_t)s_row[row] * dim + d0 + part * 4u);
+                        float* dst = &s_x[row * (kRerankSlab + 1) + part * 4u];
+                        dst[0] = v.x;
+                        dst[1] = v.y;
+                        dst[2] = v.z;
+                        dst[3] = v.w;
+                    }
+                }
+            } else {
+                for (uint32_t m = 0; m < kRerankSlab; ++m) {
+                    const uint32_t j = tid + 256u * m;
+                    const uint32_t row = j / kRerankSlab, c = j % kRerankSlab;
+                    if (row < rows_here && c < w)
+                        s_x[row * (kRerankSlab + 1) + c] = X32[(size_t)s_row[row] * dim + d0 + c];
+                }
+            }
+            __syncthreads();
+            if (tid < rows_here) {
+                const float* xr = &s_x[tid * (kRerankSlab + 1)];
+                for (uint32_t dd = 0; dd < w; ++dd) {
+                    const float a = s_q[d0 + dd], b = xr[dd];
+                    if (METRIC == METRIC_L2) {
+                        const float diff = a - b;
+                        const float sq = diff * diff;
+                        acc = acc + sq;
+                        run += acc;
+                    } else {
+                        const float p = a * b;
+                        acc = acc + p;
+                        run += fabsf(acc);
+                        sabs += fabsf(p);
+                        if (METRIC == METRIC_COSINE) {
+                            const float pa = a * a, pb = b * b;
+                            nqa = nqa + pa;
+                            nxa = nxa + pb;
+                            run_q += nqa;
+                            run_x += nxa;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
         }
+        constexpr float u = 5.9604645e-8f;  // 2^-24
+        constexpr float margin = 1.01f;     // covers the second-order terms
+        if (METRIC == METRIC_L2) {
+            d32 = acc;
+            e = margin * u * (run + 3.0f * acc);
+        } else if (METRIC == METRIC_INNER_PRODUCT) {
+            d32 = -acc;
+            e = margin * u * (run + sabs);
+        } else {
+            const float denom = sqrtf(nqa) * sqrtf(nxa) + 1e-8f;
+            d32 = 1.0f - acc / denom;
+            const float e_dot = u * (run + sabs);
+            // Relative error of the denominator: half of each norm's relative
+            // error, plus two square roots, a product and an add.
+            const float rel = 0.5f * u * ((run_q + nqa) / fmaxf(nqa, 1e-30f) +
+                                          (run_x + nxa) / fmaxf(nxa, 1e-30f)) + 4.0f * u;
+            e = margin * ((e_dot + fabsf(acc) * rel) / denom + u * fabsf(acc / denom) + u * fabsf(d32));
+        }
+    } else {
+        // No FP32 rows: report the reduced-precision score as a distance.
+        const float s = key_dist(my_key);
+        if (METRIC == METRIC_INNER_PRODUCT) {
+            d32 = s;
+        } else {
+            float qn = 0.0f;
+            for (uint32_t dd = 0; dd < dim; ++dd) qn = __builtin_fmaf(s_q[dd], s_q[dd], qn);
+            d32 = METRIC == METRIC_L2 ? fmaxf(0.0f, qn + 2.0f * s) : 1.0f + s / (sqrtf(qn) + 1e-8f);
+        }
+    }
+    if (tid < rows_here) {
+        keys[tid] = d32 < kAdmit ? make_key(d32, s_row[tid]) : kEmptyKey;
+        if (err) err[(size_t)q * smax + i0 + tid] = e;
+    }
+}
+
+// One workgroup per query: sorts the reranked keys by (distance, row) and
+// writes the first k. With FV_SEARCH_F64_BOUNDARY, two candidates adjacent
+// in FP32 order whose gap is at most twice the largest error bound may be in
+// either exact order: chains of such gaps form clusters, which are re-scored
+// in FP64 and reordered by (FP64 distance, row). Outside a cluster the FP32
+// order is the exact order, so only clusters that start before position k are
+// touched.
+template <int METRIC>
+__global__ __launch_bounds__(256) void flat_finalize_kernel(
+    const uint64_t* __restrict__ sel, const uint32_t* __restrict__ sel_n,
+    const float* __restrict__ err, uint32_t hcap, FlatSearchArgs a,
+    const float* __restrict__ X32, const float* __restrict__ Qm,
+    GpuSearchResult* __restrict__ out) {
+    extern __shared__ uint64_t s_k[];
+    __shared__ uint32_t s_emax;
+    const uint32_t q = blockIdx.x;
+    uint32_t k, S;
+    query_k_s(a, q, k, S);
+    const uint32_t have = sel_n[q];
+    const uint64_t* src = sel + (size_t)q * a.s_max;
+    const uint32_t hp = next_pow2(have < 2u ? 2u : have);
+    for (uint32_t i = threadIdx.x; i < hp; i += blockDim.x) s_k[i] = i < have ? src[i] : kEmptyKey;
+    const bool f64b = (a.flags & FV_SEARCH_F64_BOUNDARY) && X32 != nullptr && err != nullptr;
+    if (f64b) {
+        // Largest FP32 error bound (positive floats order like their bits).
+        float e = 0.0f;
+        for (uint32_t i = threadIdx.x; i < have; i += blockDim.x)
+            e = fmaxf(e, err[(size_t)q * a.s_max + i]);
         if (threadIdx.x == 0) s_emax = 0u;
         __syncthreads();
         atomicMax(&s_emax, __float_as_uint(e));
     }
-    const uint32_t hp = next_pow2(have < 2u ? 2u : have);
-    for (uint32_t i = have + threadIdx.x; i < hp; i += blockDim.x) s_k[i] = kEmptyKey;
     __syncthreads();
     block_bitonic_sort(s_k, hp);
+    const float* qv = Qm + (size_t)q * a.dim;
     if (f64b && have > 1) {
-        // Two candidates adjacent in FP32 order whose gap is at most twice the
-        // error bound may be in either exact order: chains of such gaps form
-        // clusters, which are re-scored in FP64 and reordered by (FP64
-        // distance, row). Outside a cluster the FP32 order is the exact order,
-        // so only clusters that start before position k are touched.
-        double* s_d64 = reinterpret_cast<double*>(s_k + lcap);
-        uint16_t* s_pos = reinterpret_cast<uint16_t*>(s_d64 + lcap);
+        // Candidates adjacent in FP32 order whose gap is at most twice the
+        // largest error bound may be in either exact order. Every such
+        // candidate that could reach the top k (FP32 distance at most the
+        // k-th plus that band) is re-scored in FP64; then all candidates are
+        // sorted by (value, row) with value = the FP64 distance where
+        // re-scored, else the FP32 distance. A candidate that is not
+        // re-scored is more than the band away from its neighbours, so its
+        // place in this order is its exact place.
+        uint64_t* s_v = s_k + hcap;                               // ordered value bits
+        uint32_t* s_r = reinterpret_cast<uint32_t*>(s_v + hcap);  // rows
+        __shared__ uint32_t s_any;
         const float band = 2.0000002f * __uint_as_float(s_emax);
-        auto near_next = [&](uint32_t i) -> bool {
-            if (i + 1 >= have) return false;
-            const uint64_t k0 = s_k[i], k1 = s_k[i + 1];
-            if (k0 == kEmptyKey || k1 == kEmptyKey) return false;
-            return key_dist(k1) - key_dist(k0) <= band;
-        };
-        for (uint32_t i = threadIdx.x; i < have; i += blockDim.x) {
-            uint32_t st = i;
-            while (st > 0 && near_next(st - 1)) --st;
-            const bool in_cluster = (i > 0 && near_next(i - 1)) || near_next(i);
-            s_pos[i] = (in_cluster && st < k) ? 1 : 0;
-            if (in_cluster && st < k)
-                s_d64[i] = exact_distance_f64<METRIC>(qv, X32 + (size_t)key_slot(s_k[i]) * a.dim, a.dim);
-        }
+        const uint32_t kk = k < have ? k : have;
+        const float lim = key_dist(s_k[kk - 1]) + band;
+        if (threadIdx.x == 0) s_any = 0u;
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < have; i += blockDim.x) {
-            if (s_pos[i] == 0) continue;
-            uint32_t st = i, en = i;
-            while (st > 0 && near_next(st - 1)) --st;
-            while (near_next(en)) ++en;
-            const double di = s_d64[i];
-            const uint32_t ri = key_slot(s_k[i]);
-            uint32_t rank = 0;
-            for (uint32_t j = st; j <= en; ++j) {
-                const double dj = s_d64[j];
-                const uint32_t rj = key_slot(s_k[j]);
-                rank += (dj < di || (dj == di && rj < ri)) ? 1u : 0u;
+        for (uint32_t i = threadIdx.x; i < hp; i += blockDim.x) {
+            const uint64_t key = s_k[i];
+            uint64_t v = ~0ull;
+            uint32_t r = 0xFFFFFFFFu;
+            if (i < have && key != kEmptyKey) {
+                const float d = key_dist(key);
+                r = key_slot(key);
+                double dd = (double)d;
+                const bool near_prev = i > 0 && s_k[i - 1] != kEmptyKey && d - key_dist(s_k[i - 1]) <= band;
+                const bool near_nxt = i + 1 < have && s_k[i + 1] != kEmptyKey && key_dist(s_k[i + 1]) - d <= band;
+                if ((near_prev || near_nxt) && d <= lim) {
+                    dd = exact_distance_f64<METRIC>(qv, X32 + (size_t)r * a.dim, a.dim);
+                    s_any = 1u;
+                }
+                const uint64_t bits = __double_as_longlong(dd);
+                v = (bits & 0x8000000000000000ull) ? ~bits : (bits | 0x8000000000000000ull);
             }
-            s_pos[i] = (uint16_t)(st + rank + 1);  // 0 marks "not moved"
+            s_v[i] = v;
+            s_r[i] = r;
         }
         __syncthreads();
-        // New keys (FP64 distance rounded to FP32) staged in the FP64 slots.
-        for (uint32_t i = threadIdx.x; i < have; i += blockDim.x)
-            if (s_pos[i] != 0)
-                reinterpret_cast<uint64_t*>(s_d64)[i] = make_key((float)s_d64[i], key_slot(s_k[i]));
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < have; i += blockDim.x)
-            if (s_pos[i] != 0) s_k[s_pos[i] - 1u] = reinterpret_cast<const uint64_t*>(s_d64)[i];
-        __syncthreads();
+        if (s_any) {
+            block_bitonic_sort_pairs(s_v, s_r, hp);
+            for (uint32_t i = threadIdx.x; i < kk; i += blockDim.x) {
+                const uint64_t v = s_v[i];
+                const uint64_t bits = (v & 0x8000000000000000ull) ? (v & 0x7fffffffffffffffull) : ~v;
+                s_k[i] = make_key((float)__longlong_as_double(bits), s_r[i]);
+            }
+            __syncthreads();
+        }
     }
     for (uint32_t i = threadIdx.x; i < a.top_k; i += blockDim.x) {
         const uint64_t key = i < k && i < hp ? s_k[i] : kEmptyKey;
@@ -381,7 +533,9 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
                                : 4 * ((size_t)p.cap * 8 > stage_bytes ? (size_t)p.cap * 8 : stage_bytes);
     p.lcap = next_pow2(smax + kSelectBatch);
     p.lds_select = (size_t)p.lcap * 8;
-    if (a.flags & FV_SEARCH_F64_BOUNDARY) p.lds_select += (size_t)p.lcap * (8 + 2);
+    p.hcap = next_pow2(smax < 2u ? 2u : smax);
+    p.lds_final = (size_t)p.hcap * 8;
+    if (a.flags & FV_SEARCH_F64_BOUNDARY) p.lds_final += (size_t)p.hcap * (8 + 4);
     p.ks = plan_ks(padded_dim, dtype);
     if (p.ks == 0) return p;
     p.qw = (uint32_t)scan_qw(dtype, p.ks);
@@ -401,11 +555,20 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
     if (c >= 8u) c &= ~7u;
     if (c < 1u) c = 1u;
     p.n_chunks = c;
-    // First pass: subset minima over the first bound_rows positions, 64 chunks
-    // (1024 minima per query) so the S-th smallest is a tight bound.
-    const uint32_t brows = rows_plan / 4 > 262144u ? rows_plan / 4 : (rows_plan < 262144u ? rows_plan : 262144u);
+    // First pass: 16 subset minima per query per chunk over the first
+    // bound_rows positions (a quarter of the rows, a sixteenth for rows wider
+    // than 256 bytes, at least 262144). With m >= 4*S minima the S-th smallest
+    // sits near rank S of those rows, so the main pass admits about
+    // S * rows / bound_rows rows per query from the start.
+    uint32_t brows = p.qrow_bytes <= 256 ? rows_plan / 4 : rows_plan / 16;
+    const uint32_t bmin = rows_plan < 262144u ? rows_plan : 262144u;
+    if (brows < bmin) brows = bmin;
     p.bound_rows = brows;
-    p.bound_chunks = (brows >= 64u * 64u && 16u * 64u >= smax) ? 64u : 0u;
+    uint32_t m = 1024u;
+    while (m < 4u * smax && m < 16384u) m *= 2u;
+    uint32_t bc = m / 16u;
+    while (bc > 8u && (uint64_t)bc * 64u > brows) bc /= 2u;  // at least one tile per chunk
+    p.bound_chunks = (16u * bc >= smax && brows >= bc * 64u) ? bc : 0u;
     size_t off = 0;
     auto take = [&](size_t bytes) {
         size_t o = off;
@@ -416,6 +579,9 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
     p.off_counts = take((size_t)p.n_chunks * nq * 4);
     p.off_cand = take((size_t)p.n_chunks * nq * p.cap * 8);
     p.off_qh = take((size_t)p.qblocks * p.qb * p.qrow_bytes);
+    p.off_sel = take((size_t)nq * smax * 8);
+    p.off_seln = take((size_t)nq * 4);
+    if (a.flags & FV_SEARCH_F64_BOUNDARY) p.off_err = take((size_t)nq * smax * 4);
     if (dtype == FV_DTYPE_FP8_E4M3) p.off_qscale = take((size_t)p.qblocks * p.qb * 4);
     if (p.bound_chunks) p.off_minima = take((size_t)p.bound_chunks * nq * 16 * 4);
     if (p.compact) {
@@ -436,8 +602,7 @@ __global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {
 
 void launch_bound_select(const ScanLaunch& L) {
     const uint32_t m = L.p->bound_chunks * 16;
-    bound_select_kernel<<<L.nq, 256, (size_t)next_pow2(m) * 8, L.stream>>>(L.minima, m, L.nq, L.a,
-                                                                          L.gthr);
+    bound_select_kernel<<<L.nq, 256, (size_t)m * 4, L.stream>>>(L.minima, m, L.nq, L.a, L.gthr);
 }
 
 hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
@@ -488,13 +653,24 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
     else if (t->dtype == FV_DTYPE_BF16) e = large ? scan_bf16_large(L) : scan_bf16_small(L);
     else if (t->dtype == FV_DTYPE_FP8_E4M3) e = scan_fp8(L);
     if (e != hipSuccess) return e;
-    const size_t lds = p.lds_select;
-    if (a.metric == METRIC_L2)
-        flat_select_rerank_kernel<METRIC_L2><<<nq, 256, lds, stream>>>(cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.lcap, t->d_vectors, Qm, out);
-    else if (a.metric == METRIC_INNER_PRODUCT)
-        flat_select_rerank_kernel<METRIC_INNER_PRODUCT><<<nq, 256, lds, stream>>>(cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.lcap, t->d_vectors, Qm, out);
-    else
-        flat_select_rerank_kernel<METRIC_COSINE><<<nq, 256, lds, stream>>>(cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.lcap, t->d_vectors, Qm, out);
+    uint64_t* sel = reinterpret_cast<uint64_t*>(ws + p.off_sel);
+    uint32_t* sel_n = reinterpret_cast<uint32_t*>(ws + p.off_seln);
+    float* err = (a.flags & FV_SEARCH_F64_BOUNDARY) ? reinterpret_cast<float*>(ws + p.off_err) : nullptr;
+    flat_select_kernel<<<nq, 256, p.lds_select, stream>>>(cand, counts, p.n_chunks, nq, p.cap, gthr,
+                                                          a, p.lcap, sel, sel_n);
+    const uint32_t groups = (p.s_max + kRerankRows - 1) / kRerankRows;
+    const dim3 rg((unsigned)((size_t)nq * groups));
+    const float* X32 = t->d_vectors;
+    if (a.metric == METRIC_L2) {
+        flat_rerank_kernel<METRIC_L2><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, p.s_max, groups, sel_n, sel, err);
+        flat_finalize_kernel<METRIC_L2><<<nq, 256, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
+    } else if (a.metric == METRIC_INNER_PRODUCT) {
+        flat_rerank_kernel<METRIC_INNER_PRODUCT><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, p.s_max, groups, sel_n, sel, err);
+        flat_finalize_kernel<METRIC_INNER_PRODUCT><<<nq, 256, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
+    } else {
+        flat_rerank_kernel<METRIC_COSINE><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, p.s_max, groups, sel_n, sel, err);
+        flat_finalize_kernel<METRIC_COSINE><<<nq, 256, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
+    }
     return hipGetLastError();
 }
 

@@ -25,7 +25,9 @@ struct FlatMfmaPlan {
     bool block_prune;         // prune with the whole workgroup (deep shortlists)
     size_t lds_scan;          // dynamic LDS of the scan kernel
     uint32_t lcap;            // LDS keys of the select/rerank kernel
-    size_t lds_select;        // dynamic LDS of the select/rerank kernel
+    size_t lds_select;        // dynamic LDS of the select kernel
+    uint32_t hcap;            // keys of the finalize kernel (next_pow2(s_max))
+    size_t lds_final;         // dynamic LDS of the finalize kernel
     uint32_t ks;              // K steps per row (32 elements for 16-bit, 128 for FP8)
     uint32_t qw;              // queries per wave
     uint32_t qb;              // queries per workgroup (4 * qw)
@@ -37,7 +39,7 @@ struct FlatMfmaPlan {
     uint32_t bound_rows;      // rows covered by the first pass
     uint32_t mask_blocks;
     size_t off_gthr, off_counts, off_cand, off_qh, off_qscale, off_rows, off_nrows, off_blocks,
-        off_minima;
+        off_minima, off_sel, off_seln, off_err;
     size_t workspace_bytes;
 };
 

@@ -182,3 +182,65 @@ def test_f64_boundary_equals_fp64_oracle(torch, metric):
     out = idx.search(qs, 40, metric, shortlist=3072)
     torch.cuda.synchronize()
     assert_same(L.split_results(out), O.flat_topk(dbn, qsn, 40, metric))
+
+
+@pytest.mark.parametrize("f64_boundary", [False, True])
+def test_fp8_deep_mixed_k_exact(torch, f64_boundary):
+    """k log-uniform on [1, 1024] with a 4k shortlist (up to 4096 rows: the
+    first pass takes 8192 subset minima per query, the scan prunes by
+    workgroup). Integer data: equal to the oracle for every query."""
+    db = int14(60000, 256, 21)
+    qs = int14(100, 256, 22)
+    ks = S.log_uniform_k(100, 1024, 23)
+    idx = F.FlatIndex(dev(torch, db), "fp8")
+    out = idx.search(dev(torch, qs), 1024, "ip", shortlist_factor=4,
+                     top_k_per_query=dev(torch, ks), f64_boundary=f64_boundary)
+    torch.cuda.synchronize()
+    ids, d = L.split_results(out)
+    e_ids, e_d = O.flat_topk(db, qs, 1024, "ip")
+    for i in range(100):
+        k = ks[i]
+        np.testing.assert_array_equal(ids[i, :k], e_ids[i, :k])
+        np.testing.assert_array_equal(d[i, :k].view(np.uint32), e_d[i, :k].view(np.uint32))
+        assert (ids[i, k:] == 0xFFFFFFFF).all()
+
+
+def test_graph_replay_matches_eager(torch):
+    """A whole mixed-k FP8 batch captured as one HIP graph: every replay,
+    with other eager searches and synchronizations in between, reproduces
+    the eager result exactly."""
+    db = int14(30000, 768, 31)
+    qs = int14(200, 768, 32)
+    ks = dev(torch, S.log_uniform_k(200, 256, 33))
+    idx = F.FlatIndex(dev(torch, db), "fp8")
+    other = F.FlatIndex(dev(torch, int14(5000, 768, 34)), "fp8")
+    d_q = dev(torch, qs)
+    out = L.result_tensor(200, 256, d_q.device)
+
+    def step():
+        idx.search(d_q, 256, "ip", shortlist_factor=4, top_k_per_query=ks, f64_boundary=True,
+                   out=out)
+
+    step()
+    torch.cuda.synchronize()
+    eager = out.clone()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        step()
+    torch.cuda.current_stream().wait_stream(side)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        step()
+    for r in range(3):
+        out.zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager), f"replay {r}"
+        other.search(d_q, 16, "ip")
+        torch.cuda.synchronize()
+    e_ids, _ = O.flat_topk(db, qs, 256, "ip")
+    ids, _ = L.split_results(eager)
+    k = ks.cpu().numpy()
+    for i in range(200):
+        np.testing.assert_array_equal(ids[i, :k[i]], e_ids[i, :k[i]])
