@@ -121,6 +121,8 @@ SIGNATURES = {
     "fv_ivf_pq_list_order_codes": (I, [P, P, P]),
     "fv_ivf_pq_workspace_size": (SZ, [P, U, U, U, U]),
     "fv_ivf_pq_search": (I, [P, P, P, U, P, U, P, U, U, I, I, P, P, P, SZ, P]),
+    "fv_topk_ip_backward": (I, [P, U, P, U, U, P, U, P, P, ctypes.c_float, ctypes.c_float, P, P,
+                                P, P, P]),
     "fv_kernel_timer_enable": (I, [I]),
     "fv_kernel_timer_read": (I, [P, P, I]),
 }

@@ -113,6 +113,27 @@ int fv_ivf_pq_search(const IvfPqGpuTables* tables, const uint8_t* d_codes_list_o
                      const uint32_t* d_valid_mask, GpuSearchResult* d_out_results,
                      void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* ---- top-k inner-product backward ----------------------------------------
+ * Scores s_ij = q_i . x_{ids[i][j]} of the rows a top-k search selected
+ * (d_ids [num_queries * top_k], 0xFFFFFFFF = empty). With d_grad_scores
+ * (dL/ds, same shape) the kernel forms dL/dq_i = sum_j g_ij x_{ids[i][j]}
+ * into d_grad_queries and, when d_grad_vectors is non-NULL, scatter-adds
+ * g_ij q_i into the selected rows of d_grad_vectors ([num_vectors * dim],
+ * accumulated with float atomics: the caller zeroes it). With
+ * d_grad_scores NULL it recomputes the scores and uses the gradient of the
+ * softmax cross-entropy of s_i * inv_tau against the target position
+ * d_targets[i], times loss_scale and times *d_grad_output when that device
+ * scalar is non-NULL (the upstream gradient of the loss, read on the device);
+ * the recomputed scores are written to d_scores_out when non-NULL.
+ * dim <= 4096, top_k <= 1024. No counterpart in the reference (it has no
+ * training path). */
+int fv_topk_ip_backward(const float* d_queries, uint32_t num_queries, const float* d_vectors,
+                        uint32_t num_vectors, uint32_t dim, const uint32_t* d_ids,
+                        uint32_t top_k, const float* d_grad_scores, const uint32_t* d_targets,
+                        float inv_tau, float loss_scale, const float* d_grad_output,
+                        float* d_grad_queries, float* d_grad_vectors, float* d_scores_out,
+                        void* stream);
+
 /* ---- kernel timer --------------------------------------------------------
  * While enabled, every search call brackets its dominant scan kernel with two
  * HIP events recorded on the call's stream. fv_kernel_timer_read waits for the
