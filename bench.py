@@ -278,19 +278,34 @@ def bench_ivfpq(args):
     def step():
         idx.search(w.queries, 100, 64, S, top_k=w.top_k, nprobe=w.nprobe, out=out)
 
-    for _ in range(args.warmup):
+    for _ in range(max(1, args.warmup)):
         step()
     torch.cuda.synchronize()
+    eager = out.clone()
     steps = max(1, args.steps // 4)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     with L.KernelTimer() as kt:
-        e0.record()
         for _ in range(steps):
             step()
-        e1.record()
         torch.cuda.synchronize()
         scan_ms, launches = kt.read()
+    # The whole uneven batch as one captured HIP graph; replays are the step.
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        step()
+    torch.cuda.current_stream().wait_stream(side)
+    with torch.cuda.graph(g):
+        step()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        g.replay()
+    e1.record()
+    torch.cuda.synchronize()
     step_ms = e0.elapsed_time(e1) / steps
+    graph_matches_eager = bool(torch.equal(out, eager))
     scan_avg = scan_ms / max(launches, 1)
     nbytes = W.scanned_bytes(w, S)
     ids, dists = L.split_results(out)
@@ -340,6 +355,7 @@ def bench_ivfpq(args):
                    "nprobe": "per query in {8,16,32,64}", "top_k": "per query in {10,100}"},
         "recall_at_k_vs_exact": round(hits / tot, 5), "recall_sample_queries": rs,
         "ids_bit_exact_vs_cpu_adc": ids_match,
+        "hip_graph": True, "graph_replay_matches_eager": graph_matches_eager,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(achieved / 8000.0, 4),
                      "traffic": measured_traffic("ivf_scan_kernel/ivfpq_uneven_10m"),

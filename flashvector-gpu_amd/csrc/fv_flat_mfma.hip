@@ -554,6 +554,7 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
     if (c > 256u) c = 256u;
     if (c >= 8u) c &= ~7u;
     if (c < 1u) c = 1u;
+    if (a.n_chunks) c = a.n_chunks < 4096u ? a.n_chunks : 4096u;
     p.n_chunks = c;
     // First pass: 16 subset minima per query per chunk over the first
     // bound_rows positions (a quarter of the rows, a sixteenth for rows wider
@@ -563,12 +564,14 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
     uint32_t brows = p.qrow_bytes <= 256 ? rows_plan / 4 : rows_plan / 16;
     const uint32_t bmin = rows_plan < 262144u ? rows_plan : 262144u;
     if (brows < bmin) brows = bmin;
+    if (a.bound_div && a.bound_div != 0xFFFFFFFFu) brows = rows_plan / a.bound_div;
     p.bound_rows = brows;
     uint32_t m = 1024u;
     while (m < 4u * smax && m < 16384u) m *= 2u;
     uint32_t bc = m / 16u;
     while (bc > 8u && (uint64_t)bc * 64u > brows) bc /= 2u;  // at least one tile per chunk
     p.bound_chunks = (16u * bc >= smax && brows >= bc * 64u) ? bc : 0u;
+    if (a.bound_div == 0xFFFFFFFFu) p.bound_chunks = 0u;
     size_t off = 0;
     auto take = [&](size_t bytes) {
         size_t o = off;
@@ -695,10 +698,22 @@ static bool make_args(const FvFlatTables* t, const FvFlatSearchParams* sp, FlatS
     a.metric = (int)sp->metric;
     a.dim = t->dim;
     a.flags = sp->flags;
+    a.n_chunks = sp->n_chunks;
+    a.bound_div = sp->bound_div;
     if (a.top_k == 0 || a.top_k > FV_MAX_TOPK) return false;
     if (sp->metric > METRIC_INNER_PRODUCT || (sp->flags & ~FV_SEARCH_F64_BOUNDARY) != 0) return false;
     a.s_max = deepest_shortlist(a);
     return true;
+}
+
+bool flat_plan_for(const FvFlatTables* t, uint32_t nq, const FvFlatSearchParams* sp,
+                   FlatMfmaPlan* out) {
+    if (!t || !sp || !out || t->num_vectors == 0 || check_tables(t) != 0) return false;
+    FlatSearchArgs a;
+    if (!make_args(t, sp, a)) return false;
+    *out = plan_flat_mfma(t->num_vectors, t->padded_dim, (int)t->dtype, nq, a,
+                          sp->d_valid_mask != nullptr, sp->valid_count);
+    return out->ks != 0 && out->lcap <= 8192u;
 }
 
 }  // namespace fv

@@ -16,6 +16,8 @@ struct FlatSearchArgs {
     uint32_t dim;
     int metric;
     uint32_t flags;           // FV_SEARCH_* bits
+    uint32_t n_chunks;        // plan override (0 = automatic)
+    uint32_t bound_div;       // plan override (0 = automatic, ~0u = no first pass)
 };
 
 struct FlatMfmaPlan {
@@ -76,6 +78,10 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
 void launch_encode_fp8_rows(const float* x, uint32_t n, uint32_t dim, uint32_t out_stride,
                             uint32_t total_rows, float sign, uint8_t* out, float* scales,
                             hipStream_t stream);
+
+// Plan of fv_flat_search for these arguments (false if they are invalid).
+bool flat_plan_for(const FvFlatTables* t, uint32_t nq, const FvFlatSearchParams* sp,
+                   FlatMfmaPlan* out);
 
 // First-pass bound selection (fv_flat_mfma.hip), called by the scan launchers.
 void launch_bound_select(const ScanLaunch& L);

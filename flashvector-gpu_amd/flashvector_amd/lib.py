@@ -80,6 +80,21 @@ class FvFlatSearchParams(ctypes.Structure):
         ("shortlist_factor", ctypes.c_uint32),
         ("shortlist", ctypes.c_uint32),
         ("flags", ctypes.c_uint32),
+        ("n_chunks", ctypes.c_uint32),
+        ("bound_div", ctypes.c_uint32),
+    ]
+
+
+class FvFlatTuning(ctypes.Structure):
+    """include/flashvector/fv_ext.h FvFlatTuning."""
+
+    _fields_ = [
+        ("shortlist_factor", ctypes.c_uint32),
+        ("n_chunks", ctypes.c_uint32),
+        ("bound_div", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("ms", ctypes.c_float),
+        ("recall", ctypes.c_float),
     ]
 
 
@@ -121,6 +136,7 @@ SIGNATURES = {
     "fv_ivf_pq_list_order_codes": (I, [P, P, P]),
     "fv_ivf_pq_workspace_size": (SZ, [P, U, U, U, U]),
     "fv_ivf_pq_search": (I, [P, P, P, U, P, U, P, U, U, I, I, P, P, P, SZ, P]),
+    "fv_flat_autotune": (I, [P, P, U, P, ctypes.c_float, U, P, P]),
     "fv_topk_ip_backward": (I, [P, U, P, U, U, P, U, P, P, ctypes.c_float, ctypes.c_float, P, P,
                                 P, P, P]),
     "fv_kernel_timer_enable": (I, [I]),

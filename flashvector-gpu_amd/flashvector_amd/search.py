@@ -130,17 +130,39 @@ class FlatIndex:
         self._ws = None
 
     def params(self, top_k, metric="l2", valid_mask=None, valid_count=0, shortlist=0,
-               shortlist_factor=0, top_k_per_query=None, f64_boundary=False):
+               shortlist_factor=0, top_k_per_query=None, f64_boundary=False, tuning=None):
+        """FvFlatSearchParams; `tuning` (an FvFlatTuning, e.g. from autotune)
+        supplies the shortlist factor and plan overrides when the caller
+        gives no shortlist of its own."""
+        n_chunks = bound_div = 0
+        if tuning is not None:
+            n_chunks, bound_div = int(tuning.n_chunks), int(tuning.bound_div)
+            if not shortlist and not shortlist_factor:
+                shortlist_factor = int(tuning.shortlist_factor)
         return L.FvFlatSearchParams(L.ptr(valid_mask), L.ptr(top_k_per_query), int(valid_count),
                                     _metric(metric), int(top_k), int(shortlist_factor),
                                     int(shortlist),
-                                    FV_SEARCH_F64_BOUNDARY if f64_boundary else 0)
+                                    FV_SEARCH_F64_BOUNDARY if f64_boundary else 0,
+                                    n_chunks, bound_div)
+
+    def autotune(self, queries, top_k, metric="l2", valid_mask=None, valid_count=0,
+                 target_recall=0.999, trials=3, f64_boundary=False):
+        """fv_flat_autotune on a query sample; returns an FvFlatTuning."""
+        lib = L.load()
+        base = self.params(top_k, metric, valid_mask, valid_count, f64_boundary=f64_boundary)
+        best = L.FvFlatTuning()
+        rc = lib.fv_flat_autotune(ctypes.byref(self.tables), L.ptr(queries), queries.shape[0],
+                                  ctypes.byref(base), ctypes.c_float(target_recall), int(trials),
+                                  ctypes.byref(best), L.stream_handle())
+        L.check(rc, "fv_flat_autotune")
+        return best
 
     def workspace_size(self, nq, params):
         return L.load().fv_flat_workspace_size(ctypes.byref(self.tables), nq, ctypes.byref(params))
 
     def search(self, queries, top_k, metric="l2", valid_mask=None, shortlist=0, out=None,
-               valid_count=0, shortlist_factor=0, top_k_per_query=None, f64_boundary=False):
+               valid_count=0, shortlist_factor=0, top_k_per_query=None, f64_boundary=False,
+               tuning=None):
         """fv_flat_search. `valid_count` (set bits of the mask, if known) lets
         sparse masks be compacted; `top_k_per_query` is an optional int32
         device tensor of per-query k (each <= top_k); `f64_boundary` sets
@@ -149,7 +171,7 @@ class FlatIndex:
         lib = L.load()
         nq = queries.shape[0]
         prm = self.params(top_k, metric, valid_mask, valid_count, shortlist, shortlist_factor,
-                          top_k_per_query, f64_boundary)
+                          top_k_per_query, f64_boundary, tuning)
         need = self.workspace_size(nq, prm)
         if need == 0:
             raise L.FvError("invalid flat search shape")

@@ -86,6 +86,12 @@ typedef struct {
     uint32_t shortlist_factor;
     uint32_t shortlist;
     uint32_t flags;               /* FV_SEARCH_* bits, others must be 0 */
+    /* Plan overrides (0 = automatic), normally set from fv_flat_autotune:
+     * row chunks per query block (the scan's work-item shape), and the
+     * fraction of rows the first pass scans as a divisor (1, 2, 4, ...;
+     * 0xFFFFFFFF = no first pass). */
+    uint32_t n_chunks;
+    uint32_t bound_div;
 } FvFlatSearchParams;
 
 size_t fv_flat_workspace_size(const FvFlatTables* tables, uint32_t num_queries,
@@ -93,6 +99,28 @@ size_t fv_flat_workspace_size(const FvFlatTables* tables, uint32_t num_queries,
 int fv_flat_search(const FvFlatTables* tables, const float* d_queries, uint32_t num_queries,
                    const FvFlatSearchParams* params, GpuSearchResult* d_out_results,
                    void* d_workspace, size_t workspace_bytes, void* stream);
+
+/* Autotuner of the flat search for one (table, query distribution, k,
+ * metric, dtype): on the query sample d_queries it (1) computes the exact
+ * FP32 top-k (fv_flat_exact_search, so tables->d_vectors is required), (2)
+ * picks the smallest shortlist factor in {1, 2, 4, 8, 16} whose recall@k
+ * against it is at least target_recall (16 if none), (3) times every
+ * combination of n_chunks in {auto/2, auto, 2*auto, 4*auto} and bound_div in
+ * {auto, 2*auto, 4*auto, none} with that factor (best of `trials` runs after a
+ * warm-up) and returns the fastest. `base` supplies k, metric, mask and
+ * flags. Allocates and frees its own device memory and synchronizes the
+ * stream: a set-up call, not for the search path. */
+typedef struct {
+    uint32_t shortlist_factor;
+    uint32_t n_chunks;
+    uint32_t bound_div;
+    uint32_t reserved;
+    float ms;                     /* best time of the chosen plan */
+    float recall;                 /* recall@k of the chosen shortlist factor */
+} FvFlatTuning;
+int fv_flat_autotune(const FvFlatTables* tables, const float* d_queries, uint32_t num_queries,
+                     const FvFlatSearchParams* base, float target_recall, uint32_t trials,
+                     FvFlatTuning* out_best, void* stream);
 
 /* ---- IVF-PQ scan with uneven batches ------------------------------------
  * Same arithmetic and ties rule as cuda_ivf_pq_search_batch, without its 256

@@ -244,3 +244,23 @@ def test_graph_replay_matches_eager(torch):
     k = ks.cpu().numpy()
     for i in range(200):
         np.testing.assert_array_equal(ids[i, :k[i]], e_ids[i, :k[i]])
+
+
+def test_autotune_meets_recall_and_keeps_results(torch, tmp_path, monkeypatch):
+    """fv_flat_autotune picks a shortlist factor that meets the recall target
+    on its sample; searching with the tuned plan on integer data still equals
+    the oracle; the cache returns the same tuning without re-measuring."""
+    from flashvector_amd import autotune as T
+    monkeypatch.setenv("FLASHVECTOR_TUNE_CACHE", str(tmp_path / "tune.json"))
+    db, qs = S.flat_workload(60000, 256, 128, 64, seeds=(101, 102, 103))
+    idx = F.FlatIndex(dev(torch, db), "f16")
+    d_q = dev(torch, qs)
+    best = T.tuned(idx, d_q, 10, "l2")
+    assert best.recall >= 0.999 and best.shortlist_factor in (1, 2, 4, 8, 16)
+    assert best.n_chunks >= 1 and best.ms > 0
+    out = idx.search(d_q, 10, "l2", tuning=best)
+    torch.cuda.synchronize()
+    assert_same(L.split_results(out), O.flat_topk(db, qs, 10, "l2"))
+    again = T.tuned(idx, d_q, 10, "l2")
+    assert (again.shortlist_factor, again.n_chunks, again.bound_div) == \
+        (best.shortlist_factor, best.n_chunks, best.bound_div)

@@ -107,3 +107,41 @@ def test_exhaustive_probe_matches_oracle_and_flat(torch, case):
     assert_same(got, O.ivf_pq_search_batch(h, qs[:20], 10, t.nlist, 10240))
     flat = O.flat_scan_batch(db, qs[:20], 10)
     assert np.mean(got[0] == flat[0]) >= 0.99
+
+
+def test_uneven_batch_graph_replay(torch, case):
+    """The whole uneven batch captured as one HIP graph: every replay, with
+    eager searches and synchronizations in between, equals the eager result
+    and the oracle."""
+    db, qs, t, idx, h = case
+    rng = np.random.default_rng(91)
+    ks = rng.integers(1, 21, size=qs.shape[0]).astype(np.int32)
+    nps = rng.integers(1, 17, size=qs.shape[0]).astype(np.int32)
+    d_q, d_k, d_np = dev(torch, qs), dev(torch, ks), dev(torch, nps)
+    out = L.result_tensor(qs.shape[0], 20, d_q.device)
+
+    def step():
+        idx.search(d_q, 20, 16, 60, top_k=d_k, nprobe=d_np, out=out)
+
+    step()
+    torch.cuda.synchronize()
+    eager = out.clone()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        step()
+    torch.cuda.current_stream().wait_stream(side)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        step()
+    other = L.result_tensor(qs.shape[0], 5, d_q.device)
+    for r in range(3):
+        out.zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager), f"replay {r}"
+        idx.search(d_q, 5, 4, 20, out=other)
+        torch.cuda.synchronize()
+    e_ids, e_d = O.ivf_pq_search_uneven(h, qs, ks.astype(np.uint32), nps.astype(np.uint32),
+                                        shortlist=60, out_stride=20)
+    assert_same(L.split_results(eager), (e_ids, e_d))
