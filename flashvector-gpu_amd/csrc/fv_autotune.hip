@@ -20,7 +20,7 @@ struct DevBuf {
     size_t bytes = 0;
     hipError_t ensure(size_t need) {
         if (need <= bytes) return hipSuccess;
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
         hipError_t e = hipMalloc(&p, need);
@@ -28,7 +28,7 @@ struct DevBuf {
         return e;
     }
     ~DevBuf() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     }
 };
 
@@ -117,7 +117,7 @@ extern "C" int fv_flat_autotune(const FvFlatTables* t, const float* d_queries, u
     hipEvent_t e0, e1;
     if ((e = hipEventCreate(&e0)) != hipSuccess) return (int)e;
     if ((e = hipEventCreate(&e1)) != hipSuccess) {
-        hipEventDestroy(e0);
+        (void)hipEventDestroy(e0);
         return (int)e;
     }
     FvFlatTuning best{};
@@ -129,12 +129,12 @@ extern "C" int fv_flat_autotune(const FvFlatTables* t, const float* d_queries, u
             if ((rc = run(p)) != 0) break;  // warm-up
             float ms_best = 3.0e38f;
             for (uint32_t r = 0; r < std::max(1u, trials) && rc == 0; ++r) {
-                hipEventRecord(e0, s);
+                (void)hipEventRecord(e0, s);
                 rc = run(p);
-                hipEventRecord(e1, s);
-                hipEventSynchronize(e1);
+                (void)hipEventRecord(e1, s);
+                (void)hipEventSynchronize(e1);
                 float ms = 0.0f;
-                hipEventElapsedTime(&ms, e0, e1);
+                (void)hipEventElapsedTime(&ms, e0, e1);
                 ms_best = std::min(ms_best, ms);
             }
             if (rc) break;
@@ -146,8 +146,8 @@ extern "C" int fv_flat_autotune(const FvFlatTables* t, const float* d_queries, u
         }
         if (rc) break;
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (rc) return rc;
     best.shortlist_factor = factor;
     best.recall = recall;

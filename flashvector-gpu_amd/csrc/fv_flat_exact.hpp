@@ -35,4 +35,9 @@ void fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t stream);
 hipError_t fill_results(GpuSearchResult* out, size_t count, uint32_t id_bits, uint32_t dist_bits,
                         hipStream_t stream);
 
+// Top-k merge of num_shards lists per query (fv_merge.hip), out_k <= 1024.
+hipError_t launch_merge_topk(const GpuSearchResult* in, uint32_t num_shards, uint32_t nq,
+                             uint32_t shard_k, uint32_t out_k, GpuSearchResult* out,
+                             hipStream_t stream);
+
 }  // namespace fv

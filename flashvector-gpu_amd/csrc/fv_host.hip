@@ -1,0 +1,566 @@
+// Host search engine (fvh_*): stream pool with pre-allocated per-stream
+// scratch, device-side mask build, main + delta search on two streams,
+// device merge. Counterpart of crates/engine/src/index.rs:1264-1487,
+// 1979-2237 and crates/engine/src/streams.rs:22-143.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/flashvector/cuda_bridge.h"
+#include "../../include/flashvector/fv_host.h"
+#include "fv_flat_exact.hpp"
+
+namespace fvh {
+
+using fv::kWave;
+
+// Slot-indexed validity words: bit s is set when slot s is alive (valid
+// words) and, with a caller mask, its logical id's bit is set there
+// (index.rs:1264-1316, evaluated per slot on the device). One thread per
+// slot; each wavefront writes the two words of its 64 slots.
+__global__ __launch_bounds__(256) void slot_mask_kernel(const uint32_t* __restrict__ valid,
+                                                        const uint32_t* __restrict__ slot2lid,
+                                                        uint32_t n,
+                                                        const uint32_t* __restrict__ user,
+                                                        uint32_t user_words,
+                                                        uint32_t* __restrict__ out) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    bool ok = s < n && ((valid[s >> 5] >> (s & 31u)) & 1u);
+    if (ok && user) {
+        const uint32_t lid = slot2lid ? slot2lid[s] : s;
+        const uint32_t w = lid >> 5;
+        ok = w < user_words && ((user[w] >> (lid & 31u)) & 1u);
+    }
+    const uint64_t b = __ballot(ok);
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t base = (s - lane) >> 5;  // first word of this wavefront
+    const uint32_t nwords = (n + 31) / 32;
+    if (lane == 0 && base < nwords) out[base] = (uint32_t)b;
+    if (lane == 32 && base + 1 < nwords) out[base + 1] = (uint32_t)(b >> 32);
+}
+
+// Result slot -> logical id.
+__global__ void remap_ids_kernel(GpuSearchResult* __restrict__ r, size_t count,
+                                 const uint32_t* __restrict__ map, uint32_t map_n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t id = r[i].id;
+    if (id != FV_EMPTY_ID && id < map_n) r[i].id = map[id];
+}
+
+template <typename T>
+struct DevArray {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count) {
+        release();
+        if (count == 0) return hipSuccess;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+        if (e == hipSuccess) n = count;
+        else p = nullptr;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevArray() { release(); }
+};
+
+template <typename T>
+struct PinnedArray {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count) {
+        release();
+        if (count == 0) return hipSuccess;
+        hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&p), count * sizeof(T), hipHostMallocDefault);
+        if (e == hipSuccess) n = count;
+        else p = nullptr;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~PinnedArray() { release(); }
+};
+
+// Per-stream scratch (streams.rs:22-69): everything one search touches.
+struct Scratch {
+    hipStream_t stream = nullptr;
+    hipEvent_t queries_ready = nullptr, delta_done = nullptr, free_ev = nullptr;
+    bool used = false;
+    std::mutex mu;
+    DevArray<float> q;
+    DevArray<GpuSearchResult> res, merge;
+    DevArray<uint32_t> mask, delta_mask, user_mask;
+    DevArray<uint8_t> ws, dws;
+    PinnedArray<float> hq;
+    PinnedArray<GpuSearchResult> hres;
+    PinnedArray<uint32_t> hmask;
+    ~Scratch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (queries_ready) (void)hipEventDestroy(queries_ready);
+        if (delta_done) (void)hipEventDestroy(delta_done);
+        if (free_ev) (void)hipEventDestroy(free_ev);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+}  // namespace fvh
+
+struct FvEngine {
+    FvEngineConfig cfg{};
+    std::vector<std::unique_ptr<fvh::Scratch>> pool;
+    std::atomic<uint64_t> rr{0};
+    hipStream_t delta_stream = nullptr;
+    int mode = 0;  // 0 none, 1 flat, 2 IVF-PQ
+    uint32_t n_main = 0;
+    // flat tables
+    fvh::DevArray<float> rows, norms, scales;
+    fvh::DevArray<double> norms64;
+    fvh::DevArray<uint8_t> codes;
+    FvFlatTables flat{};
+    // IVF-PQ tables
+    fvh::DevArray<float> centroids, codebooks;
+    fvh::DevArray<uint8_t> pq_codes, pq_codes_lo;
+    fvh::DevArray<uint32_t> offsets, vec_ids;
+    IvfPqGpuTables ivf{};
+    // ids, tombstones
+    fvh::DevArray<uint32_t> slot2lid, valid;
+    std::vector<uint32_t> h_slot2lid, h_valid;
+    std::unordered_map<uint32_t, uint32_t> lid2slot;
+    uint32_t main_dead = 0;
+    uint32_t max_lid = 0;
+    // delta buffer
+    uint32_t n_delta = 0, delta_cap = 0, delta_dead = 0;
+    fvh::DevArray<float> delta_rows;
+    fvh::DevArray<uint32_t> delta_ids, delta_alive;
+    std::vector<uint32_t> h_delta_ids, h_delta_alive;
+    std::unordered_map<uint32_t, uint32_t> lid2delta;
+    // stats (metrics.rs)
+    std::atomic<uint64_t> queries{0}, total_ns{0};
+
+    ~FvEngine() {
+        pool.clear();
+        if (delta_stream) {
+            (void)hipStreamSynchronize(delta_stream);
+            (void)hipStreamDestroy(delta_stream);
+        }
+    }
+};
+
+namespace fvh {
+
+static uint32_t words_for(uint32_t n) { return (n + 31) / 32; }
+
+static hipError_t upload(void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return hipSuccess;
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+}
+
+static FvFlatSearchParams flat_params(const FvEngine* e, uint32_t k, const uint32_t* mask) {
+    FvFlatSearchParams p{};
+    p.d_valid_mask = mask;
+    p.metric = e->cfg.metric;
+    p.top_k = k;
+    p.shortlist_factor = e->cfg.shortlist_factor;
+    p.flags = e->cfg.flags;
+    return p;
+}
+
+// Sizes every stream's scratch for the loaded tables (largest query batch,
+// largest k); called from the load / add paths, never from a search.
+static hipError_t size_scratch(FvEngine* e) {
+    const FvEngineConfig& c = e->cfg;
+    const uint32_t B = c.max_batch, K = c.max_top_k;
+    size_t ws = 0;
+    if (e->mode == 1) {
+        for (uint32_t nq = B; nq >= 1; nq /= 2) {
+            FvFlatSearchParams p = flat_params(e, K, e->valid.p);
+            ws = std::max(ws, fv_flat_workspace_size(&e->flat, nq, &p));
+            if (nq == 1) break;
+        }
+        if (ws == 0) return hipErrorInvalidValue;
+    } else if (e->mode == 2) {
+        for (uint32_t nq = B; nq >= 1; nq /= 2) {
+            ws = std::max(ws, fv_ivf_pq_workspace_size(&e->ivf, nq, c.ivf_nprobe, c.ivf_shortlist, K));
+            if (nq == 1) break;
+        }
+        if (ws == 0) return hipErrorInvalidValue;
+    }
+    size_t dws = 0;
+    if (e->delta_cap)
+        for (uint32_t nq = B; nq >= 1; nq /= 2) {
+            dws = std::max(dws, fv_flat_exact_workspace_size(e->delta_cap, c.dim, nq, K,
+                                                             (MetricType)c.metric));
+            if (nq == 1) break;
+        }
+    const uint32_t umw = words_for(e->max_lid + 1);
+    for (auto& sp : e->pool) {
+        Scratch& s = *sp;
+        hipError_t r;
+        if (s.ws.n < ws && (r = s.ws.alloc(ws)) != hipSuccess) return r;
+        if (s.dws.n < dws && (r = s.dws.alloc(dws)) != hipSuccess) return r;
+        if (s.mask.n < words_for(e->n_main) && (r = s.mask.alloc(words_for(e->n_main))) != hipSuccess) return r;
+        if (s.delta_mask.n < words_for(e->delta_cap) &&
+            (r = s.delta_mask.alloc(words_for(e->delta_cap))) != hipSuccess)
+            return r;
+        if (s.user_mask.n < umw && (r = s.user_mask.alloc(umw)) != hipSuccess) return r;
+        if (s.hmask.n < umw && (r = s.hmask.alloc(umw)) != hipSuccess) return r;
+    }
+    return hipSuccess;
+}
+
+static void track_lids(FvEngine* e, const uint32_t* lids, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) e->max_lid = std::max(e->max_lid, lids ? lids[i] : i);
+}
+
+// One device pass of at most max_batch queries (index.rs:2180-2237 for the
+// flat mode, :1979-2108 for IVF-PQ, :1353-1471 for the delta scan and merge).
+static int search_pass(FvEngine* e, Scratch& s, hipStream_t st, const float* d_q, uint32_t nq,
+                       uint32_t k, const uint32_t* d_user, uint32_t user_words,
+                       GpuSearchResult* d_out) {
+    hipError_t he;
+    const FvEngineConfig& c = e->cfg;
+    if (s.used && (he = hipStreamWaitEvent(st, s.free_ev, 0)) != hipSuccess) return (int)he;
+    const bool has_delta = e->n_delta > e->delta_dead;
+    GpuSearchResult* main_out = has_delta ? s.merge.p : d_out;
+    GpuSearchResult* delta_out = s.merge.p + (size_t)nq * k;
+    // Delta scan on the second stream, after the queries are in place.
+    if (has_delta) {
+        if ((he = hipEventRecord(s.queries_ready, st)) != hipSuccess) return (int)he;
+        if ((he = hipStreamWaitEvent(e->delta_stream, s.queries_ready, 0)) != hipSuccess) return (int)he;
+        const uint32_t* dmask = nullptr;
+        if (d_user || e->delta_dead) {
+            slot_mask_kernel<<<(e->n_delta + 255) / 256, 256, 0, e->delta_stream>>>(
+                e->delta_alive.p, e->delta_ids.p, e->n_delta, d_user, user_words, s.delta_mask.p);
+            dmask = s.delta_mask.p;
+        }
+        int rc = fv_flat_exact_search(e->delta_rows.p, e->n_delta, c.dim, d_q, nq, k,
+                                      (MetricType)c.metric, dmask, delta_out, s.dws.p, s.dws.n,
+                                      e->delta_stream);
+        if (rc) return rc;
+        const size_t cnt = (size_t)nq * k;
+        remap_ids_kernel<<<(unsigned)((cnt + 255) / 256), 256, 0, e->delta_stream>>>(
+            delta_out, cnt, e->delta_ids.p, e->n_delta);
+        if ((he = hipEventRecord(s.delta_done, e->delta_stream)) != hipSuccess) return (int)he;
+    }
+    // Main index.
+    const uint32_t* mmask = nullptr;
+    if (d_user || e->main_dead) {
+        slot_mask_kernel<<<(e->n_main + 255) / 256, 256, 0, st>>>(
+            e->valid.p, e->slot2lid.p, e->n_main, d_user, user_words, s.mask.p);
+        mmask = s.mask.p;
+    }
+    int rc;
+    if (e->mode == 1) {
+        FvFlatSearchParams p = flat_params(e, k, mmask);
+        rc = fv_flat_search(&e->flat, d_q, nq, &p, main_out, s.ws.p, s.ws.n, st);
+    } else {
+        rc = fv_ivf_pq_search(&e->ivf, e->pq_codes_lo.p, d_q, nq, nullptr, k, nullptr,
+                              c.ivf_nprobe, c.ivf_shortlist, 1, (MetricType)c.metric, mmask,
+                              main_out, s.ws.p, s.ws.n, st);
+    }
+    if (rc) return rc;
+    if (e->slot2lid.p) {
+        const size_t cnt = (size_t)nq * k;
+        remap_ids_kernel<<<(unsigned)((cnt + 255) / 256), 256, 0, st>>>(main_out, cnt,
+                                                                      e->slot2lid.p, e->n_main);
+    }
+    if (has_delta) {
+        if ((he = hipStreamWaitEvent(st, s.delta_done, 0)) != hipSuccess) return (int)he;
+        if ((he = fv::launch_merge_topk(s.merge.p, 2, nq, k, k, d_out, st)) != hipSuccess) return (int)he;
+    }
+    if ((he = hipEventRecord(s.free_ev, st)) != hipSuccess) return (int)he;
+    s.used = true;
+    he = hipGetLastError();
+    return he == hipSuccess ? 0 : (int)he;
+}
+
+static Scratch& pick(FvEngine* e) {
+    const uint64_t i = e->rr.fetch_add(1, std::memory_order_relaxed);  // streams.rs:117-120
+    return *e->pool[i % e->pool.size()];
+}
+
+static int check_search(const FvEngine* e, uint32_t top_k) {
+    if (e->mode == 0) return -1;
+    if (top_k == 0 || top_k > e->cfg.max_top_k) return (int)hipErrorInvalidValue;
+    if (e->n_delta > e->delta_dead && 2ull * top_k > 4096ull) return (int)hipErrorInvalidValue;
+    return 0;
+}
+
+}  // namespace fvh
+
+extern "C" {
+
+int fvh_create(const FvEngineConfig* cfg, FvEngine** out) {
+    if (!cfg || !out) return -1;
+    *out = nullptr;
+    if (cfg->dim == 0 || cfg->metric > METRIC_INNER_PRODUCT || cfg->max_top_k > FV_MAX_TOPK)
+        return (int)hipErrorInvalidValue;
+    std::unique_ptr<FvEngine> e(new FvEngine());
+    e->cfg = *cfg;
+    FvEngineConfig& c = e->cfg;
+    if (!c.num_streams) c.num_streams = 4;
+    if (!c.max_batch) c.max_batch = 1024;
+    if (!c.max_top_k) c.max_top_k = 256;
+    if (!c.shortlist_factor) c.shortlist_factor = 8;
+    if (!c.ivf_nprobe) c.ivf_nprobe = 32;
+    if (!c.ivf_shortlist) c.ivf_shortlist = 1000;
+    hipError_t he = hipStreamCreateWithFlags(&e->delta_stream, hipStreamNonBlocking);
+    if (he != hipSuccess) return (int)he;
+    for (uint32_t i = 0; i < c.num_streams; ++i) {
+        std::unique_ptr<fvh::Scratch> s(new fvh::Scratch());
+        if ((he = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess ||
+            (he = hipEventCreateWithFlags(&s->queries_ready, hipEventDisableTiming)) != hipSuccess ||
+            (he = hipEventCreateWithFlags(&s->delta_done, hipEventDisableTiming)) != hipSuccess ||
+            (he = hipEventCreateWithFlags(&s->free_ev, hipEventDisableTiming)) != hipSuccess ||
+            (he = s->q.alloc((size_t)c.max_batch * c.dim)) != hipSuccess ||
+            (he = s->res.alloc((size_t)c.max_batch * c.max_top_k)) != hipSuccess ||
+            (he = s->merge.alloc((size_t)2 * c.max_batch * c.max_top_k)) != hipSuccess ||
+            (he = s->hq.alloc((size_t)c.max_batch * c.dim)) != hipSuccess ||
+            (he = s->hres.alloc((size_t)c.max_batch * c.max_top_k)) != hipSuccess)
+            return (int)he;
+        e->pool.push_back(std::move(s));
+    }
+    *out = e.release();
+    return 0;
+}
+
+void fvh_destroy(FvEngine* e) { delete e; }
+
+static int set_ids(FvEngine* e, const uint32_t* lids, uint32_t n) {
+    e->lid2slot.clear();
+    e->h_slot2lid.clear();
+    e->slot2lid.release();
+    if (lids) {
+        e->h_slot2lid.assign(lids, lids + n);
+        for (uint32_t s = 0; s < n; ++s) e->lid2slot[lids[s]] = s;
+        hipError_t he = e->slot2lid.alloc(n);
+        if (he == hipSuccess) he = fvh::upload(e->slot2lid.p, lids, (size_t)n * 4);
+        if (he != hipSuccess) return (int)he;
+    }
+    fvh::track_lids(e, lids, n);
+    e->h_valid.assign(fvh::words_for(n), 0xFFFFFFFFu);
+    if (n % 32) e->h_valid.back() = (1u << (n % 32)) - 1u;
+    e->main_dead = 0;
+    hipError_t he = e->valid.alloc(e->h_valid.size());
+    if (he == hipSuccess) he = fvh::upload(e->valid.p, e->h_valid.data(), e->h_valid.size() * 4);
+    return (int)he;
+}
+
+int fvh_load_flat(FvEngine* e, const float* h_vectors, uint32_t n, uint32_t dtype,
+                  const uint32_t* h_logical_ids) {
+    if (!e || !h_vectors) return -1;
+    if (n == 0 || (dtype != FV_DTYPE_F16 && dtype != FV_DTYPE_BF16 && dtype != FV_DTYPE_FP8_E4M3))
+        return (int)hipErrorInvalidValue;
+    const uint32_t dim = e->cfg.dim;
+    const uint32_t step = dtype == FV_DTYPE_FP8_E4M3 ? 128u : 32u;
+    const uint32_t pd = (dim + step - 1) / step * step;
+    if (pd > 768) return (int)hipErrorInvalidValue;
+    const size_t esz = dtype == FV_DTYPE_FP8_E4M3 ? 1 : 2;
+    hipError_t he;
+    if ((he = e->rows.alloc((size_t)n * dim)) != hipSuccess ||
+        (he = e->codes.alloc((size_t)n * pd * esz)) != hipSuccess ||
+        (he = e->norms.alloc(n)) != hipSuccess || (he = e->norms64.alloc(n)) != hipSuccess ||
+        (he = e->scales.alloc(dtype == FV_DTYPE_FP8_E4M3 ? n : 0)) != hipSuccess ||
+        (he = fvh::upload(e->rows.p, h_vectors, (size_t)n * dim * 4)) != hipSuccess)
+        return (int)he;
+    int rc = fv_encode_flat(e->rows.p, n, dim, pd, dtype, e->codes.p, e->norms.p, e->norms64.p,
+                            e->scales.p, nullptr);
+    if (rc) return rc;
+    if ((he = hipDeviceSynchronize()) != hipSuccess) return (int)he;
+    e->flat = FvFlatTables{e->codes.p, e->scales.p, e->norms.p, e->norms64.p, e->rows.p, n, dim, pd, dtype};
+    e->n_main = n;
+    e->mode = 1;
+    if ((rc = set_ids(e, h_logical_ids, n))) return rc;
+    return (int)fvh::size_scratch(e);
+}
+
+int fvh_load_ivfpq(FvEngine* e, const float* h_centroids, uint32_t nlist, const float* h_codebooks,
+                   uint32_t m_pq, const uint8_t* h_codes, const uint32_t* h_offsets,
+                   const uint32_t* h_vec_ids, const float* h_vectors, uint32_t n,
+                   const uint32_t* h_logical_ids) {
+    if (!e || !h_centroids || !h_codebooks || !h_codes || !h_offsets || !h_vec_ids || !h_vectors)
+        return -1;
+    const uint32_t dim = e->cfg.dim;
+    if (n == 0 || nlist == 0 || m_pq == 0 || dim % m_pq) return (int)hipErrorInvalidValue;
+    hipError_t he;
+    if ((he = e->centroids.alloc((size_t)nlist * dim)) != hipSuccess ||
+        (he = e->codebooks.alloc((size_t)m_pq * 256 * (dim / m_pq))) != hipSuccess ||
+        (he = e->pq_codes.alloc((size_t)n * m_pq)) != hipSuccess ||
+        (he = e->pq_codes_lo.alloc((size_t)n * m_pq)) != hipSuccess ||
+        (he = e->offsets.alloc(nlist + 1)) != hipSuccess || (he = e->vec_ids.alloc(n)) != hipSuccess ||
+        (he = e->rows.alloc((size_t)n * dim)) != hipSuccess ||
+        (he = fvh::upload(e->centroids.p, h_centroids, (size_t)nlist * dim * 4)) != hipSuccess ||
+        (he = fvh::upload(e->codebooks.p, h_codebooks, (size_t)m_pq * 256 * (dim / m_pq) * 4)) != hipSuccess ||
+        (he = fvh::upload(e->pq_codes.p, h_codes, (size_t)n * m_pq)) != hipSuccess ||
+        (he = fvh::upload(e->offsets.p, h_offsets, (size_t)(nlist + 1) * 4)) != hipSuccess ||
+        (he = fvh::upload(e->vec_ids.p, h_vec_ids, (size_t)n * 4)) != hipSuccess ||
+        (he = fvh::upload(e->rows.p, h_vectors, (size_t)n * dim * 4)) != hipSuccess)
+        return (int)he;
+    e->ivf = IvfPqGpuTables{e->centroids.p, e->codebooks.p, e->pq_codes.p, e->offsets.p,
+                            e->vec_ids.p, e->rows.p, n, dim, nlist, m_pq, dim / m_pq};
+    int rc = fv_ivf_pq_list_order_codes(&e->ivf, e->pq_codes_lo.p, nullptr);
+    if (rc) return rc;
+    if ((he = hipDeviceSynchronize()) != hipSuccess) return (int)he;
+    e->n_main = n;
+    e->mode = 2;
+    if ((rc = set_ids(e, h_logical_ids, n))) return rc;
+    return (int)fvh::size_scratch(e);
+}
+
+int fvh_add_delta(FvEngine* e, const float* h_vectors, const uint32_t* h_logical_ids, uint32_t n) {
+    if (!e || !h_vectors || !h_logical_ids) return -1;
+    if (n == 0) return 0;
+    const uint32_t dim = e->cfg.dim;
+    const uint32_t need = e->n_delta + n;
+    hipError_t he;
+    if (need > e->delta_cap) {
+        uint32_t cap = std::max(need, e->delta_cap * 2);
+        fvh::DevArray<float> rows;
+        if ((he = rows.alloc((size_t)cap * dim)) != hipSuccess) return (int)he;
+        if (e->n_delta &&
+            (he = hipMemcpy(rows.p, e->delta_rows.p, (size_t)e->n_delta * dim * 4,
+                            hipMemcpyDeviceToDevice)) != hipSuccess)
+            return (int)he;
+        std::swap(rows.p, e->delta_rows.p);
+        std::swap(rows.n, e->delta_rows.n);
+        if ((he = e->delta_ids.alloc(cap)) != hipSuccess ||
+            (he = e->delta_alive.alloc(fvh::words_for(cap))) != hipSuccess)
+            return (int)he;
+        e->delta_cap = cap;
+    }
+    if ((he = fvh::upload(e->delta_rows.p + (size_t)e->n_delta * dim, h_vectors,
+                          (size_t)n * dim * 4)) != hipSuccess)
+        return (int)he;
+    for (uint32_t i = 0; i < n; ++i) {
+        e->lid2delta[h_logical_ids[i]] = e->n_delta + i;
+        e->h_delta_ids.push_back(h_logical_ids[i]);
+    }
+    e->n_delta = need;
+    e->h_delta_alive.assign(fvh::words_for(e->delta_cap), 0u);
+    for (uint32_t i = 0; i < e->n_delta; ++i) e->h_delta_alive[i >> 5] |= 1u << (i & 31u);
+    e->delta_dead = 0;
+    fvh::track_lids(e, h_logical_ids, n);
+    if ((he = fvh::upload(e->delta_ids.p, e->h_delta_ids.data(), (size_t)e->n_delta * 4)) != hipSuccess ||
+        (he = fvh::upload(e->delta_alive.p, e->h_delta_alive.data(), e->h_delta_alive.size() * 4)) != hipSuccess)
+        return (int)he;
+    return (int)fvh::size_scratch(e);
+}
+
+int fvh_delete(FvEngine* e, const uint32_t* h_logical_ids, uint32_t n) {
+    if (!e || !h_logical_ids) return -1;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t lid = h_logical_ids[i];
+        uint32_t slot = 0xFFFFFFFFu;
+        if (e->slot2lid.p) {
+            auto it = e->lid2slot.find(lid);
+            if (it != e->lid2slot.end()) slot = it->second;
+        } else if (lid < e->n_main) {
+            slot = lid;
+        }
+        if (slot != 0xFFFFFFFFu && ((e->h_valid[slot >> 5] >> (slot & 31u)) & 1u)) {
+            e->h_valid[slot >> 5] &= ~(1u << (slot & 31u));
+            ++e->main_dead;
+        }
+        auto dt = e->lid2delta.find(lid);
+        if (dt != e->lid2delta.end()) {
+            const uint32_t j = dt->second;
+            if ((e->h_delta_alive[j >> 5] >> (j & 31u)) & 1u) {
+                e->h_delta_alive[j >> 5] &= ~(1u << (j & 31u));
+                ++e->delta_dead;
+            }
+        }
+    }
+    hipError_t he = fvh::upload(e->valid.p, e->h_valid.data(), e->h_valid.size() * 4);
+    if (he == hipSuccess && e->delta_alive.p)
+        he = fvh::upload(e->delta_alive.p, e->h_delta_alive.data(), e->h_delta_alive.size() * 4);
+    return (int)he;
+}
+
+int fvh_search(FvEngine* e, const float* h_queries, uint32_t nq, uint32_t top_k,
+               const uint32_t* h_mask_words, uint32_t mask_words, GpuSearchResult* h_out) {
+    if (!e || !h_queries || !h_out) return -1;
+    int rc = fvh::check_search(e, top_k);
+    if (rc) return rc;
+    if (nq == 0) return 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    fvh::Scratch& s = fvh::pick(e);
+    std::lock_guard<std::mutex> lock(s.mu);
+    const uint32_t dim = e->cfg.dim, B = e->cfg.max_batch;
+    hipError_t he;
+    const uint32_t* d_user = nullptr;
+    uint32_t uw = 0;
+    if (h_mask_words) {
+        uw = std::min<uint32_t>(mask_words, (uint32_t)s.user_mask.n);
+        if (s.used && (he = hipEventSynchronize(s.free_ev)) != hipSuccess) return (int)he;
+        std::memcpy(s.hmask.p, h_mask_words, (size_t)uw * 4);
+        if (uw && (he = hipMemcpyAsync(s.user_mask.p, s.hmask.p, (size_t)uw * 4,
+                                       hipMemcpyHostToDevice, s.stream)) != hipSuccess)
+            return (int)he;
+        d_user = s.user_mask.p;
+    }
+    for (uint32_t q0 = 0; q0 < nq; q0 += B) {
+        const uint32_t nb = std::min(B, nq - q0);
+        if (s.used && (he = hipEventSynchronize(s.free_ev)) != hipSuccess) return (int)he;
+        std::memcpy(s.hq.p, h_queries + (size_t)q0 * dim, (size_t)nb * dim * 4);
+        if ((he = hipMemcpyAsync(s.q.p, s.hq.p, (size_t)nb * dim * 4, hipMemcpyHostToDevice,
+                                 s.stream)) != hipSuccess)
+            return (int)he;
+        if ((rc = fvh::search_pass(e, s, s.stream, s.q.p, nb, top_k, d_user, uw, s.res.p))) return rc;
+        if ((he = hipMemcpyAsync(s.hres.p, s.res.p, (size_t)nb * top_k * sizeof(GpuSearchResult),
+                                 hipMemcpyDeviceToHost, s.stream)) != hipSuccess ||
+            (he = hipStreamSynchronize(s.stream)) != hipSuccess)
+            return (int)he;
+        std::memcpy(h_out + (size_t)q0 * top_k, s.hres.p, (size_t)nb * top_k * sizeof(GpuSearchResult));
+    }
+    e->queries.fetch_add(nq, std::memory_order_relaxed);
+    e->total_ns.fetch_add((uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                              std::chrono::steady_clock::now() - t0).count(),
+                          std::memory_order_relaxed);
+    return 0;
+}
+
+int fvh_search_device(FvEngine* e, const float* d_queries, uint32_t nq, uint32_t top_k,
+                      const uint32_t* d_mask_words, uint32_t mask_words, GpuSearchResult* d_out,
+                      void* stream) {
+    if (!e || !d_queries || !d_out) return -1;
+    int rc = fvh::check_search(e, top_k);
+    if (rc) return rc;
+    if (nq == 0) return 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    fvh::Scratch& s = fvh::pick(e);
+    std::lock_guard<std::mutex> lock(s.mu);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const uint32_t dim = e->cfg.dim, B = e->cfg.max_batch;
+    for (uint32_t q0 = 0; q0 < nq; q0 += B) {
+        const uint32_t nb = std::min(B, nq - q0);
+        if ((rc = fvh::search_pass(e, s, st, d_queries + (size_t)q0 * dim, nb, top_k, d_mask_words,
+                                   d_mask_words ? mask_words : 0u, d_out + (size_t)q0 * top_k)))
+            return rc;
+    }
+    e->queries.fetch_add(nq, std::memory_order_relaxed);
+    e->total_ns.fetch_add((uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                              std::chrono::steady_clock::now() - t0).count(),
+                          std::memory_order_relaxed);
+    return 0;
+}
+
+int fvh_stats(const FvEngine* e, uint64_t* queries, double* total_ms) {
+    if (!e) return -1;
+    if (queries) *queries = e->queries.load();
+    if (total_ms) *total_ms = (double)e->total_ns.load() / 1e6;
+    return 0;
+}
+
+}  // extern "C"

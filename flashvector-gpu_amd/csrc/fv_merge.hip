@@ -53,6 +53,20 @@ __global__ __launch_bounds__(256) void merge_topk_kernel(const GpuSearchResult* 
     }
 }
 
+// Merge without the reference entry point's 256 clamp (out_k <= 1024 while
+// num_shards * shard_k <= 4096); used by the host engine's main + delta merge.
+hipError_t launch_merge_topk(const GpuSearchResult* in, uint32_t num_shards, uint32_t nq,
+                             uint32_t shard_k, uint32_t out_k, GpuSearchResult* out,
+                             hipStream_t stream) {
+    if (num_shards == 0 || nq == 0 || out_k == 0) return hipSuccess;
+    uint64_t total64 = (uint64_t)num_shards * shard_k;
+    uint32_t total = total64 > kMergeCap ? kMergeCap : (uint32_t)total64;
+    uint32_t n_pow2 = next_pow2(total < 2u ? 2u : total);
+    merge_topk_kernel<<<nq, 256, (size_t)2 * n_pow2 * sizeof(uint64_t), stream>>>(
+        in, num_shards, nq, shard_k ? shard_k : 1u, total, n_pow2, out_k, out);
+    return hipGetLastError();
+}
+
 }  // namespace fv
 
 extern "C" int cuda_merge_topk(const GpuSearchResult* d_shard_results, uint32_t num_shards,

@@ -85,6 +85,14 @@ class FvFlatSearchParams(ctypes.Structure):
     ]
 
 
+class FvEngineConfig(ctypes.Structure):
+    """include/flashvector/fv_host.h FvEngineConfig."""
+
+    _fields_ = [(n, ctypes.c_uint32) for n in (
+        "dim", "metric", "num_streams", "max_batch", "max_top_k", "shortlist_factor", "flags",
+        "ivf_nprobe", "ivf_shortlist", "reserved")]
+
+
 class FvFlatTuning(ctypes.Structure):
     """include/flashvector/fv_ext.h FvFlatTuning."""
 
@@ -137,6 +145,15 @@ SIGNATURES = {
     "fv_ivf_pq_workspace_size": (SZ, [P, U, U, U, U]),
     "fv_ivf_pq_search": (I, [P, P, P, U, P, U, P, U, U, I, I, P, P, P, SZ, P]),
     "fv_flat_autotune": (I, [P, P, U, P, ctypes.c_float, U, P, P]),
+    "fvh_create": (I, [P, P]),
+    "fvh_destroy": (None, [P]),
+    "fvh_load_flat": (I, [P, P, U, U, P]),
+    "fvh_load_ivfpq": (I, [P, P, U, P, U, P, P, P, P, U, P]),
+    "fvh_add_delta": (I, [P, P, P, U]),
+    "fvh_delete": (I, [P, P, U]),
+    "fvh_search": (I, [P, P, U, U, P, U, P]),
+    "fvh_search_device": (I, [P, P, U, U, P, U, P, P]),
+    "fvh_stats": (I, [P, P, P]),
     "fv_topk_ip_backward": (I, [P, U, P, U, U, P, U, P, P, ctypes.c_float, ctypes.c_float, P, P,
                                 P, P, P]),
     "fv_kernel_timer_enable": (I, [I]),

@@ -11,7 +11,7 @@ from flashvector_amd import lib as L
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADERS = [os.path.join(ROOT, "include", "flashvector", h)
-           for h in ("cuda_bridge.h", "fv_ext.h")]
+           for h in ("cuda_bridge.h", "fv_ext.h", "fv_host.h")]
 
 
 def declared_symbols():
@@ -19,7 +19,7 @@ def declared_symbols():
     for h in HEADERS:
         text = open(h).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        names += re.findall(r"^\s*(?:int|size_t|void|uint32_t)\s+(fv_\w+|cuda_\w+)\s*\(",
+        names += re.findall(r"^\s*(?:int|size_t|void|uint32_t)\s+(fvh_\w+|fv_\w+|cuda_\w+)\s*\(",
                             text, flags=re.M)
     return names
 
@@ -62,3 +62,13 @@ def test_null_pointers_rejected(so):
 def test_workspace_sizes_are_host_only(so):
     assert so.fv_flat_exact_workspace_size(1000, 128, 10, 10, 0) > 0
     assert so.fv_flat_exact_workspace_size(1000, 128, 10, 2000, 0) == 0   # k above the limit
+
+
+def test_host_engine_rejects_null_and_bad_config(so):
+    assert so.fvh_create(None, None) == -1
+    assert so.fvh_search(None, None, 1, 1, None, 0, None) == -1
+    assert so.fvh_search_device(None, None, 1, 1, None, 0, None, None) == -1
+    assert so.fvh_load_flat(None, None, 1, 1, None) == -1
+    assert so.fvh_delete(None, None, 1) == -1
+    assert so.fvh_stats(None, None, None) == -1
+    assert ctypes.sizeof(L.FvEngineConfig) == 40
