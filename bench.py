@@ -77,6 +77,8 @@ def main():
                     help="second mask density reported beside the headline (0 = skip)")
     ap.add_argument("--workload", choices=["flat", "ivfpq", "fp8", "backward"], default="flat",
                     help="flat: the headline matrix-core scan; ivfpq: 10M-row IVF-PQ uneven batch")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the built-in plan instead of the autotuned one")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy first (reference point for traffic counters)")
     args = ap.parse_args()
@@ -121,10 +123,18 @@ def main():
     idx = F.FlatIndex(d_db, args.dtype)
     out = L.result_tensor(nq, k, d_q.device)
     torch.cuda.synchronize()
+    tuning = None
+    if not args.no_autotune:
+        from flashvector_amd import autotune as T
+        log("autotuning the scan plan on this batch (shortlist fixed at 8k)")
+        tuning = T.tuned(idx, d_q, k, "l2", d_m, valid_count, shortlist_factor=8)
+
+    def search(m=d_m, vc=valid_count, tn=tuning):
+        idx.search(d_q, k, "l2", m, out=out, valid_count=vc, shortlist_factor=8, tuning=tn)
 
     log("warmup")
     for _ in range(args.warmup):
-        idx.search(d_q, k, "l2", d_m, out=out, valid_count=valid_count)
+        search()
     torch.cuda.synchronize()
 
     log(f"timing {args.steps} steps")
@@ -134,7 +144,7 @@ def main():
         t0 = time.perf_counter()
         start.record()
         for _ in range(args.steps):
-            idx.search(d_q, k, "l2", d_m, out=out, valid_count=valid_count)
+            search()
         stop.record()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
@@ -182,13 +192,16 @@ def main():
         abits = S.unpack_bits(amask, n)
         d_am = torch.from_numpy(amask).cuda()
         avc = int(abits.sum())
-        idx.search(d_q, k, "l2", d_am, out=out, valid_count=avc)
+        atun = None
+        if not args.no_autotune:
+            atun = T.tuned(idx, d_q, k, "l2", d_am, avc, shortlist_factor=8)
+        search(d_am, avc, atun)
         torch.cuda.synchronize()
         a0, a1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         asteps = max(1, args.steps // 2)
         a0.record()
         for _ in range(asteps):
-            idx.search(d_q, k, "l2", d_am, out=out, valid_count=avc)
+            search(d_am, avc, atun)
         a1.record()
         torch.cuda.synchronize()
         ams = a0.elapsed_time(a1) / asteps
@@ -231,7 +244,10 @@ def main():
         "data": "synthetic (seeded integer Gaussian mixture, 256 clusters)",
         "config": {"workload": "flat_l2_mfma_rerank", "n": n, "dim": dim, "num_queries": nq,
                    "top_k": k, "shortlist": 8 * k, "valid_fraction": args.valid_frac,
-                   "storage": args.dtype, "parallelism": "single_gpu"},
+                   "storage": args.dtype, "parallelism": "single_gpu",
+                   "plan": None if tuning is None else {
+                       "autotuned": True, "n_chunks": int(tuning.n_chunks),
+                       "bound_div": int(tuning.bound_div)}},
         "recall_at_k": round(recall, 5),
         "valid_rows": valid_count,
         "recall_sample_queries": rs,

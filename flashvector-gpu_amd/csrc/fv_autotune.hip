@@ -83,7 +83,8 @@ extern "C" int fv_flat_autotune(const FvFlatTables* t, const float* d_queries, u
                               ws.bytes, stream);
     };
 
-    // (1) Shortlist depth: the smallest factor that meets the recall target.
+    // (1) Shortlist depth: the smallest factor that meets the recall target,
+    // unless the caller fixed it (then only its recall is measured).
     FvFlatSearchParams p = *base;
     p.shortlist = 0;
     p.n_chunks = 0;
@@ -91,7 +92,10 @@ extern "C" int fv_flat_autotune(const FvFlatTables* t, const float* d_queries, u
     p.d_top_k = nullptr;
     float recall = 0.0f;
     uint32_t factor = 16;
-    for (uint32_t f : {1u, 2u, 4u, 8u, 16u}) {
+    const bool fixed = base->shortlist_factor != 0;
+    const std::vector<uint32_t> factors =
+        fixed ? std::vector<uint32_t>{base->shortlist_factor} : std::vector<uint32_t>{1u, 2u, 4u, 8u, 16u};
+    for (uint32_t f : factors) {
         p.shortlist_factor = f;
         if ((rc = run(p)) != 0) return rc;
         if ((e = hipMemcpyAsync(got.data(), res.p, res_bytes, hipMemcpyDeviceToHost, s)) != hipSuccess ||

@@ -21,45 +21,61 @@ namespace fv {
 
 constexpr uint32_t kSelectBatch = 256;
 
-__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* s_red) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    __syncthreads();
-    if ((threadIdx.x & (kWave - 1)) == 0) s_red[threadIdx.x / kWave] = v;
-    __syncthreads();
-    uint32_t t = 0;
-    for (uint32_t w = 0; w < blockDim.x / kWave; ++w) t += s_red[w];
-    return t;
-}
-
-// Per query: a value with at least S of the first pass's subset minima at or
-// below it (the S-th smallest, or slightly above it), found by bisection over
-// the ordered distance held in LDS. Each minimum is the score of a distinct
-// row, so at least S rows score at or below the result: it bounds the global
-// S-th best and seeds the shared bound of the main pass.
+// Per query (one wavefront each, 4 per workgroup): a value with at least S of
+// the first pass's subset minima at or below it (the S-th smallest, or
+// slightly above it), found by bisection over the ordered distance with
+// ballot counts; the minima stay in registers (NR per lane, m <= 64*NR) or,
+// for NR = 0, in LDS (one wavefront per workgroup). Each minimum is the score
+// of a distinct row, so at least S rows score at or below the result: it
+// bounds the global S-th best and seeds the shared bound of the main pass.
+template <int NR>
 __global__ __launch_bounds__(256) void bound_select_kernel(const float* __restrict__ minima,
                                                            uint32_t m, uint32_t nq,
                                                            FlatSearchArgs a,
                                                            uint32_t* __restrict__ gthr) {
-    extern __shared__ uint32_t s_o[];
-    __shared__ uint32_t s_red[256 / kWave];
-    const uint32_t q = blockIdx.x;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t q = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (q >= nq) return;
     uint32_t k, S;
     query_k_s(a, q, k, S);
+    constexpr int kR = NR > 0 ? NR : 1;
+    uint32_t o[kR];
+    extern __shared__ uint32_t s_o[];
+    const uint32_t nr = (m + kWave - 1) / kWave;
+    auto load = [&](uint32_t r) -> uint32_t {
+        const uint32_t i = r * kWave + lane;
+        uint32_t v = 0xFFFFFFFFu;
+        if (i < m) {
+            const float f = minima[((size_t)(i >> 4) * nq + q) * 16 + (i & 15u)];
+            if (f < kAdmit) v = ord_f32(f);
+        }
+        return v;
+    };
     uint32_t valid = 0;
-    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
-        const uint32_t c = i >> 4, e = i & 15u;
-        const float v = minima[((size_t)c * nq + q) * 16 + e];
-        const uint32_t o = v < kAdmit ? ord_f32(v) : 0xFFFFFFFFu;
-        s_o[i] = o;
-        valid += o != 0xFFFFFFFFu;
+    if (NR > 0) {
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+            o[r] = load((uint32_t)r);
+            valid += (uint32_t)__popcll(__ballot(o[r] != 0xFFFFFFFFu));
+        }
+    } else {
+        for (uint32_t r = 0; r < nr; ++r) {
+            const uint32_t v = load(r);
+            s_o[r * kWave + lane] = v;
+            valid += (uint32_t)__popcll(__ballot(v != 0xFFFFFFFFu));
+        }
     }
-    if (block_sum_u32(valid, s_red) < S) return;
+    if (valid < S) return;
     uint32_t lo = 0u, hi = 0xFFFFFFFEu;
     while (lo < hi) {
         const uint32_t mid = lo + ((hi - lo) >> 1);
         uint32_t c = 0;
-        for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) c += s_o[i] <= mid;
-        c = block_sum_u32(c, s_red);
+        if (NR > 0) {
+#pragma unroll
+            for (int r = 0; r < kR; ++r) c += (uint32_t)__popcll(__ballot(o[r] <= mid));
+        } else {
+            for (uint32_t r = 0; r < nr; ++r) c += (uint32_t)__popcll(__ballot(s_o[r * kWave + lane] <= mid));
+        }
         if (c >= S) {
             hi = mid;
             if (c <= S + (S >> 3) + 16u) break;
@@ -67,7 +83,7 @@ __global__ __launch_bounds__(256) void bound_select_kernel(const float* __restri
             lo = mid + 1u;
         }
     }
-    if (threadIdx.x == 0) atomicMin(&gthr[q], hi);
+    if (lane == 0) atomicMin(&gthr[q], hi);
 }
 
 // One workgroup per query: streams the candidates of every chunk that are
@@ -127,37 +143,46 @@ __global__ __launch_bounds__(256) void flat_select_kernel(
     if (threadIdx.x == 0) sel_n[q] = have;
 }
 
-// Exact FP32 rerank of the selected rows: one workgroup per (query, group of
-// 256 selected rows), one row per thread. Rows are staged through LDS in
-// slabs of 32 dims with coalesced loads; each thread keeps its row's sums in
-// the reference CPU path's order (sequential over d, non-fused), so the
-// distance equals exact_distance<METRIC> bit for bit. Writes (distance, row)
-// keys in place and, when err != NULL, the FP32 error bound of each distance.
-// Without FP32 rows the approximate score is turned into a distance instead.
+// Exact FP32 rerank of the selected rows. Workgroup b covers items
+// [256 b, 256 b + 256) of the padded [query][s_max] array of selected keys,
+// one item per thread (a workgroup may span several queries; items past a
+// query's count are idle). Rows are staged through LDS in slabs of 32 dims
+// with coalesced loads, the workgroup's queries too when they fit; each
+// thread keeps its row's sums in the reference CPU path's order (sequential
+// over d, non-fused), so the distance equals exact_distance<METRIC> bit for
+// bit. Writes (distance, row) keys in place and, when err != NULL, the FP32
+// error bound of each distance. Without FP32 rows the approximate score is
+// turned into a distance instead.
 constexpr uint32_t kRerankRows = 256;
 constexpr uint32_t kRerankSlab = 32;
+constexpr uint32_t kRerankQFloats = 4096;
 
 template <int METRIC>
 __global__ __launch_bounds__(256) void flat_rerank_kernel(
-    const float* __restrict__ X32, const float* __restrict__ Qm, uint32_t dim, uint32_t smax,
-    uint32_t groups, const uint32_t* __restrict__ sel_n, uint64_t* __restrict__ sel,
+    const float* __restrict__ X32, const float* __restrict__ Qm, uint32_t dim, uint32_t nq,
+    uint32_t smax, const uint32_t* __restrict__ sel_n, uint64_t* __restrict__ sel,
     float* __restrict__ err) {
     __shared__ float s_x[kRerankRows * (kRerankSlab + 1)];
-    __shared__ float s_q[768];
+    __shared__ float s_qv[kRerankQFloats];
     __shared__ uint32_t s_row[kRerankRows];
-    const uint32_t q = blockIdx.x / groups, g = blockIdx.x % groups;
-    const uint32_t have = sel_n[q];
-    const uint32_t i0 = g * kRerankRows;
-    if (i0 >= have) return;
-    const uint32_t rows_here = min(kRerankRows, have - i0);
+    __shared__ uint8_t s_act[kRerankRows];
     const uint32_t tid = threadIdx.x;
-    uint64_t* keys = sel + (size_t)q * smax + i0;
-    const uint64_t my_key = tid < rows_here ? keys[tid] : kEmptyKey;
-    s_row[tid] = key_slot(tid < rows_here ? my_key : keys[0]);
-    const float* qv = Qm + (size_t)q * dim;
-    for (uint32_t d = tid; d < dim; d += blockDim.x) s_q[d] = qv[d];
+    const size_t item = (size_t)blockIdx.x * kRerankRows + tid;
+    const size_t first = (size_t)blockIdx.x * kRerankRows;
+    const uint32_t q = (uint32_t)(item / smax), j = (uint32_t)(item % smax);
+    const bool act = q < nq && j < sel_n[q];
+    const uint64_t my_key = act ? sel[item] : kEmptyKey;
+    s_row[tid] = act ? key_slot(my_key) : 0u;
+    s_act[tid] = act ? 1 : 0;
+    // Queries of this workgroup: q0 .. q1.
+    const uint32_t q0 = (uint32_t)(first / smax);
+    const uint32_t q1 = (uint32_t)min((size_t)nq - 1, (first + kRerankRows - 1) / smax);
+    const bool staged = (size_t)(q1 - q0 + 1) * dim <= kRerankQFloats;
+    if (staged)
+        for (uint32_t i = tid; i < (q1 - q0 + 1) * dim; i += blockDim.x) s_qv[i] = Qm[(size_t)q0 * dim + i];
     __syncthreads();
-    float d32;
+    const float* qv = act ? (staged ? &s_qv[(q - q0) * dim] : Qm + (size_t)q * dim) : s_qv;
+    float d32 = 0.0f;
     float e = 0.0f;
     if (X32) {
         // Sums in the reference order plus the running error bound of each
@@ -174,9 +199,9 @@ __global__ __launch_bounds__(256) void flat_rerank_kernel(
             if (vec) {
 #pragma unroll
                 for (uint32_t m = 0; m < 8; ++m) {
-                    const uint32_t j = tid + 256u * m;
-                    const uint32_t row = j >> 3, part = j & 7u;
-                    if (row < rows_here && part * 4u < w) {
+                    const uint32_t jj = tid + 256u * m;
+                    const uint32_t row = jj >> 3, part = jj & 7u;
+                    if (s_act[row] && part * 4u < w) {
                         const float4 v = *reinterpret_cast<const float4*>(
                             X32 + (size_t)s_row[row] * dim + d0 + part * 4u);
                         float* dst = &s_x[row * (kRerankSlab + 1) + part * 4u];
@@ -188,17 +213,17 @@ __global__ __launch_bounds__(256) void flat_rerank_kernel(
                 }
             } else {
                 for (uint32_t m = 0; m < kRerankSlab; ++m) {
-                    const uint32_t j = tid + 256u * m;
-                    const uint32_t row = j / kRerankSlab, c = j % kRerankSlab;
-                    if (row < rows_here && c < w)
+                    const uint32_t jj = tid + 256u * m;
+                    const uint32_t row = jj / kRerankSlab, c = jj % kRerankSlab;
+                    if (s_act[row] && c < w)
                         s_x[row * (kRerankSlab + 1) + c] = X32[(size_t)s_row[row] * dim + d0 + c];
                 }
             }
             __syncthreads();
-            if (tid < rows_here) {
+            if (act) {
                 const float* xr = &s_x[tid * (kRerankSlab + 1)];
                 for (uint32_t dd = 0; dd < w; ++dd) {
-                    const float a = s_q[d0 + dd], b = xr[dd];
+                    const float a = qv[d0 + dd], b = xr[dd];
                     if (METRIC == METRIC_L2) {
                         const float diff = a - b;
                         const float sq = diff * diff;
@@ -239,20 +264,20 @@ __global__ __launch_bounds__(256) void flat_rerank_kernel(
                                           (run_x + nxa) / fmaxf(nxa, 1e-30f)) + 4.0f * u;
             e = margin * ((e_dot + fabsf(acc) * rel) / denom + u * fabsf(acc / denom) + u * fabsf(d32));
         }
-    } else {
+    } else if (act) {
         // No FP32 rows: report the reduced-precision score as a distance.
         const float s = key_dist(my_key);
         if (METRIC == METRIC_INNER_PRODUCT) {
             d32 = s;
         } else {
             float qn = 0.0f;
-            for (uint32_t dd = 0; dd < dim; ++dd) qn = __builtin_fmaf(s_q[dd], s_q[dd], qn);
+            for (uint32_t dd = 0; dd < dim; ++dd) qn = __builtin_fmaf(qv[dd], qv[dd], qn);
             d32 = METRIC == METRIC_L2 ? fmaxf(0.0f, qn + 2.0f * s) : 1.0f + s / (sqrtf(qn) + 1e-8f);
         }
     }
-    if (tid < rows_here) {
-        keys[tid] = d32 < kAdmit ? make_key(d32, s_row[tid]) : kEmptyKey;
-        if (err) err[(size_t)q * smax + i0 + tid] = e;
+    if (act) {
+        sel[item] = d32 < kAdmit ? make_key(d32, s_row[tid]) : kEmptyKey;
+        if (err) err[item] = e;
     }
 }
 
@@ -605,7 +630,12 @@ __global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {
 
 void launch_bound_select(const ScanLaunch& L) {
     const uint32_t m = L.p->bound_chunks * 16;
-    bound_select_kernel<<<L.nq, 256, (size_t)m * 4, L.stream>>>(L.minima, m, L.nq, L.a, L.gthr);
+    const dim3 g((L.nq + 3) / 4);
+    if (m <= 1024u) bound_select_kernel<16><<<g, 256, 0, L.stream>>>(L.minima, m, L.nq, L.a, L.gthr);
+    else if (m <= 4096u) bound_select_kernel<64><<<g, 256, 0, L.stream>>>(L.minima, m, L.nq, L.a, L.gthr);
+    else  // one wavefront per workgroup, minima in LDS (m * 4 bytes, m padded to whole wavefronts)
+        bound_select_kernel<0><<<L.nq, kWave, (size_t)((m + kWave - 1) / kWave) * kWave * 4, L.stream>>>(
+            L.minima, m, L.nq, L.a, L.gthr);
 }
 
 hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
@@ -661,17 +691,16 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
     float* err = (a.flags & FV_SEARCH_F64_BOUNDARY) ? reinterpret_cast<float*>(ws + p.off_err) : nullptr;
     flat_select_kernel<<<nq, 256, p.lds_select, stream>>>(cand, counts, p.n_chunks, nq, p.cap, gthr,
                                                           a, p.lcap, sel, sel_n);
-    const uint32_t groups = (p.s_max + kRerankRows - 1) / kRerankRows;
-    const dim3 rg((unsigned)((size_t)nq * groups));
+    const dim3 rg((unsigned)(((size_t)nq * p.s_max + kRerankRows - 1) / kRerankRows));
     const float* X32 = t->d_vectors;
     if (a.metric == METRIC_L2) {
-        flat_rerank_kernel<METRIC_L2><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, p.s_max, groups, sel_n, sel, err);
+        flat_rerank_kernel<METRIC_L2><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, nq, p.s_max, sel_n, sel, err);
         flat_finalize_kernel<METRIC_L2><<<nq, 256, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
     } else if (a.metric == METRIC_INNER_PRODUCT) {
-        flat_rerank_kernel<METRIC_INNER_PRODUCT><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, p.s_max, groups, sel_n, sel, err);
+        flat_rerank_kernel<METRIC_INNER_PRODUCT><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, nq, p.s_max, sel_n, sel, err);
         flat_finalize_kernel<METRIC_INNER_PRODUCT><<<nq, 256, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
     } else {
-        flat_rerank_kernel<METRIC_COSINE><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, p.s_max, groups, sel_n, sel, err);
+        flat_rerank_kernel<METRIC_COSINE><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, nq, p.s_max, sel_n, sel, err);
         flat_finalize_kernel<METRIC_COSINE><<<nq, 256, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
     }
     return hipGetLastError();

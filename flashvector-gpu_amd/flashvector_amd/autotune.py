@@ -21,9 +21,13 @@ def _cache_path() -> str:
                                        "flat_tuning.json"))
 
 
-def tuning_key(index, top_k, metric, rows, f64_boundary=False) -> str:
-    bucket = 1 << max(0, int(math.ceil(math.log2(max(rows, 1)))))
-    return f"d{index.dim}_n{bucket}_k{top_k}_{index.dtype}_{metric}_b{int(bool(f64_boundary))}"
+def _bucket(v: int) -> int:
+    return 1 << max(0, int(math.ceil(math.log2(max(v, 1)))))
+
+
+def tuning_key(index, top_k, metric, rows, nq, f64_boundary=False, shortlist_factor=0) -> str:
+    return (f"d{index.dim}_n{_bucket(rows)}_q{_bucket(nq)}_k{top_k}_{index.dtype}_{metric}"
+            f"_b{int(bool(f64_boundary))}_s{shortlist_factor}")
 
 
 def _load() -> dict:
@@ -54,16 +58,19 @@ def to_tuning(entry: dict):
 
 
 def tuned(index, sample_queries, top_k, metric="l2", valid_mask=None, valid_count=0,
-          target_recall=0.999, f64_boundary=False, refresh=False):
-    """The cached FvFlatTuning for this shape, measuring it on `sample_queries`
-    when absent (or when refresh is set)."""
+          target_recall=0.999, f64_boundary=False, refresh=False, shortlist_factor=0):
+    """The cached FvFlatTuning for this shape (rows, batch size, k, dtype,
+    metric), measuring it on `sample_queries` — a batch of the size that will
+    be searched, since the work shape depends on it — when absent (or when
+    refresh is set). A non-zero shortlist_factor is kept fixed."""
     rows = valid_count if (valid_mask is not None and valid_count) else index.n
-    key = tuning_key(index, top_k, metric, rows, f64_boundary)
+    key = tuning_key(index, top_k, metric, rows, sample_queries.shape[0], f64_boundary,
+                     shortlist_factor)
     table = _load()
     if key in table and not refresh:
         return to_tuning(table[key])
     best = index.autotune(sample_queries, top_k, metric, valid_mask, valid_count, target_recall,
-                          f64_boundary=f64_boundary)
+                          f64_boundary=f64_boundary, shortlist_factor=shortlist_factor)
     table[key] = {"shortlist_factor": best.shortlist_factor, "n_chunks": best.n_chunks,
                   "bound_div": best.bound_div, "ms": round(best.ms, 5),
                   "recall": round(best.recall, 6), "target_recall": target_recall}

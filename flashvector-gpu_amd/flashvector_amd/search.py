@@ -146,10 +146,12 @@ class FlatIndex:
                                     n_chunks, bound_div)
 
     def autotune(self, queries, top_k, metric="l2", valid_mask=None, valid_count=0,
-                 target_recall=0.999, trials=3, f64_boundary=False):
-        """fv_flat_autotune on a query sample; returns an FvFlatTuning."""
+                 target_recall=0.999, trials=3, f64_boundary=False, shortlist_factor=0):
+        """fv_flat_autotune on a query sample; returns an FvFlatTuning. A
+        non-zero shortlist_factor is kept and only the plan is tuned."""
         lib = L.load()
-        base = self.params(top_k, metric, valid_mask, valid_count, f64_boundary=f64_boundary)
+        base = self.params(top_k, metric, valid_mask, valid_count,
+                           shortlist_factor=shortlist_factor, f64_boundary=f64_boundary)
         best = L.FvFlatTuning()
         rc = lib.fv_flat_autotune(ctypes.byref(self.tables), L.ptr(queries), queries.shape[0],
                                   ctypes.byref(base), ctypes.c_float(target_recall), int(trials),

@@ -104,7 +104,8 @@ int fv_flat_search(const FvFlatTables* tables, const float* d_queries, uint32_t 
  * metric, dtype): on the query sample d_queries it (1) computes the exact
  * FP32 top-k (fv_flat_exact_search, so tables->d_vectors is required), (2)
  * picks the smallest shortlist factor in {1, 2, 4, 8, 16} whose recall@k
- * against it is at least target_recall (16 if none), (3) times every
+ * against it is at least target_recall (16 if none) — or keeps
+ * base->shortlist_factor when that is non-zero — (3) times every
  * combination of n_chunks in {auto/2, auto, 2*auto, 4*auto} and bound_div in
  * {auto, 2*auto, 4*auto, none} with that factor (best of `trials` runs after a
  * warm-up) and returns the fastest. `base` supplies k, metric, mask and
