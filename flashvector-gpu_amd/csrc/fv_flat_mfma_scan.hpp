@@ -248,7 +248,7 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
         s_cnt[my_ql] = 0;
         s_thr[my_ql] = my_live
             ? bound_from_ord(__hip_atomic_load(&gthr[my_q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            : kAdmit;
+            : -__builtin_inff();
     }
 
     const uint32_t ntiles = (r_end - r_begin + RT - 1) / RT;
@@ -349,14 +349,16 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
 #pragma unroll
         for (int j = 0; j < NJ; ++j) thr[j] = s_thr[wave * QW + 16 * j + l16];
 
-#pragma unroll
-        for (int rt = 0; rt < RT / 16; ++rt) {
+        // Row tile rt (16 rows): MMAs into acc; scores after the FP8 / cosine
+        // scaling; then selection. The MMAs of tile rt+1 are issued before
+        // the selection of tile rt, so that VALU work overlaps the matrix
+        // pipe.
+        auto mma_tile = [&](int rt, floatx4* acc) {
             const uint8_t* arow = &s_tile[buf][(rt * 16 + l16) * kLdsRow + grp * (LB * KS)];
             vec afrag[KS];
 #pragma unroll
             for (int s = 0; s < KS; ++s) afrag[s] = ld_vec<vec>(arow + LB * s);
             const floatx4 cin = *reinterpret_cast<const floatx4*>(&s_cinit[buf][rt * 16 + 4 * grp]);
-            floatx4 acc[NJ];
 #pragma unroll
             for (int j = 0; j < NJ; ++j) acc[j] = kFP8 ? floatx4{0.0f, 0.0f, 0.0f, 0.0f} : cin;
             // NJ independent accumulation chains, interleaved.
@@ -364,8 +366,11 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
             for (int s = 0; s < KS; ++s)
 #pragma unroll
                 for (int j = 0; j < NJ; ++j) acc[j] = Ops::mma(afrag[s], bfrag[j][s], acc[j]);
+        };
+        auto select_tile = [&](int rt, floatx4* acc) {
             if (kFP8 || METRIC == METRIC_COSINE) {
                 const floatx4 scl = *reinterpret_cast<const floatx4*>(&s_scale[buf][rt * 16 + 4 * grp]);
+                const floatx4 cin = *reinterpret_cast<const floatx4*>(&s_cinit[buf][rt * 16 + 4 * grp]);
 #pragma unroll
                 for (int j = 0; j < NJ; ++j) {
                     if (kFP8) acc[j] = cin + acc[j] * (scl * qsc[j]);
@@ -377,16 +382,19 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
                 for (int j = 0; j < NJ; ++j)
 #pragma unroll
                     for (int i = 0; i < 4; ++i) runmin[j][i] = fminf(runmin[j][i], acc[j][i]);
-                continue;
+                return;
             }
-            // Survivors are compacted wave-wide into the staging list (ballot +
-            // mbcnt), so a slot costs one compare unless some lane inserts.
+            // One compare per lane and query column on the minimum of its 4
+            // scores; only when some lane passes are the 4 scores staged
+            // (compacted wave-wide with ballot + mbcnt). Columns of queries
+            // past nq have threshold -inf.
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
-                const bool live_q = qw0 + 16 * j + l16 < nq;
+                const float m4 = fminf(fminf(acc[j][0], acc[j][1]), fminf(acc[j][2], acc[j][3]));
+                if (__ballot(m4 < thr[j]) == 0) continue;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const bool f = live_q && acc[j][i] < thr[j];
+                    const bool f = acc[j][i] < thr[j];
                     const uint64_t bm = __ballot(f);
                     if (bm) {
                         const uint32_t idx = nstage + __builtin_amdgcn_mbcnt_hi(
@@ -401,6 +409,15 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
                 // One query column block stages at most 4 * 64 survivors.
                 if (nstage > kStage - 4 * kWave) flush_stage();
             }
+        };
+        floatx4 acc0[NJ], acc1[NJ];
+        mma_tile(0, acc0);
+#pragma unroll
+        for (int rt = 0; rt < RT / 16; ++rt) {
+            floatx4* cur = (rt & 1) ? acc1 : acc0;
+            floatx4* nxt = (rt & 1) ? acc0 : acc1;
+            if (rt + 1 < RT / 16) mma_tile(rt + 1, nxt);
+            select_tile(rt, cur);
         }
         if (!BOUND) flush_stage();
         if (t + 1 < ntiles) store_tile(buf ^ 1);
