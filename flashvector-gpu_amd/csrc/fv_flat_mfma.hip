@@ -105,14 +105,35 @@ __global__ __launch_bounds__(256) void flat_select_kernel(
         s_g = gthr[q];
     }
     __syncthreads();
-    for (uint32_t c = 0; c < n_chunks; ++c) {
-        const uint32_t cnt = counts[(size_t)c * nq + q];
-        const uint64_t* bq = cand + ((size_t)c * nq + q) * cap;
-        for (uint32_t e0 = 0; e0 < cnt; e0 += kSelectBatch) {
+    // Candidate counts of every chunk, prefix-summed in LDS (chunks processed
+    // 256 at a time), then the concatenated candidates streamed in rounds of
+    // 256: one barrier per round instead of one per chunk.
+    __shared__ uint32_t s_pre[kSelectBatch + 1];
+    for (uint32_t c0 = 0; c0 < n_chunks; c0 += kSelectBatch) {
+        const uint32_t nc = min(kSelectBatch, n_chunks - c0);
+        const uint32_t my = threadIdx.x < nc ? counts[(size_t)(c0 + threadIdx.x) * nq + q] : 0u;
+        s_pre[threadIdx.x + 1] = my;
+        if (threadIdx.x == 0) s_pre[0] = 0;
+        __syncthreads();
+        for (uint32_t off = 1; off < kSelectBatch; off <<= 1) {
+            const uint32_t v = threadIdx.x + 1 > off ? s_pre[threadIdx.x + 1 - off] : 0u;
+            __syncthreads();
+            s_pre[threadIdx.x + 1] += v;
+            __syncthreads();
+        }
+        const uint32_t total = s_pre[nc];
+        for (uint32_t e0 = 0; e0 < total; e0 += kSelectBatch) {
             const uint32_t e = e0 + threadIdx.x;
             const uint32_t g = s_g;
-            if (e < cnt) {
-                const uint64_t key = bq[e];
+            if (e < total) {
+                // Chunk of entry e: the last c with s_pre[c] <= e.
+                uint32_t lo = 0, hi = nc - 1;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (s_pre[mid] <= e) lo = mid;
+                    else hi = mid - 1;
+                }
+                const uint64_t key = cand[((size_t)(c0 + lo) * nq + q) * cap + (e - s_pre[lo])];
                 if ((uint32_t)(key >> 32) <= g) s_k[atomicAdd(&s_n, 1u)] = key;
             }
             __syncthreads();
@@ -129,6 +150,7 @@ __global__ __launch_bounds__(256) void flat_select_kernel(
                 __syncthreads();
             }
         }
+        __syncthreads();
     }
     const uint32_t n = s_n;
     if (n > S) {
@@ -583,7 +605,8 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
     p.n_chunks = c;
     // First pass: 16 subset minima per query per chunk over the first
     // bound_rows positions (a quarter of the rows, a sixteenth for rows wider
-    // than 256 bytes, at least 262144). With m >= 4*S minima the S-th smallest
+    // than 256 bytes, at least 262144 or all). With m >= 4*S minima (at least
+    // 256, a power of two, so at least 16 chunks) the S-th smallest
     // sits near rank S of those rows, so the main pass admits about
     // S * rows / bound_rows rows per query from the start.
     uint32_t brows = p.qrow_bytes <= 256 ? rows_plan / 4 : rows_plan / 16;
@@ -591,7 +614,7 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
     if (brows < bmin) brows = bmin;
     if (a.bound_div && a.bound_div != 0xFFFFFFFFu) brows = rows_plan / a.bound_div;
     p.bound_rows = brows;
-    uint32_t m = 1024u;
+    uint32_t m = 256u;
     while (m < 4u * smax && m < 16384u) m *= 2u;
     uint32_t bc = m / 16u;
     while (bc > 8u && (uint64_t)bc * 64u > brows) bc /= 2u;  // at least one tile per chunk
