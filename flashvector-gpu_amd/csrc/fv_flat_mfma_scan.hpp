@@ -252,6 +252,10 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     }
 
     const uint32_t ntiles = (r_end - r_begin + RT - 1) / RT;
+    // Tiles between buffer checks: after a prune a buffer holds at most
+    // S + 64 keys (bisection slack), so pk more tiles fit in cap.
+    const uint32_t smax_c = a.s_max < cap ? a.s_max : cap;
+    const uint32_t pk = cap > smax_c + 64u + RT ? (cap - smax_c - 64u) / RT : 1u;
     const int st_row = tid / kThrPerRow;
     const int st_seg = tid % kThrPerRow;
     // Staging registers of the next tile. Nothing loaded here is used until
@@ -424,10 +428,17 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
         __syncthreads();
         if (BOUND) continue;
 
+        // A buffer gains at most one key per row, so buffers are checked
+        // every pk tiles and pruned when pk more tiles could overflow them.
+        if ((t + 1) % pk != 0) {
+            if (my_live) s_thr[my_ql] = fminf(s_thr[my_ql], bound_from_ord(gpre));
+            wave_lds_sync();
+            continue;
+        }
         if (!kBlockPrune) {
             // Wave-private prunes of buffers that could overflow on the next tile.
             const uint32_t c = my_live ? s_cnt[my_ql] : 0u;
-            uint64_t need = __ballot(my_live && c + RT > cap);
+            uint64_t need = __ballot(my_live && c + pk * RT > cap);
             while (need) {
                 const int bl = __builtin_ctzll(need);
                 need &= need - 1;
@@ -437,7 +448,7 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
                 uint64_t* bq = cand + ((size_t)chunk * nq + qgb) * cap;
                 uint32_t th;
                 uint32_t kept = wave_bisect_prune<(R > 0 ? R : 1)>(bq, s_cnt[qlb], keep, lane, &th);
-                if (kept + RT > cap) {
+                if (kept + pk * RT > cap) {
                     // Heavy ties at the bound: exact (distance, row) selection.
                     kept = wave_prune(bq, kept, keep, my_scratch, lane);
                     th = kept == keep ? (uint32_t)(my_scratch[keep - 1] >> 32) : 0xFFFFFFFFu;
@@ -455,7 +466,7 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
             // Workgroup prunes, one query at a time, through shared scratch.
             if (tid < QB / 32) s_need[tid] = 0;
             __syncthreads();
-            if (my_live && s_cnt[my_ql] + RT > cap) atomicOr(&s_need[my_ql >> 5], 1u << (my_ql & 31));
+            if (my_live && s_cnt[my_ql] + pk * RT > cap) atomicOr(&s_need[my_ql >> 5], 1u << (my_ql & 31));
             __syncthreads();
             for (int w = 0; w < QB / 32; ++w) {
                 uint32_t bits = s_need[w];
