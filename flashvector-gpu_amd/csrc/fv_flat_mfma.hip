@@ -193,6 +193,7 @@ __global__ __launch_bounds__(256) void flat_rerank_kernel(
     const size_t first = (size_t)blockIdx.x * kRerankRows;
     const uint32_t q = (uint32_t)(item / smax), j = (uint32_t)(item % smax);
     const bool act = q < nq && j < sel_n[q];
+    if (!__syncthreads_or(act)) return;  // no selected row in this workgroup
     const uint64_t my_key = act ? sel[item] : kEmptyKey;
     s_row[tid] = act ? key_slot(my_key) : 0u;
     s_act[tid] = act ? 1 : 0;
@@ -355,25 +356,67 @@ __global__ __launch_bounds__(256) void flat_finalize_kernel(
         const float lim = key_dist(s_k[kk - 1]) + band;
         if (threadIdx.x == 0) s_any = 0u;
         __syncthreads();
+        // Pass 1: which candidates are re-scored (s_r bit 31 marks them; rows
+        // are < 2^31 here, checked by the caller).
         for (uint32_t i = threadIdx.x; i < hp; i += blockDim.x) {
             const uint64_t key = s_k[i];
-            uint64_t v = ~0ull;
             uint32_t r = 0xFFFFFFFFu;
+            uint64_t v = ~0ull;
             if (i < have && key != kEmptyKey) {
                 const float d = key_dist(key);
                 r = key_slot(key);
-                double dd = (double)d;
                 const bool near_prev = i > 0 && s_k[i - 1] != kEmptyKey && d - key_dist(s_k[i - 1]) <= band;
                 const bool near_nxt = i + 1 < have && s_k[i + 1] != kEmptyKey && key_dist(s_k[i + 1]) - d <= band;
+                const uint64_t bits = __double_as_longlong((double)d);
+                v = (bits & 0x8000000000000000ull) ? ~bits : (bits | 0x8000000000000000ull);
                 if ((near_prev || near_nxt) && d <= lim) {
-                    dd = exact_distance_f64<METRIC>(qv, X32 + (size_t)r * a.dim, a.dim);
+                    r |= 0x80000000u;
                     s_any = 1u;
                 }
-                const uint64_t bits = __double_as_longlong(dd);
-                v = (bits & 0x8000000000000000ull) ? ~bits : (bits | 0x8000000000000000ull);
             }
             s_v[i] = v;
             s_r[i] = r;
+        }
+        __syncthreads();
+        // Pass 2: FP64 distances of the marked candidates, one wavefront per
+        // candidate, lanes splitting the dims.
+        if (s_any) {
+            const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+            for (uint32_t i = wave; i < have; i += blockDim.x / kWave) {
+                const uint32_t r = s_r[i];
+                if (r == 0xFFFFFFFFu || !(r & 0x80000000u)) continue;
+                const float* x = X32 + (size_t)(r & 0x7FFFFFFFu) * a.dim;
+                double acc = 0.0, nqd = 0.0, nxd = 0.0;
+                for (uint32_t d = lane; d < a.dim; d += kWave) {
+                    const double qd = qv[d], xd = x[d];
+                    if (METRIC == METRIC_L2) {
+                        const double t = qd - xd;
+                        acc = __builtin_fma(t, t, acc);
+                    } else {
+                        acc = __builtin_fma(qd, xd, acc);
+                        if (METRIC == METRIC_COSINE) {
+                            nqd = __builtin_fma(qd, qd, nqd);
+                            nxd = __builtin_fma(xd, xd, nxd);
+                        }
+                    }
+                }
+                for (int off = 32; off > 0; off >>= 1) {
+                    acc += __shfl_xor(acc, off);
+                    if (METRIC == METRIC_COSINE) {
+                        nqd += __shfl_xor(nqd, off);
+                        nxd += __shfl_xor(nxd, off);
+                    }
+                }
+                if (lane == 0) {
+                    double dd;
+                    if (METRIC == METRIC_L2) dd = acc;
+                    else if (METRIC == METRIC_INNER_PRODUCT) dd = -acc;
+                    else dd = 1.0 - acc / (sqrt(nqd) * sqrt(nxd) + 1e-8);
+                    const uint64_t bits = __double_as_longlong(dd);
+                    s_v[i] = (bits & 0x8000000000000000ull) ? ~bits : (bits | 0x8000000000000000ull);
+                    s_r[i] = r & 0x7FFFFFFFu;
+                }
+            }
         }
         __syncthreads();
         if (s_any) {
@@ -802,6 +845,8 @@ int fv_flat_search(const FvFlatTables* tables, const float* d_queries, uint32_t 
                                             (int)tables->dtype, num_queries, a,
                                             params->d_valid_mask != nullptr, params->valid_count);
     if (p.lcap > 8192u || p.ks == 0) return (int)hipErrorInvalidValue;  // shortlist deeper than 7936
+    if ((a.flags & FV_SEARCH_F64_BOUNDARY) && tables->num_vectors > 0x7FFFFFFFu)
+        return (int)hipErrorInvalidValue;  // the boundary pass marks rows with bit 31
     if (!d_workspace || workspace_bytes < p.workspace_bytes) return (int)hipErrorInvalidValue;
     hipError_t e = fv::launch_flat_mfma(tables, d_queries, num_queries, a, d_out_results,
                                         d_workspace, p, s);
