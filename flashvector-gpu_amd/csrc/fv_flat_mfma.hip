@@ -182,24 +182,43 @@ constexpr uint32_t kRerankQFloats = 4096;
 template <int METRIC>
 __global__ __launch_bounds__(256) void flat_rerank_kernel(
     const float* __restrict__ X32, const float* __restrict__ Qm, uint32_t dim, uint32_t nq,
-    uint32_t smax, const uint32_t* __restrict__ sel_n, uint64_t* __restrict__ sel,
-    float* __restrict__ err) {
+    uint32_t smax, const uint32_t* __restrict__ sel_n, const uint32_t* __restrict__ sel_off,
+    const uint32_t* __restrict__ sel_total, uint64_t* __restrict__ sel, float* __restrict__ err) {
     __shared__ float s_x[kRerankRows * (kRerankSlab + 1)];
     __shared__ float s_qv[kRerankQFloats];
     __shared__ uint32_t s_row[kRerankRows];
     __shared__ uint8_t s_act[kRerankRows];
+    // Items are the selected rows of all queries, concatenated (sel_off is
+    // the exclusive prefix of the per-query counts): every workgroup but the
+    // last gets 256 rows.
     const uint32_t tid = threadIdx.x;
-    const size_t item = (size_t)blockIdx.x * kRerankRows + tid;
-    const size_t first = (size_t)blockIdx.x * kRerankRows;
-    const uint32_t q = (uint32_t)(item / smax), j = (uint32_t)(item % smax);
-    const bool act = q < nq && j < sel_n[q];
-    if (!__syncthreads_or(act)) return;  // no selected row in this workgroup
+    const uint32_t total = *sel_total;
+    const uint32_t first = blockIdx.x * kRerankRows;
+    if (first >= total) return;
+    const uint32_t it = first + tid;
+    const bool act = it < total;
+    // Query of item `it`: the last q with sel_off[q] <= it.
+    uint32_t q = 0;
+    {
+        uint32_t lo = 0, hi = nq - 1;
+        const uint32_t key = act ? it : total - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (sel_off[mid] <= key) lo = mid;
+            else hi = mid - 1;
+        }
+        q = lo;
+    }
+    const size_t item = (size_t)q * smax + (act ? it - sel_off[q] : 0u);
     const uint64_t my_key = act ? sel[item] : kEmptyKey;
     s_row[tid] = act ? key_slot(my_key) : 0u;
     s_act[tid] = act ? 1 : 0;
-    // Queries of this workgroup: q0 .. q1.
-    const uint32_t q0 = (uint32_t)(first / smax);
-    const uint32_t q1 = (uint32_t)min((size_t)nq - 1, (first + kRerankRows - 1) / smax);
+    // Queries of this workgroup: q0 .. q1 (items are in query order).
+    __shared__ uint32_t s_q01[2];
+    if (tid == 0) s_q01[0] = q;
+    if (it == min(first + kRerankRows, total) - 1) s_q01[1] = q;
+    __syncthreads();
+    const uint32_t q0 = s_q01[0], q1 = s_q01[1];
     const bool staged = (size_t)(q1 - q0 + 1) * dim <= kRerankQFloats;
     if (staged)
         for (uint32_t i = tid; i < (q1 - q0 + 1) * dim; i += blockDim.x) s_qv[i] = Qm[(size_t)q0 * dim + i];
@@ -501,6 +520,87 @@ void launch_encode_fp8_rows(const float* x, uint32_t n, uint32_t dim, uint32_t o
                                                                     total_rows, sign, out, scales);
 }
 
+// Same selection for large m (m <= 256 * 64): one workgroup per query, 64
+// minima per thread in registers, counts reduced across the workgroup.
+__global__ __launch_bounds__(256) void bound_select_block_kernel(const float* __restrict__ minima,
+                                                                 uint32_t m, uint32_t nq,
+                                                                 FlatSearchArgs a,
+                                                                 uint32_t* __restrict__ gthr) {
+    constexpr int NR = 64;
+    __shared__ uint32_t s_part[2][256 / kWave];
+    const uint32_t q = blockIdx.x;
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    uint32_t k, S;
+    query_k_s(a, q, k, S);
+    uint32_t o[NR];
+    uint32_t valid = 0;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const uint32_t i = (uint32_t)r * 256u + threadIdx.x;
+        uint32_t v = 0xFFFFFFFFu;
+        if (i < m) {
+            const float f = minima[((size_t)(i >> 4) * nq + q) * 16 + (i & 15u)];
+            if (f < kAdmit) v = ord_f32(f);
+        }
+        o[r] = v;
+        valid += v != 0xFFFFFFFFu;
+    }
+    int par = 0;
+    auto block_count = [&](uint32_t c) -> uint32_t {
+        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+        if (lane == 0) s_part[par][wave] = c;
+        __syncthreads();
+        uint32_t t = 0;
+        for (uint32_t w = 0; w < 256 / kWave; ++w) t += s_part[par][w];
+        par ^= 1;  // alternate buffers: the next write cannot race this read
+        return t;
+    };
+    if (block_count(valid) < S) return;
+    uint32_t lo = 0u, hi = 0xFFFFFFFEu;
+    while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        uint32_t c = 0;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) c += o[r] <= mid;
+        c = block_count(c);
+        if (c >= S) {
+            hi = mid;
+            if (c <= S + (S >> 3) + 16u) break;
+        } else {
+            lo = mid + 1u;
+        }
+    }
+    if (threadIdx.x == 0) atomicMin(&gthr[q], hi);
+}
+
+// Exclusive prefix sum of n counts by one workgroup (each thread sums a
+// contiguous run) into out; the total goes to *total.
+__global__ void exclusive_scan_kernel(const uint32_t* __restrict__ v, uint32_t* __restrict__ out,
+                                      uint32_t n, uint32_t* __restrict__ total) {
+    __shared__ uint32_t s_part[256];
+    const uint32_t per = (n + 255) / 256;
+    const uint32_t b0 = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (uint32_t i = 0; i < per && b0 + i < n; ++i) sum += v[b0 + i];
+    s_part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int i = 0; i < 256; ++i) {
+            const uint32_t x = s_part[i];
+            s_part[i] = run;
+            run += x;
+        }
+        *total = run;
+    }
+    __syncthreads();
+    uint32_t run = s_part[threadIdx.x];
+    for (uint32_t i = 0; i < per && b0 + i < n; ++i) {
+        out[b0 + i] = run;
+        run += v[b0 + i];
+    }
+}
+
 // ---- mask compaction: ascending list of the rows whose bit is set ---------
 
 __global__ void mask_block_counts_kernel(const uint32_t* __restrict__ mask, uint32_t nwords,
@@ -675,6 +775,7 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
     p.off_qh = take((size_t)p.qblocks * p.qb * p.qrow_bytes);
     p.off_sel = take((size_t)nq * smax * 8);
     p.off_seln = take((size_t)nq * 4);
+    p.off_seloff = take((size_t)(nq + 1) * 4);
     if (a.flags & FV_SEARCH_F64_BOUNDARY) p.off_err = take((size_t)nq * smax * 4);
     if (dtype == FV_DTYPE_FP8_E4M3) p.off_qscale = take((size_t)p.qblocks * p.qb * 4);
     if (p.bound_chunks) p.off_minima = take((size_t)p.bound_chunks * nq * 16 * 4);
@@ -699,6 +800,8 @@ void launch_bound_select(const ScanLaunch& L) {
     const dim3 g((L.nq + 3) / 4);
     if (m <= 1024u) bound_select_kernel<16><<<g, 256, 0, L.stream>>>(L.minima, m, L.nq, L.a, L.gthr);
     else if (m <= 4096u) bound_select_kernel<64><<<g, 256, 0, L.stream>>>(L.minima, m, L.nq, L.a, L.gthr);
+    else if (m <= 16384u)
+        bound_select_block_kernel<<<L.nq, 256, 0, L.stream>>>(L.minima, m, L.nq, L.a, L.gthr);
     else  // one wavefront per workgroup, minima in LDS (m * 4 bytes, m padded to whole wavefronts)
         bound_select_kernel<0><<<L.nq, kWave, (size_t)((m + kWave - 1) / kWave) * kWave * 4, L.stream>>>(
             L.minima, m, L.nq, L.a, L.gthr);
@@ -754,19 +857,23 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
     if (e != hipSuccess) return e;
     uint64_t* sel = reinterpret_cast<uint64_t*>(ws + p.off_sel);
     uint32_t* sel_n = reinterpret_cast<uint32_t*>(ws + p.off_seln);
+    uint32_t* sel_off = reinterpret_cast<uint32_t*>(ws + p.off_seloff);
     float* err = (a.flags & FV_SEARCH_F64_BOUNDARY) ? reinterpret_cast<float*>(ws + p.off_err) : nullptr;
     flat_select_kernel<<<nq, 256, p.lds_select, stream>>>(cand, counts, p.n_chunks, nq, p.cap, gthr,
                                                           a, p.lcap, sel, sel_n);
+    // Compact work list for the rerank: exclusive prefix of the per-query
+    // counts (the last slot holds the total).
+    exclusive_scan_kernel<<<1, 256, 0, stream>>>(sel_n, sel_off, nq, sel_off + nq);
     const dim3 rg((unsigned)(((size_t)nq * p.s_max + kRerankRows - 1) / kRerankRows));
     const float* X32 = t->d_vectors;
     if (a.metric == METRIC_L2) {
-        flat_rerank_kernel<METRIC_L2><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, nq, p.s_max, sel_n, sel, err);
+        flat_rerank_kernel<METRIC_L2><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, nq, p.s_max, sel_n, sel_off, sel_off + nq, sel, err);
         flat_finalize_kernel<METRIC_L2><<<nq, 256, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
     } else if (a.metric == METRIC_INNER_PRODUCT) {
-        flat_rerank_kernel<METRIC_INNER_PRODUCT><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, nq, p.s_max, sel_n, sel, err);
+        flat_rerank_kernel<METRIC_INNER_PRODUCT><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, nq, p.s_max, sel_n, sel_off, sel_off + nq, sel, err);
         flat_finalize_kernel<METRIC_INNER_PRODUCT><<<nq, 256, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
     } else {
-        flat_rerank_kernel<METRIC_COSINE><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, nq, p.s_max, sel_n, sel, err);
+        flat_rerank_kernel<METRIC_COSINE><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, nq, p.s_max, sel_n, sel_off, sel_off + nq, sel, err);
         flat_finalize_kernel<METRIC_COSINE><<<nq, 256, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
     }
     return hipGetLastError();

@@ -41,7 +41,7 @@ struct FlatMfmaPlan {
     uint32_t bound_rows;      // rows covered by the first pass
     uint32_t mask_blocks;
     size_t off_gthr, off_counts, off_cand, off_qh, off_qscale, off_rows, off_nrows, off_blocks,
-        off_minima, off_sel, off_seln, off_err;
+        off_minima, off_sel, off_seln, off_seloff, off_err;
     size_t workspace_bytes;
 };
 
