@@ -406,6 +406,10 @@ def bench_fp8(args):
     qs = S.vmf_points(mu, nq, 50.0, sd[2])
     ks = S.log_uniform_k(nq, kmax, sd[3])
     d_k = torch.from_numpy(ks).cuda()
+    if args.calibrate_copy:
+        src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        torch.empty_like(src).copy_(src)
+        del src
     idx = F.FlatIndex(db, "fp8")
     torch.cuda.synchronize()
     log(f"generated and encoded in {time.perf_counter() - t0:.1f} s")
