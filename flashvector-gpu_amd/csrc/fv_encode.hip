@@ -34,11 +34,23 @@ __global__ void norms_kernel(const float* __restrict__ x, uint32_t n, uint32_t d
     const float* row = x + (size_t)r * dim;
     float a = 0.0f;
     double b = 0.0;
-    for (uint32_t d = 0; d < dim; ++d) {
-        float v = row[d];
+    auto add = [&](float v) {
         float sq = v * v;
         a = a + sq;
         b = __builtin_fma((double)v, (double)v, b);
+    };
+    if ((dim & 3u) == 0u && (reinterpret_cast<uintptr_t>(row) & 15u) == 0u) {
+        // Same sequential order, 128-bit loads.
+        const float4* r4 = reinterpret_cast<const float4*>(row);
+        for (uint32_t i = 0; i < dim / 4; ++i) {
+            const float4 v = r4[i];
+            add(v.x);
+            add(v.y);
+            add(v.z);
+            add(v.w);
+        }
+    } else {
+        for (uint32_t d = 0; d < dim; ++d) add(row[d]);
     }
     if (out32) out32[r] = a;
     if (out64) out64[r] = b;

@@ -748,12 +748,14 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
     p.n_chunks = c;
     // First pass: 16 subset minima per query per chunk over the first
     // bound_rows positions (a quarter of the rows, a sixteenth for rows wider
-    // than 256 bytes, at least 262144 or all). With m >= 4*S minima (at least
+    // than 256 bytes, at least 64 MB of rows or all). With m >= 4*S minima (at least
     // 256, a power of two, so at least 16 chunks) the S-th smallest
     // sits near rank S of those rows, so the main pass admits about
     // S * rows / bound_rows rows per query from the start.
     uint32_t brows = p.qrow_bytes <= 256 ? rows_plan / 4 : rows_plan / 16;
-    const uint32_t bmin = rows_plan < 262144u ? rows_plan : 262144u;
+    // Floor of 64 MB of 256-byte rows, scaled to the row width.
+    const uint32_t floor_rows = (uint32_t)((262144ull * 256u) / p.qrow_bytes);
+    const uint32_t bmin = rows_plan < floor_rows ? rows_plan : floor_rows;
     if (brows < bmin) brows = bmin;
     if (a.bound_div && a.bound_div != 0xFFFFFFFFu) brows = rows_plan / a.bound_div;
     p.bound_rows = brows;

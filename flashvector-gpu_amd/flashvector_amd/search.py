@@ -146,7 +146,7 @@ class FlatIndex:
                                     n_chunks, bound_div)
 
     def autotune(self, queries, top_k, metric="l2", valid_mask=None, valid_count=0,
-                 target_recall=0.999, trials=3, f64_boundary=False, shortlist_factor=0):
+                 target_recall=0.999, trials=5, f64_boundary=False, shortlist_factor=0):
         """fv_flat_autotune on a query sample; returns an FvFlatTuning. A
         non-zero shortlist_factor is kept and only the plan is tuned."""
         lib = L.load()
