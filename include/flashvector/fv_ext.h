@@ -107,8 +107,9 @@ int fv_flat_search(const FvFlatTables* tables, const float* d_queries, uint32_t 
  * against it is at least target_recall (16 if none) — or keeps
  * base->shortlist_factor when that is non-zero — (3) times every
  * combination of n_chunks in {auto/2, auto, 2*auto, 4*auto} and bound_div in
- * {auto, 2*auto, 4*auto, none} with that factor (best of `trials` runs after a
- * warm-up) and returns the fastest. `base` supplies k, metric, mask and
+ * {auto, 2*auto, 4*auto, none} with that factor (median of `trials` runs
+ * after a warm-up) and returns the fastest, keeping the automatic plan
+ * (n_chunks = bound_div = 0) unless a candidate is at least 3% faster. `base` supplies k, metric, mask and
  * flags. Allocates and frees its own device memory and synchronizes the
  * stream: a set-up call, not for the search path. */
 typedef struct {

@@ -257,7 +257,7 @@ def test_autotune_meets_recall_and_keeps_results(torch, tmp_path, monkeypatch):
     d_q = dev(torch, qs)
     best = T.tuned(idx, d_q, 10, "l2")
     assert best.recall >= 0.999 and best.shortlist_factor in (1, 2, 4, 8, 16)
-    assert best.n_chunks >= 1 and best.ms > 0
+    assert best.ms > 0   # n_chunks / bound_div 0 = the automatic plan was kept
     out = idx.search(d_q, 10, "l2", tuning=best)
     torch.cuda.synchronize()
     assert_same(L.split_results(out), O.flat_topk(db, qs, 10, "l2"))
