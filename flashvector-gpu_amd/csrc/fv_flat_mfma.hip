@@ -734,18 +734,6 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
     p.qblocks = (nq + p.qb - 1) / p.qb;
     p.compact = has_mask && valid_count > 0 && (uint64_t)valid_count * 2 <= n;
     const uint32_t rows_plan = p.compact ? valid_count : n;
-    // Chunks: enough workgroups to cover 256 CUs several times, but >= 8192
-    // rows each (every chunk pays a cold start of about S*ln(rows/S) inserts
-    // before its threshold settles), a multiple of 8 (one slice per XCD) when
-    // there are at least 8.
-    const uint32_t want = (2560u + p.qblocks - 1) / p.qblocks;
-    const uint32_t max_by_rows = (rows_plan + 8191u) / 8192u;
-    uint32_t c = want < max_by_rows ? want : max_by_rows;
-    if (c > 256u) c = 256u;
-    if (c >= 8u) c &= ~7u;
-    if (c < 1u) c = 1u;
-    if (a.n_chunks) c = a.n_chunks < 4096u ? a.n_chunks : 4096u;
-    p.n_chunks = c;
     // First pass: 16 subset minima per query per chunk over the first
     // bound_rows positions (a quarter of the rows, a sixteenth for rows wider
     // than 256 bytes, at least 64 MB of rows or all). With m >= 4*S minima (at least
@@ -765,6 +753,20 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
     while (bc > 8u && (uint64_t)bc * 64u > brows) bc /= 2u;  // at least one tile per chunk
     p.bound_chunks = (16u * bc >= smax && brows >= bc * 64u) ? bc : 0u;
     if (a.bound_div == 0xFFFFFFFFu) p.bound_chunks = 0u;
+    // Chunks: enough workgroups to cover 256 CUs several times, but at least
+    // 8192 rows each without a first pass (every chunk pays a cold start of
+    // about S*ln(rows/S) inserts before its threshold settles) or 2048 with
+    // one (the seeded bound removes most of it); a multiple of 8 (one slice
+    // per XCD) when there are at least 8.
+    const uint32_t min_rows = p.bound_chunks ? 2048u : 8192u;
+    const uint32_t want = (2560u + p.qblocks - 1) / p.qblocks;
+    const uint32_t max_by_rows = (rows_plan + min_rows - 1) / min_rows;
+    uint32_t c = want < max_by_rows ? want : max_by_rows;
+    if (c > 256u) c = 256u;
+    if (c >= 8u) c &= ~7u;
+    if (c < 1u) c = 1u;
+    if (a.n_chunks) c = a.n_chunks < 4096u ? a.n_chunks : 4096u;
+    p.n_chunks = c;
     size_t off = 0;
     auto take = [&](size_t bytes) {
         size_t o = off;

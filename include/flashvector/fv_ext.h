@@ -106,7 +106,7 @@ int fv_flat_search(const FvFlatTables* tables, const float* d_queries, uint32_t 
  * picks the smallest shortlist factor in {1, 2, 4, 8, 16} whose recall@k
  * against it is at least target_recall (16 if none) — or keeps
  * base->shortlist_factor when that is non-zero — (3) times every
- * combination of n_chunks in {auto/2, auto, 2*auto, 4*auto} and bound_div in
+ * combination of n_chunks in {auto/2, auto, 2, 4, 8 x auto} and bound_div in
  * {auto, 2*auto, 4*auto, none} with that factor (median of `trials` runs
  * after a warm-up) and returns the fastest, keeping the automatic plan
  * (n_chunks = bound_div = 0) unless a candidate is at least 3% faster. `base` supplies k, metric, mask and
