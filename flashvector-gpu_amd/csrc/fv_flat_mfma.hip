@@ -718,8 +718,9 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
     p.block_prune = p.prune_regs == 0;
     // Per wave: prune scratch and staging list share one region; workgroup
     // prunes use one shared scratch followed by four staging lists.
-    const size_t stage_bytes = (size_t)kStage * 9;
-    p.lds_scan = p.block_prune ? (size_t)p.cap * 8 + 4 * (kStage + kStage / 8) * 8
+    static_assert(kRecBytes <= 1024u * 8u, "records must fit a wave's 1024-key prune scratch");
+    const size_t stage_bytes = (size_t)kRecBytes;
+    p.lds_scan = p.block_prune ? (size_t)p.cap * 8 + 4 * (size_t)kRecBytes
                                : 4 * ((size_t)p.cap * 8 > stage_bytes ? (size_t)p.cap * 8 : stage_bytes);
     p.lcap = next_pow2(smax + kSelectBatch);
     p.lds_select = (size_t)p.lcap * 8;

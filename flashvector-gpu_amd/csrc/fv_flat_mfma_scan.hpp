@@ -44,7 +44,11 @@ typedef int int32x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMfmaThreads = 256;
-constexpr uint32_t kStage = 512;  // staged survivors per wave (two query tiles' worth)
+// Staged candidate records per wave: a lane whose 4 scores for one query
+// column have a minimum under the threshold writes them (16 B) and their
+// position (4 B); keys are built when the records are flushed.
+constexpr uint32_t kRec = 256;
+constexpr uint32_t kRecBytes = kRec * 20;
 
 template <int DT>
 struct MfmaOps;
@@ -307,22 +311,37 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
         }
     };
 
-    // Wave-private staging list of survivors, flushed into the per-query
-    // buffers by all 64 lanes at once.
-    // Wave mode: the staging list aliases the wave's prune scratch (never live
-    // at the same time). Workgroup mode: it follows the shared scratch.
-    uint64_t* st_key = kBlockPrune ? s_scratch + cap + (size_t)wave * (kStage + kStage / 8)
-                                   : my_scratch;
-    uint8_t* st_q = reinterpret_cast<uint8_t*>(st_key + kStage);
-    uint32_t nstage = 0;
-    auto flush_stage = [&]() {
+    // Wave-private staging of candidate records, flushed into the per-query
+    // buffers by all 64 lanes at once: the scan loop only ballots the
+    // per-lane minimum of 4 scores and writes one record per passing lane;
+    // the per-score threshold tests and key building happen at the flush,
+    // only for recorded lanes.
+    // Wave mode: the records alias the wave's prune scratch (never live at the
+    // same time). Workgroup mode: they follow the shared scratch.
+    uint8_t* rec_base = kBlockPrune
+        ? reinterpret_cast<uint8_t*>(s_scratch + cap) + (size_t)wave * kRecBytes
+        : reinterpret_cast<uint8_t*>(my_scratch);
+    floatx4* rec_score = reinterpret_cast<floatx4*>(rec_base);
+    uint32_t* rec_meta = reinterpret_cast<uint32_t*>(rec_base + (size_t)kRec * 16);
+    uint32_t nrec = 0;
+    auto flush_records = [&](int buf) {
         wave_lds_sync();
-        for (uint32_t e = lane; e < nstage; e += kWave) {
-            const uint32_t ql = wave * QW + st_q[e];
-            const uint32_t pos = atomicAdd(&s_cnt[ql], 1u);
-            cand[((size_t)chunk * nq + qb0 + ql) * cap + pos] = st_key[e];
+        for (uint32_t e = lane; e < nrec; e += kWave) {
+            const uint32_t meta = rec_meta[e];
+            const uint32_t ql = wave * QW + (meta & 0xFFu);
+            const uint32_t rbase = ((meta >> 8) & 0xFFu);  // first of the 4 rows within the tile
+            const float th = s_thr[ql];
+            const floatx4 sc = rec_score[e];
+            uint64_t* bq = cand + ((size_t)chunk * nq + qb0 + ql) * cap;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (sc[i] < th) {
+                    const uint32_t pos = atomicAdd(&s_cnt[ql], 1u);
+                    bq[pos] = make_key(sc[i], s_rowid[buf][rbase + i]);
+                }
+            }
         }
-        nstage = 0;
+        nrec = 0;
         wave_lds_sync();
     };
 
@@ -389,29 +408,22 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
                 return;
             }
             // One compare per lane and query column on the minimum of its 4
-            // scores; only when some lane passes are the 4 scores staged
-            // (compacted wave-wide with ballot + mbcnt). Columns of queries
-            // past nq have threshold -inf.
+            // scores; passing lanes write a record (compacted wave-wide with
+            // ballot + mbcnt). Columns of queries past nq have threshold -inf.
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
                 const float m4 = fminf(fminf(acc[j][0], acc[j][1]), fminf(acc[j][2], acc[j][3]));
-                if (__ballot(m4 < thr[j]) == 0) continue;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const bool f = acc[j][i] < thr[j];
-                    const uint64_t bm = __ballot(f);
-                    if (bm) {
-                        const uint32_t idx = nstage + __builtin_amdgcn_mbcnt_hi(
-                            (uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
-                        if (f) {
-                            st_key[idx] = make_key(acc[j][i], s_rowid[buf][rt * 16 + 4 * grp + i]);
-                            st_q[idx] = (uint8_t)(16 * j + l16);
-                        }
-                        nstage += (uint32_t)__popcll(bm);
-                    }
+                const bool pass = m4 < thr[j];
+                const uint64_t bm = __ballot(pass);
+                if (bm == 0) continue;
+                if (pass) {
+                    const uint32_t idx = nrec + __builtin_amdgcn_mbcnt_hi(
+                        (uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
+                    rec_score[idx] = acc[j];
+                    rec_meta[idx] = (uint32_t)(16 * j + l16) | ((uint32_t)(rt * 16 + 4 * grp) << 8);
                 }
-                // One query column block stages at most 4 * 64 survivors.
-                if (nstage > kStage - 4 * kWave) flush_stage();
+                nrec += (uint32_t)__popcll(bm);
+                if (nrec > kRec - kWave) flush_records(buf);
             }
         };
         floatx4 acc0[NJ], acc1[NJ];
@@ -423,7 +435,7 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
             if (rt + 1 < RT / 16) mma_tile(rt + 1, nxt);
             select_tile(rt, cur);
         }
-        if (!BOUND) flush_stage();
+        if (!BOUND) flush_records(buf);
         if (t + 1 < ntiles) store_tile(buf ^ 1);
         __syncthreads();
         if (BOUND) continue;
