@@ -118,6 +118,30 @@ __device__ __forceinline__ void block_bitonic_sort_pairs(uint64_t* keys, uint32_
     }
 }
 
+// Same network on (key, value) pairs by one wavefront.
+__device__ __forceinline__ void wave_bitonic_sort_pairs(uint64_t* keys, uint32_t* vals, uint32_t n,
+                                                        int lane) {
+    for (uint32_t k = 2; k <= n; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t p = (uint32_t)lane; p < (n >> 1); p += kWave) {
+                uint32_t i = ((p & ~(j - 1u)) << 1) | (p & (j - 1u));
+                uint32_t m = i + j;
+                const uint64_t a = keys[i], b = keys[m];
+                const uint32_t va = vals[i], vb = vals[m];
+                const bool gt = a > b || (a == b && va > vb);
+                bool up = (i & k) == 0;
+                if (gt == up) {
+                    keys[i] = b;
+                    keys[m] = a;
+                    vals[i] = vb;
+                    vals[m] = va;
+                }
+            }
+            wave_lds_sync();
+        }
+    }
+}
+
 // Prune of one query's candidate buffer by one wavefront: sorts the `count`
 // buffered keys through LDS scratch (capacity >= next_pow2(count)), writes the
 // `keep` smallest back in ascending order and returns how many remain.

@@ -15,6 +15,8 @@
 //    order are re-scored in FP64 and reordered by (FP64 distance, row), so the
 //    order of the returned rows, and which rows make the top k, follow the
 //    exact distances.
+//  * flat_tail_kernel: the three steps above fused into one wavefront per
+//    query, used when the shortlist has at most 256 rows.
 #include "fv_flat_mfma_scan.hpp"
 
 namespace fv {
@@ -165,6 +167,71 @@ __global__ __launch_bounds__(256) void flat_select_kernel(
     if (threadIdx.x == 0) sel_n[q] = have;
 }
 
+// Exact FP32 distance of one (query, row) pair, summed in the reference CPU
+// path's order (sequential over d, non-fused: the library is built with
+// -ffp-contract=off, so it equals exact_distance<METRIC> bit for bit), plus
+// the running error bound of each recursive sum (Higham, Accuracy and
+// Stability of Numerical Algorithms, 2002, sec. 4.2): |error| <= u * sum_i
+// |partial sum_i| + c * u * sum_i |term_i| to first order, c = 1 for products
+// and 3 for squared differences.
+template <int METRIC>
+struct RerankSum {
+    float acc = 0.0f, nqa = 0.0f, nxa = 0.0f, sabs = 0.0f;
+    float run = 0.0f, run_q = 0.0f, run_x = 0.0f;
+    __device__ __forceinline__ void add(float a, float b) {
+        if (METRIC == METRIC_L2) {
+            const float diff = a - b;
+            const float sq = diff * diff;
+            acc = acc + sq;
+            run += acc;
+        } else {
+            const float p = a * b;
+            acc = acc + p;
+            run += fabsf(acc);
+            sabs += fabsf(p);
+            if (METRIC == METRIC_COSINE) {
+                const float pa = a * a, pb = b * b;
+                nqa = nqa + pa;
+                nxa = nxa + pb;
+                run_q += nqa;
+                run_x += nxa;
+            }
+        }
+    }
+    // Distance and its error bound.
+    __device__ __forceinline__ void finish(float& d32, float& e) const {
+        constexpr float u = 5.9604645e-8f;  // 2^-24
+        constexpr float margin = 1.01f;     // covers the second-order terms
+        if (METRIC == METRIC_L2) {
+            d32 = acc;
+            e = margin * u * (run + 3.0f * acc);
+        } else if (METRIC == METRIC_INNER_PRODUCT) {
+            d32 = -acc;
+            e = margin * u * (run + sabs);
+        } else {
+            const float denom = sqrtf(nqa) * sqrtf(nxa) + 1e-8f;
+            d32 = 1.0f - acc / denom;
+            const float e_dot = u * (run + sabs);
+            // Relative error of the denominator: half of each norm's relative
+            // error, plus two square roots, a product and an add.
+            const float rel = 0.5f * u * ((run_q + nqa) / fmaxf(nqa, 1e-30f) +
+                                          (run_x + nxa) / fmaxf(nxa, 1e-30f)) + 4.0f * u;
+            e = margin * ((e_dot + fabsf(acc) * rel) / denom + u * fabsf(acc / denom) + u * fabsf(d32));
+        }
+    }
+};
+
+// Distance reported for a selected row when no FP32 rows are stored: the
+// reduced-precision score s (0.5*||x||^2 - q.x for L2, -q.x for IP,
+// -q.x/||x|| for cosine) turned into a distance.
+template <int METRIC>
+__device__ __forceinline__ float score_as_distance(float s, const float* __restrict__ qv, uint32_t dim) {
+    if (METRIC == METRIC_INNER_PRODUCT) return s;
+    float qn = 0.0f;
+    for (uint32_t dd = 0; dd < dim; ++dd) qn = __builtin_fmaf(qv[dd], qv[dd], qn);
+    return METRIC == METRIC_L2 ? fmaxf(0.0f, qn + 2.0f * s) : 1.0f + s / (sqrtf(qn) + 1e-8f);
+}
+
 // Exact FP32 rerank of the selected rows. Workgroup b covers items
 // [256 b, 256 b + 256) of the padded [query][s_max] array of selected keys,
 // one item per thread (a workgroup may span several queries; items past a
@@ -227,13 +294,7 @@ __global__ __launch_bounds__(256) void flat_rerank_kernel(
     float d32 = 0.0f;
     float e = 0.0f;
     if (X32) {
-        // Sums in the reference order plus the running error bound of each
-        // recursive sum (Higham, Accuracy and Stability of Numerical
-        // Algorithms, 2002, sec. 4.2): |error| <= u * sum_i |partial sum_i|
-        // + c * u * sum_i |term_i| to first order, c = 1 for products and 3
-        // for squared differences.
-        float acc = 0.0f, nqa = 0.0f, nxa = 0.0f, sabs = 0.0f;
-        float run = 0.0f, run_q = 0.0f, run_x = 0.0f;
+        RerankSum<METRIC> sum;
         const bool vec = (dim & 3u) == 0u && (reinterpret_cast<uintptr_t>(X32) & 15u) == 0u;
         for (uint32_t d0 = 0; d0 < dim; d0 += kRerankSlab) {
             const uint32_t w = min(kRerankSlab, dim - d0);
@@ -264,58 +325,14 @@ __global__ __launch_bounds__(256) void flat_rerank_kernel(
             __syncthreads();
             if (act) {
                 const float* xr = &s_x[tid * (kRerankSlab + 1)];
-                for (uint32_t dd = 0; dd < w; ++dd) {
-                    const float a = qv[d0 + dd], b = xr[dd];
-                    if (METRIC == METRIC_L2) {
-                        const float diff = a - b;
-                        const float sq = diff * diff;
-                        acc = acc + sq;
-                        run += acc;
-                    } else {
-                        const float p = a * b;
-                        acc = acc + p;
-                        run += fabsf(acc);
-                        sabs += fabsf(p);
-                        if (METRIC == METRIC_COSINE) {
-                            const float pa = a * a, pb = b * b;
-                            nqa = nqa + pa;
-                            nxa = nxa + pb;
-                            run_q += nqa;
-                            run_x += nxa;
-                        }
-                    }
-                }
+                for (uint32_t dd = 0; dd < w; ++dd) sum.add(qv[d0 + dd], xr[dd]);
             }
             __syncthreads();
         }
-        constexpr float u = 5.9604645e-8f;  // 2^-24
-        constexpr float margin = 1.01f;     // covers the second-order terms
-        if (METRIC == METRIC_L2) {
-            d32 = acc;
-            e = margin * u * (run + 3.0f * acc);
-        } else if (METRIC == METRIC_INNER_PRODUCT) {
-            d32 = -acc;
-            e = margin * u * (run + sabs);
-        } else {
-            const float denom = sqrtf(nqa) * sqrtf(nxa) + 1e-8f;
-            d32 = 1.0f - acc / denom;
-            const float e_dot = u * (run + sabs);
-            // Relative error of the denominator: half of each norm's relative
-            // error, plus two square roots, a product and an add.
-            const float rel = 0.5f * u * ((run_q + nqa) / fmaxf(nqa, 1e-30f) +
-                                          (run_x + nxa) / fmaxf(nxa, 1e-30f)) + 4.0f * u;
-            e = margin * ((e_dot + fabsf(acc) * rel) / denom + u * fabsf(acc / denom) + u * fabsf(d32));
-        }
+        sum.finish(d32, e);
     } else if (act) {
         // No FP32 rows: report the reduced-precision score as a distance.
-        const float s = key_dist(my_key);
-        if (METRIC == METRIC_INNER_PRODUCT) {
-            d32 = s;
-        } else {
-            float qn = 0.0f;
-            for (uint32_t dd = 0; dd < dim; ++dd) qn = __builtin_fmaf(qv[dd], qv[dd], qn);
-            d32 = METRIC == METRIC_L2 ? fmaxf(0.0f, qn + 2.0f * s) : 1.0f + s / (sqrtf(qn) + 1e-8f);
-        }
+        d32 = score_as_distance<METRIC>(key_dist(my_key), qv, dim);
     }
     if (act) {
         sel[item] = d32 < kAdmit ? make_key(d32, s_row[tid]) : kEmptyKey;
@@ -450,6 +467,302 @@ __global__ __launch_bounds__(256) void flat_finalize_kernel(
     }
     for (uint32_t i = threadIdx.x; i < a.top_k; i += blockDim.x) {
         const uint64_t key = i < k && i < hp ? s_k[i] : kEmptyKey;
+        GpuSearchResult r;
+        if (key == kEmptyKey) {
+            r.id = FV_EMPTY_ID;
+            r.distance = FV_EMPTY_DIST;
+        } else {
+            r.id = key_slot(key);
+            r.distance = key_dist(key);
+        }
+        out[(size_t)q * a.top_k + i] = r;
+    }
+}
+
+// Fused select + rerank + finalize for shortlists of at most kTailMaxS rows:
+// one wavefront per query, kTailWaves per workgroup, no workgroup barriers.
+// The wave streams the candidates of every chunk that are within the final
+// shared bound into a wave-private LDS list kept to the S smallest keys by
+// bitonic prunes, re-scores the selected rows exactly (lane i takes rows i,
+// i+64, ...; two rows at a time so that their loads overlap), sorts them by
+// (distance, row), applies the FP64 boundary when asked and writes the first
+// k: the same results as flat_select_kernel, flat_rerank_kernel and
+// flat_finalize_kernel in sequence.
+constexpr uint32_t kTailMaxS = 256;
+constexpr uint32_t kTailWaves = 4;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t t = __shfl_up(v, off);
+        if (lane >= (uint32_t)off) v += t;
+    }
+    return v;
+}
+
+// Exact distances and error bounds of NR rows of one query (rows[] valid,
+// possibly repeated), summed as RerankSum does.
+template <int METRIC, int NR>
+__device__ __forceinline__ void rerank_rows(const float* __restrict__ X32, const float* __restrict__ qv,
+                                            uint32_t dim, bool vec, const uint32_t (&rows)[NR],
+                                            float (&d32)[NR], float (&err)[NR]) {
+    RerankSum<METRIC> sum[NR];
+    if (vec) {
+        const float* xr[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) xr[r] = X32 + (size_t)rows[r] * dim;
+        uint32_t d = 0;
+        for (; d + 32u <= dim; d += 32u) {
+            float4 q4[8], x4[NR][8];
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                q4[m] = *reinterpret_cast<const float4*>(qv + d + 4 * m);
+#pragma unroll
+                for (int r = 0; r < NR; ++r) x4[r][m] = *reinterpret_cast<const float4*>(xr[r] + d + 4 * m);
+            }
+#pragma unroll
+            for (int m = 0; m < 8; ++m)
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    sum[r].add(q4[m].x, x4[r][m].x);
+                    sum[r].add(q4[m].y, x4[r][m].y);
+                    sum[r].add(q4[m].z, x4[r][m].z);
+                    sum[r].add(q4[m].w, x4[r][m].w);
+                }
+        }
+        for (; d < dim; d += 4u) {
+            const float4 q4 = *reinterpret_cast<const float4*>(qv + d);
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const float4 x4 = *reinterpret_cast<const float4*>(xr[r] + d);
+                sum[r].add(q4.x, x4.x);
+                sum[r].add(q4.y, x4.y);
+                sum[r].add(q4.z, x4.z);
+                sum[r].add(q4.w, x4.w);
+            }
+        }
+    } else {
+        for (uint32_t d = 0; d < dim; ++d) {
+            const float a = qv[d];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) sum[r].add(a, X32[(size_t)rows[r] * dim + d]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) sum[r].finish(d32[r], err[r]);
+}
+
+template <int METRIC>
+__global__ __launch_bounds__(256) void flat_tail_kernel(
+    const uint64_t* __restrict__ cand, const uint32_t* __restrict__ counts, uint32_t n_chunks,
+    uint32_t nq, uint32_t cap, const uint32_t* __restrict__ gthr, FlatSearchArgs a, uint32_t lw,
+    uint32_t hw, uint32_t wbytes, const float* __restrict__ X32, const float* __restrict__ Qm,
+    GpuSearchResult* __restrict__ out) {
+    extern __shared__ uint64_t s_mem[];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t q = blockIdx.x * kTailWaves + wave;
+    if (q >= nq) return;  // whole wavefront; no workgroup barriers follow
+    // Wave-private LDS: keys[lw] | errs[hw] | FP64 values[hw] | rows[hw].
+    uint64_t* keys = reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(s_mem) + (size_t)wave * wbytes);
+    float* errs = reinterpret_cast<float*>(keys + lw);
+    uint64_t* s_v = reinterpret_cast<uint64_t*>(errs + hw);
+    uint32_t* s_r = reinterpret_cast<uint32_t*>(s_v + hw);
+    uint32_t k, S;
+    query_k_s(a, q, k, S);
+    uint32_t g = gthr[q];
+    uint32_t n = 0;  // keys in the list (wave-uniform)
+
+    // Keeps the S smallest keys of keys[0..n) (sorted) and tightens g to the
+    // S-th. n <= lw, so the padded length never exceeds lw.
+    auto prune = [&]() {
+        const uint32_t np = next_pow2(n < 2u ? 2u : n);
+        for (uint32_t i = n + lane; i < np; i += kWave) keys[i] = kEmptyKey;
+        wave_lds_sync();
+        wave_bitonic_sort(keys, np, lane);
+        if (n >= S) {
+            const uint32_t gs = (uint32_t)(keys[S - 1] >> 32);
+            g = gs < g ? gs : g;
+            n = S;
+        }
+        wave_lds_sync();
+    };
+
+    // Candidates of up to 64 chunks at a time: lane c holds chunk c's count;
+    // entry e belongs to the last chunk whose exclusive prefix is <= e
+    // (bisection over the lanes, 6 steps for 64 lanes).
+    for (uint32_t c0 = 0; c0 < n_chunks; c0 += kWave) {
+        const uint32_t nc = n_chunks - c0 < (uint32_t)kWave ? n_chunks - c0 : (uint32_t)kWave;
+        const uint32_t cnt = lane < nc ? counts[(size_t)(c0 + lane) * nq + q] : 0u;
+        const uint32_t incl = wave_incl_scan(cnt, lane);
+        const uint32_t excl = incl - cnt;
+        const uint32_t total = (uint32_t)__shfl((int)incl, kWave - 1);
+        for (uint32_t e0 = 0; e0 < total; e0 += 4u * kWave) {
+            uint64_t kb[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t e = e0 + (uint32_t)r * kWave + lane;
+                const uint32_t ee = e < total ? e : total - 1u;
+                uint32_t lo = 0, hi = nc - 1u;
+#pragma unroll
+                for (int it = 0; it < 6; ++it) {
+                    const uint32_t mid = (lo + hi + 1u) >> 1;
+                    if ((uint32_t)__shfl((int)excl, (int)mid) <= ee) lo = mid;
+                    else hi = mid - 1u;
+                }
+                const uint32_t base = (uint32_t)__shfl((int)excl, (int)lo);
+                kb[r] = e < total ? cand[((size_t)(c0 + lo) * nq + q) * cap + (ee - base)] : kEmptyKey;
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                if (n + kWave > lw) prune();
+                const bool pass = kb[r] != kEmptyKey && (uint32_t)(kb[r] >> 32) <= g;
+                const uint64_t bm = __ballot(pass);
+                if (pass)
+                    keys[n + __builtin_amdgcn_mbcnt_hi((uint32_t)(bm >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u))] = kb[r];
+                n += (uint32_t)__popcll(bm);
+            }
+        }
+    }
+    wave_lds_sync();
+    if (n > S) prune();
+    const uint32_t have = n;
+    const uint32_t dim = a.dim;
+    const float* qv = Qm + (size_t)q * dim;
+
+    // Exact rerank in place: keys[i] becomes (distance, row), errs[i] its bound.
+    if (X32) {
+        const bool vec = (dim & 3u) == 0u && (reinterpret_cast<uintptr_t>(X32) & 15u) == 0u &&
+                         (reinterpret_cast<uintptr_t>(Qm) & 15u) == 0u;
+        for (uint32_t i0 = 0; i0 < have; i0 += 2u * kWave) {
+            const uint32_t ia = i0 + lane, ib = i0 + kWave + lane;
+            // Lanes past the list take a valid row; their results are dropped.
+            const uint32_t ra = ia < have ? key_slot(keys[ia]) : key_slot(keys[0]);
+            const uint32_t rb = ib < have ? key_slot(keys[ib]) : ra;
+            float d[2], e[2];
+            if (i0 + kWave < have) {
+                const uint32_t rows[2] = {ra, rb};
+                rerank_rows<METRIC, 2>(X32, qv, dim, vec, rows, d, e);
+            } else {
+                const uint32_t rows[1] = {ra};
+                float d1[1], e1[1];
+                rerank_rows<METRIC, 1>(X32, qv, dim, vec, rows, d1, e1);
+                d[0] = d1[0];
+                e[0] = e1[0];
+                d[1] = 0.0f;
+                e[1] = 0.0f;
+            }
+            if (ia < have) {
+                keys[ia] = d[0] < kAdmit ? make_key(d[0], ra) : kEmptyKey;
+                errs[ia] = e[0];
+            }
+            if (ib < have) {
+                keys[ib] = d[1] < kAdmit ? make_key(d[1], rb) : kEmptyKey;
+                errs[ib] = e[1];
+            }
+        }
+    } else {
+        for (uint32_t i = lane; i < have; i += kWave) {
+            const float d = score_as_distance<METRIC>(key_dist(keys[i]), qv, dim);
+            keys[i] = d < kAdmit ? make_key(d, key_slot(keys[i])) : kEmptyKey;
+            errs[i] = 0.0f;
+        }
+    }
+    wave_lds_sync();
+    const uint32_t hp = next_pow2(have < 2u ? 2u : have);
+    for (uint32_t i = have + lane; i < hp; i += kWave) keys[i] = kEmptyKey;
+    wave_lds_sync();
+    wave_bitonic_sort(keys, hp, lane);
+
+    if ((a.flags & FV_SEARCH_F64_BOUNDARY) && X32 != nullptr && have > 1) {
+        // As in flat_finalize_kernel: candidates adjacent in FP32 order within
+        // twice the largest error bound that could reach the top k are
+        // re-scored in FP64 (lanes split the dims), then all candidates are
+        // ordered by (value, row).
+        float em = 0.0f;
+        for (uint32_t i = lane; i < have; i += kWave) em = fmaxf(em, errs[i]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) em = fmaxf(em, __shfl_xor(em, off));
+        const float band = 2.0000002f * em;
+        const uint32_t kk = k < have ? k : have;
+        const float lim = key_dist(keys[kk - 1]) + band;
+        bool mine = false;
+        for (uint32_t i = lane; i < hp; i += kWave) {
+            const uint64_t key = keys[i];
+            uint32_t r = 0xFFFFFFFFu;
+            uint64_t v = ~0ull;
+            if (i < have && key != kEmptyKey) {
+                const float d = key_dist(key);
+                r = key_slot(key);
+                const bool near_prev = i > 0 && keys[i - 1] != kEmptyKey && d - key_dist(keys[i - 1]) <= band;
+                const bool near_nxt = i + 1 < have && keys[i + 1] != kEmptyKey && key_dist(keys[i + 1]) - d <= band;
+                const uint64_t bits = __double_as_longlong((double)d);
+                v = (bits & 0x8000000000000000ull) ? ~bits : (bits | 0x8000000000000000ull);
+                if ((near_prev || near_nxt) && d <= lim) {
+                    r |= 0x80000000u;
+                    mine = true;
+                }
+            }
+            s_v[i] = v;
+            s_r[i] = r;
+        }
+        const bool any = __ballot(mine) != 0;
+        wave_lds_sync();
+        if (any) {
+            for (uint32_t i0 = 0; i0 < have; i0 += kWave) {
+                const uint32_t i = i0 + lane;
+                const uint32_t ri = i < have ? s_r[i] : 0xFFFFFFFFu;
+                uint64_t bm = __ballot(ri != 0xFFFFFFFFu && (ri & 0x80000000u));
+                while (bm) {
+                    const uint32_t j = i0 + (uint32_t)__builtin_ctzll(bm);
+                    bm &= bm - 1;
+                    const uint32_t r = s_r[j] & 0x7FFFFFFFu;
+                    const float* x = X32 + (size_t)r * dim;
+                    double acc = 0.0, nqd = 0.0, nxd = 0.0;
+                    for (uint32_t d = lane; d < dim; d += kWave) {
+                        const double qd = qv[d], xd = x[d];
+                        if (METRIC == METRIC_L2) {
+                            const double t = qd - xd;
+                            acc = __builtin_fma(t, t, acc);
+                        } else {
+                            acc = __builtin_fma(qd, xd, acc);
+                            if (METRIC == METRIC_COSINE) {
+                                nqd = __builtin_fma(qd, qd, nqd);
+                                nxd = __builtin_fma(xd, xd, nxd);
+                            }
+                        }
+                    }
+                    for (int off = 32; off > 0; off >>= 1) {
+                        acc += __shfl_xor(acc, off);
+                        if (METRIC == METRIC_COSINE) {
+                            nqd += __shfl_xor(nqd, off);
+                            nxd += __shfl_xor(nxd, off);
+                        }
+                    }
+                    if (lane == 0) {
+                        double dd;
+                        if (METRIC == METRIC_L2) dd = acc;
+                        else if (METRIC == METRIC_INNER_PRODUCT) dd = -acc;
+                        else dd = 1.0 - acc / (sqrt(nqd) * sqrt(nxd) + 1e-8);
+                        const uint64_t bits = __double_as_longlong(dd);
+                        s_v[j] = (bits & 0x8000000000000000ull) ? ~bits : (bits | 0x8000000000000000ull);
+                        s_r[j] = r;
+                    }
+                }
+            }
+            wave_lds_sync();
+            wave_bitonic_sort_pairs(s_v, s_r, hp, (int)lane);
+            for (uint32_t i = lane; i < kk; i += kWave) {
+                const uint64_t v = s_v[i];
+                const uint64_t bits = (v & 0x8000000000000000ull) ? (v & 0x7fffffffffffffffull) : ~v;
+                keys[i] = make_key((float)__longlong_as_double(bits), s_r[i]);
+            }
+            wave_lds_sync();
+        }
+    }
+    for (uint32_t i = lane; i < a.top_k; i += kWave) {
+        const uint64_t key = i < k && i < hp ? keys[i] : kEmptyKey;
         GpuSearchResult r;
         if (key == kEmptyKey) {
             r.id = FV_EMPTY_ID;
@@ -722,6 +1035,14 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
     const size_t stage_bytes = (size_t)kRecBytes;
     p.lds_scan = p.block_prune ? (size_t)p.cap * 8 + 4 * (size_t)kRecBytes
                                : 4 * ((size_t)p.cap * 8 > stage_bytes ? (size_t)p.cap * 8 : stage_bytes);
+    // Shallow shortlists: one fused wavefront per query after the scan.
+    p.tail = smax <= kTailMaxS;
+    p.tail_lw = next_pow2(smax + kWave);
+    p.tail_hw = next_pow2(smax < 2u ? 2u : smax);
+    p.tail_wbytes = p.tail_lw * 8 + p.tail_hw * 4;
+    if (a.flags & FV_SEARCH_F64_BOUNDARY) p.tail_wbytes += p.tail_hw * (8 + 4);
+    p.tail_wbytes = (p.tail_wbytes + 15u) & ~15u;
+    p.lds_tail = (size_t)kTailWaves * p.tail_wbytes;
     p.lcap = next_pow2(smax + kSelectBatch);
     p.lds_select = (size_t)p.lcap * 8;
     p.hcap = next_pow2(smax < 2u ? 2u : smax);
@@ -860,6 +1181,21 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
     else if (t->dtype == FV_DTYPE_BF16) e = large ? scan_bf16_large(L) : scan_bf16_small(L);
     else if (t->dtype == FV_DTYPE_FP8_E4M3) e = scan_fp8(L);
     if (e != hipSuccess) return e;
+    const float* X32 = t->d_vectors;
+    if (p.tail) {
+        const dim3 tg((nq + kTailWaves - 1) / kTailWaves);
+        const unsigned tb = kTailWaves * kWave;
+        if (a.metric == METRIC_L2)
+            flat_tail_kernel<METRIC_L2><<<tg, tb, p.lds_tail, stream>>>(
+                cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.tail_lw, p.tail_hw, p.tail_wbytes, X32, Qm, out);
+        else if (a.metric == METRIC_INNER_PRODUCT)
+            flat_tail_kernel<METRIC_INNER_PRODUCT><<<tg, tb, p.lds_tail, stream>>>(
+                cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.tail_lw, p.tail_hw, p.tail_wbytes, X32, Qm, out);
+        else
+            flat_tail_kernel<METRIC_COSINE><<<tg, tb, p.lds_tail, stream>>>(
+                cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.tail_lw, p.tail_hw, p.tail_wbytes, X32, Qm, out);
+        return hipGetLastError();
+    }
     uint64_t* sel = reinterpret_cast<uint64_t*>(ws + p.off_sel);
     uint32_t* sel_n = reinterpret_cast<uint32_t*>(ws + p.off_seln);
     uint32_t* sel_off = reinterpret_cast<uint32_t*>(ws + p.off_seloff);
@@ -870,7 +1206,6 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
     // counts (the last slot holds the total).
     exclusive_scan_kernel<<<1, 256, 0, stream>>>(sel_n, sel_off, nq, sel_off + nq);
     const dim3 rg((unsigned)(((size_t)nq * p.s_max + kRerankRows - 1) / kRerankRows));
-    const float* X32 = t->d_vectors;
     if (a.metric == METRIC_L2) {
         flat_rerank_kernel<METRIC_L2><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, nq, p.s_max, sel_n, sel_off, sel_off + nq, sel, err);
         flat_finalize_kernel<METRIC_L2><<<nq, 256, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);

@@ -28,6 +28,11 @@ struct FlatMfmaPlan {
     size_t lds_scan;          // dynamic LDS of the scan kernel
     uint32_t lcap;            // LDS keys of the select/rerank kernel
     size_t lds_select;        // dynamic LDS of the select kernel
+    bool tail;                // fused wave-per-query select/rerank/finalize
+    uint32_t tail_lw;         // its LDS keys per wavefront (next_pow2(s_max + 64))
+    uint32_t tail_hw;         // its rerank slots per wavefront (next_pow2(s_max))
+    uint32_t tail_wbytes;     // its LDS bytes per wavefront
+    size_t lds_tail;          // its dynamic LDS per workgroup
     uint32_t hcap;            // keys of the finalize kernel (next_pow2(s_max))
     size_t lds_final;         // dynamic LDS of the finalize kernel
     uint32_t ks;              // K steps per row (32 elements for 16-bit, 128 for FP8)

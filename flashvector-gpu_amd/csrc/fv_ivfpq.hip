@@ -361,9 +361,10 @@ hipError_t launch_ivfpq(const IvfPqGpuTables* t, const uint8_t* codes_lo, const 
         // Tiled exact scan with fmaf accumulation: same values and (distance,
         // cluster) order as the per-query coarse kernel, at GEMM-like reuse.
         GpuSearchResult* cres = reinterpret_cast<GpuSearchResult*>(ws + p.off_coarse_res);
-        launch_flat_exact(t->d_centroids, nullptr, t->nlist, t->dim, Qm, nq, p.max_nprobe,
-                          METRIC_L2, nullptr, cres, p.max_nprobe, ws + p.off_coarse_ws,
-                          p.coarse_plan, stream, true);
+        const hipError_t ce = launch_flat_exact(t->d_centroids, nullptr, t->nlist, t->dim, Qm, nq,
+                                                p.max_nprobe, METRIC_L2, nullptr, cres, p.max_nprobe,
+                                                ws + p.off_coarse_ws, p.coarse_plan, stream, true);
+        if (ce != hipSuccess) return ce;
         const size_t tot = (size_t)nq * p.max_nprobe;
         probes_from_results_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, stream>>>(
             cres, nq, p.max_nprobe, nprobe_q, nprobe_u, t->nlist, probes);
