@@ -552,6 +552,33 @@ __device__ __forceinline__ void rerank_rows(const float* __restrict__ X32, const
     for (int r = 0; r < NR; ++r) sum[r].finish(d32[r], err[r]);
 }
 
+// Ranks of the keys at positions lane, lane+64, ... (< n) among keys[0..n)
+// of a wave-private LDS list of distinct keys: one broadcast read per key and
+// one 64-bit compare per slot, no sorting network. Slots at or past n get
+// rank n or more.
+constexpr int kRankSlots = 4;
+__device__ __forceinline__ void wave_key_ranks(const uint64_t* keys, uint32_t n, uint32_t lane,
+                                               uint64_t (&mine)[kRankSlots], uint32_t (&rank)[kRankSlots]) {
+    const uint32_t ns = (n + kWave - 1) / kWave;
+#pragma unroll
+    for (int s = 0; s < kRankSlots; ++s) {
+        const uint32_t idx = lane + (uint32_t)s * kWave;
+        mine[s] = idx < n ? keys[idx] : kEmptyKey;
+        rank[s] = idx < n ? 0u : n;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint64_t ki = keys[i];
+#pragma unroll
+        for (int s = 0; s < kRankSlots; ++s)
+            if ((uint32_t)s < ns) rank[s] += ki < mine[s] ? 1u : 0u;
+    }
+}
+
+// Keys of rows whose distance is not admitted (>= 1e30 or NaN): the row in
+// the low word keeps them distinct, and they order after every admitted key.
+__device__ __forceinline__ uint64_t unadmitted_key(uint32_t row) { return 0xFFFFFFFF00000000ull | row; }
+__device__ __forceinline__ bool key_unadmitted(uint64_t k) { return (uint32_t)(k >> 32) == 0xFFFFFFFFu; }
+
 template <int METRIC>
 __global__ __launch_bounds__(256) void flat_tail_kernel(
     const uint64_t* __restrict__ cand, const uint32_t* __restrict__ counts, uint32_t n_chunks,
@@ -560,7 +587,8 @@ __global__ __launch_bounds__(256) void flat_tail_kernel(
     GpuSearchResult* __restrict__ out) {
     extern __shared__ uint64_t s_mem[];
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint32_t wave = threadIdx.x / kWave;
+    // Wave-uniform, so the query row is read with scalar loads.
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
     const uint32_t q = blockIdx.x * kTailWaves + wave;
     if (q >= nq) return;  // whole wavefront; no workgroup barriers follow
     // Wave-private LDS: keys[lw] | errs[hw] | FP64 values[hw] | rows[hw].
@@ -626,7 +654,22 @@ __global__ __launch_bounds__(256) void flat_tail_kernel(
         }
     }
     wave_lds_sync();
-    if (n > S) prune();
+    if (n > S) {
+        if (n <= (uint32_t)(kRankSlots * kWave)) {
+            // The S smallest by rank, written in order to keys[0..S).
+            uint64_t mk[kRankSlots];
+            uint32_t rk[kRankSlots];
+            wave_key_ranks(keys, n, lane, mk, rk);
+            wave_lds_sync();
+#pragma unroll
+            for (int s2 = 0; s2 < kRankSlots; ++s2)
+                if (rk[s2] < S) keys[rk[s2]] = mk[s2];
+            n = S;
+            wave_lds_sync();
+        } else {
+            prune();
+        }
+    }
     const uint32_t have = n;
     const uint32_t dim = a.dim;
     const float* qv = Qm + (size_t)q * dim;
@@ -654,28 +697,58 @@ __global__ __launch_bounds__(256) void flat_tail_kernel(
                 e[1] = 0.0f;
             }
             if (ia < have) {
-                keys[ia] = d[0] < kAdmit ? make_key(d[0], ra) : kEmptyKey;
+                keys[ia] = d[0] < kAdmit ? make_key(d[0], ra) : unadmitted_key(ra);
                 errs[ia] = e[0];
             }
             if (ib < have) {
-                keys[ib] = d[1] < kAdmit ? make_key(d[1], rb) : kEmptyKey;
+                keys[ib] = d[1] < kAdmit ? make_key(d[1], rb) : unadmitted_key(rb);
                 errs[ib] = e[1];
             }
         }
     } else {
         for (uint32_t i = lane; i < have; i += kWave) {
             const float d = score_as_distance<METRIC>(key_dist(keys[i]), qv, dim);
-            keys[i] = d < kAdmit ? make_key(d, key_slot(keys[i])) : kEmptyKey;
+            keys[i] = d < kAdmit ? make_key(d, key_slot(keys[i])) : unadmitted_key(key_slot(keys[i]));
             errs[i] = 0.0f;
         }
     }
     wave_lds_sync();
+    GpuSearchResult* oq = out + (size_t)q * a.top_k;
+    const uint32_t kk_out = k < have ? k : have;
+    if (!((a.flags & FV_SEARCH_F64_BOUNDARY) && X32 != nullptr && have > 1)) {
+        // Without the FP64 boundary only the first k in (distance, row)
+        // order are needed: each lane writes its keys of rank < k.
+        uint64_t mk[kRankSlots];
+        uint32_t rk[kRankSlots];
+        wave_key_ranks(keys, have, lane, mk, rk);
+#pragma unroll
+        for (int s2 = 0; s2 < kRankSlots; ++s2) {
+            if (rk[s2] < kk_out) {
+                GpuSearchResult r;
+                if (key_unadmitted(mk[s2])) {
+                    r.id = FV_EMPTY_ID;
+                    r.distance = FV_EMPTY_DIST;
+                } else {
+                    r.id = key_slot(mk[s2]);
+                    r.distance = key_dist(mk[s2]);
+                }
+                oq[rk[s2]] = r;
+            }
+        }
+        for (uint32_t i = kk_out + lane; i < a.top_k; i += kWave) {
+            GpuSearchResult r;
+            r.id = FV_EMPTY_ID;
+            r.distance = FV_EMPTY_DIST;
+            oq[i] = r;
+        }
+        return;
+    }
     const uint32_t hp = next_pow2(have < 2u ? 2u : have);
     for (uint32_t i = have + lane; i < hp; i += kWave) keys[i] = kEmptyKey;
     wave_lds_sync();
     wave_bitonic_sort(keys, hp, lane);
 
-    if ((a.flags & FV_SEARCH_F64_BOUNDARY) && X32 != nullptr && have > 1) {
+    {
         // As in flat_finalize_kernel: candidates adjacent in FP32 order within
         // twice the largest error bound that could reach the top k are
         // re-scored in FP64 (lanes split the dims), then all candidates are
@@ -692,11 +765,11 @@ __global__ __launch_bounds__(256) void flat_tail_kernel(
             const uint64_t key = keys[i];
             uint32_t r = 0xFFFFFFFFu;
             uint64_t v = ~0ull;
-            if (i < have && key != kEmptyKey) {
+            if (i < have && !key_unadmitted(key)) {
                 const float d = key_dist(key);
                 r = key_slot(key);
-                const bool near_prev = i > 0 && keys[i - 1] != kEmptyKey && d - key_dist(keys[i - 1]) <= band;
-                const bool near_nxt = i + 1 < have && keys[i + 1] != kEmptyKey && key_dist(keys[i + 1]) - d <= band;
+                const bool near_prev = i > 0 && !key_unadmitted(keys[i - 1]) && d - key_dist(keys[i - 1]) <= band;
+                const bool near_nxt = i + 1 < have && !key_unadmitted(keys[i + 1]) && key_dist(keys[i + 1]) - d <= band;
                 const uint64_t bits = __double_as_longlong((double)d);
                 v = (bits & 0x8000000000000000ull) ? ~bits : (bits | 0x8000000000000000ull);
                 if ((near_prev || near_nxt) && d <= lim) {
@@ -764,14 +837,14 @@ __global__ __launch_bounds__(256) void flat_tail_kernel(
     for (uint32_t i = lane; i < a.top_k; i += kWave) {
         const uint64_t key = i < k && i < hp ? keys[i] : kEmptyKey;
         GpuSearchResult r;
-        if (key == kEmptyKey) {
+        if (key_unadmitted(key)) {
             r.id = FV_EMPTY_ID;
             r.distance = FV_EMPTY_DIST;
         } else {
             r.id = key_slot(key);
             r.distance = key_dist(key);
         }
-        out[(size_t)q * a.top_k + i] = r;
+        oq[i] = r;
     }
 }
 
