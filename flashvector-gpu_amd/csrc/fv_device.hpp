@@ -142,6 +142,16 @@ __device__ __forceinline__ void wave_bitonic_sort_pairs(uint64_t* keys, uint32_t
     }
 }
 
+// Inclusive prefix sum of v over the lanes of a wavefront.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t t = __shfl_up(v, off);
+        if (lane >= (uint32_t)off) v += t;
+    }
+    return v;
+}
+
 // Prune of one query's candidate buffer by one wavefront: sorts the `count`
 // buffered keys through LDS scratch (capacity >= next_pow2(count)), writes the
 // `keep` smallest back in ascending order and returns how many remain.

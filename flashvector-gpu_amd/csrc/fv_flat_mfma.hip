@@ -491,15 +491,6 @@ __global__ __launch_bounds__(256) void flat_finalize_kernel(
 constexpr uint32_t kTailMaxS = 256;
 constexpr uint32_t kTailWaves = 4;
 
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const uint32_t t = __shfl_up(v, off);
-        if (lane >= (uint32_t)off) v += t;
-    }
-    return v;
-}
-
 // Exact distances and error bounds of NR rows of one query (rows[] valid,
 // possibly repeated), summed as RerankSum does.
 template <int METRIC, int NR>

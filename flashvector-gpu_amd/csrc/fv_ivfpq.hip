@@ -64,26 +64,213 @@ __global__ __launch_bounds__(256) void ivf_coarse_kernel(
     }
 }
 
-// Sorts the buffer, keeps the `keep` smallest and returns the new threshold
-// key (kEmptyKey while fewer than `keep` candidates exist). All threads call.
-__device__ __forceinline__ uint64_t block_prune(uint64_t* buf, uint32_t* s_cnt, uint32_t keep) {
-    const uint32_t c = *s_cnt;
-    const uint32_t n = next_pow2(c < 2u ? 2u : c);
-    for (uint32_t i = c + threadIdx.x; i < n; i += blockDim.x) buf[i] = kEmptyKey;
+// Workgroup control words and selection scratch of the scan kernel, in
+// dynamic LDS after the LUT and the candidate buffer.
+struct IvfCtl {
+    uint64_t thr;                              // key returned by block_select
+    uint32_t item;                             // position of the current query in `order`
+    uint32_t sel[3];                           // radix round: digit, keys below it, keys in it
+    uint32_t wn[3][kIvfThreads / kWave];       // per-wave keys to append: 2 checks, after a cut
+    uint64_t red[2 * (kIvfThreads / kWave)];   // per-wave smallest / largest key
+    uint32_t wsum[2][kIvfThreads / kWave];     // per-wave survivor counts
+    uint32_t hist[2][256];                     // digit histograms of consecutive rounds
+};
+constexpr size_t kIvfCtlBytes = (sizeof(IvfCtl) + 15) / 16 * 16;
+static_assert(kIvfThreads == 256, "one thread per histogram bin and per PQ codeword");
+
+// Keeps the `keep` smallest of the c distinct keys buf[0..c) (c > keep >= 1),
+// compacted into buf[0..keep) in no particular order, and returns the
+// keep-th smallest to every thread.
+//
+// Radix selection: the bits above the highest bit in which the smallest and
+// largest key differ are common to all keys; below it, 8-bit digits are
+// histogrammed one round at a time over the keys that share the prefix found
+// so far, until the digit holding the keep-th smallest holds no other key (or
+// every bit is fixed). The survivors are then exactly the keys whose prefix
+// is at most the final one. All threads call this after a barrier; it ends
+// with every read of buf done and the survivors written by their own
+// threads, so the caller must barrier before other threads read buf[0..keep).
+__device__ uint64_t block_select(uint64_t* buf, uint32_t c, uint32_t keep, IvfCtl& ctl) {
+    constexpr uint32_t kW = kIvfThreads / kWave;
+    const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    unsigned long long lo = kEmptyKey, hi = 0;
+    for (uint32_t i = tid; i < c; i += kIvfThreads) {
+        const unsigned long long k = buf[i];
+        lo = k < lo ? k : lo;
+        hi = k > hi ? k : hi;
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const unsigned long long a = __shfl_xor(lo, off), b = __shfl_xor(hi, off);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+    if (lane == 0) {
+        ctl.red[wave] = lo;
+        ctl.red[kW + wave] = hi;
+    }
+    ctl.hist[0][tid] = 0;
     __syncthreads();
-    block_bitonic_sort(buf, n);
-    const uint32_t kept = c < keep ? c : keep;
-    uint64_t thr = kept == keep ? buf[keep - 1] : kEmptyKey;
-    __syncthreads();
-    if (threadIdx.x == 0) *s_cnt = kept;
-    __syncthreads();
-    return thr;
+    lo = ctl.red[0];
+    hi = ctl.red[kW];
+    for (uint32_t w = 1; w < kW; ++w) {
+        lo = ctl.red[w] < lo ? ctl.red[w] : lo;
+        hi = ctl.red[kW + w] > hi ? ctl.red[kW + w] : hi;
+    }
+    uint32_t hb = 64u - (uint32_t)__clzll((long long)(lo ^ hi));  // undetermined low bits
+    uint64_t pmask = hb >= 64u ? 0ull : ~0ull << hb;
+    uint64_t prefix = lo & pmask;
+    uint32_t need = keep;  // rank of the wanted key among those sharing the prefix
+    for (uint32_t r = 0;; ++r) {
+        const uint32_t nb = hb < 8u ? hb : 8u, shift = hb - nb;
+        const uint64_t dmask = (1ull << nb) - 1ull;
+        uint32_t* h = ctl.hist[r & 1u];
+        for (uint32_t i = tid; i < c; i += kIvfThreads) {
+            const uint64_t k = buf[i];
+            if ((k & pmask) == prefix) atomicAdd(&h[(uint32_t)((k >> shift) & dmask)], 1u);
+        }
+        ctl.hist[(r + 1u) & 1u][tid] = 0;
+        __syncthreads();
+        if (wave == 0) {
+            const uint32_t h0 = h[4 * lane], h1 = h[4 * lane + 1], h2 = h[4 * lane + 2],
+                           h3 = h[4 * lane + 3];
+            const uint32_t s = h0 + h1 + h2 + h3;
+            const uint32_t incl = wave_incl_scan(s, lane), excl = incl - s;
+            if (excl < need && need <= incl) {
+                uint32_t b = 4 * lane, before = excl, cnt = h0;
+                if (before + h0 < need) {
+                    before += h0; ++b; cnt = h1;
+                    if (before + h1 < need) {
+                        before += h1; ++b; cnt = h2;
+                        if (before + h2 < need) {
+                            before += h2; ++b; cnt = h3;
+                        }
+                    }
+                }
+                ctl.sel[0] = b;
+                ctl.sel[1] = before;
+                ctl.sel[2] = cnt;
+            }
+        }
+        __syncthreads();
+        const uint32_t b = ctl.sel[0], cnt = ctl.sel[2];
+        need -= ctl.sel[1];
+        prefix |= (uint64_t)b << shift;
+        pmask |= dmask << shift;
+        hb = shift;
+        if (cnt == 1u || hb == 0u) break;
+    }
+    // In-place compaction, 8 keys per thread per chunk: every key of a chunk
+    // is read before the chunk's barrier and survivors are written after it,
+    // to positions below the number of keys read so far.
+    uint32_t running = 0;
+    for (uint32_t base = 0, ch = 0; base < c; base += 8u * kIvfThreads, ++ch) {
+        uint64_t v[8];
+        uint32_t keepmask = 0, n = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < 8u; ++u) {
+            const uint32_t i = base + u * kIvfThreads + tid;
+            v[u] = i < c ? buf[i] : kEmptyKey;
+            const uint64_t mp = v[u] & pmask;
+            if (i < c && mp <= prefix) {
+                keepmask |= 1u << u;
+                ++n;
+                if (mp == prefix) ctl.thr = v[u];
+            }
+        }
+        const uint32_t incl = wave_incl_scan(n, lane);
+        if (lane == kWave - 1) ctl.wsum[ch & 1u][wave] = incl;
+        __syncthreads();
+        uint32_t off = running + incl - n;
+        for (uint32_t w = 0; w < kW; ++w) {
+            const uint32_t ws = ctl.wsum[ch & 1u][w];
+            off += w < wave ? ws : 0u;
+            running += ws;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 8u; ++u)
+            if (keepmask & (1u << u)) buf[off++] = v[u];
+    }
+    return ctl.thr;
+}
+
+// LUT [M][256] of one probed list: lut[m][j] = sum over s of
+// (res[m*SUB+s] - codebook[m][j][s])^2 as an fmaf chain in s. One codeword
+// per thread; the codebook rows of NM sub-quantizers (8 float4 loads per
+// thread) are issued together before their sums.
+template <int UPM>
+__device__ __forceinline__ void build_lut_vec(float* lut, const float* __restrict__ cbk,
+                                              const float* res, uint32_t M) {
+    constexpr uint32_t SUB = 4 * UPM, NM = 8 / UPM;
+    const uint32_t j = threadIdx.x;
+    for (uint32_t m0 = 0; m0 < M; m0 += NM) {
+        float4 v[NM][UPM];
+#pragma unroll
+        for (uint32_t a = 0; a < NM; ++a)
+#pragma unroll
+            for (uint32_t u = 0; u < (uint32_t)UPM; ++u)
+                if (m0 + a < M)
+                    v[a][u] = reinterpret_cast<const float4*>(cbk + ((size_t)(m0 + a) * 256u + j) * SUB)[u];
+#pragma unroll
+        for (uint32_t a = 0; a < NM; ++a) {
+            if (m0 + a < M) {
+                const float* r = res + (m0 + a) * SUB;
+                float dist = 0.0f, diff;
+#pragma unroll
+                for (uint32_t u = 0; u < (uint32_t)UPM; ++u) {
+                    diff = r[4 * u + 0] - v[a][u].x; dist = __builtin_fmaf(diff, diff, dist);
+                    diff = r[4 * u + 1] - v[a][u].y; dist = __builtin_fmaf(diff, diff, dist);
+                    diff = r[4 * u + 2] - v[a][u].z; dist = __builtin_fmaf(diff, diff, dist);
+                    diff = r[4 * u + 3] - v[a][u].w; dist = __builtin_fmaf(diff, diff, dist);
+                }
+                lut[(m0 + a) * 256u + j] = dist;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void build_lut(float* lut, const float* __restrict__ cbk, const float* res,
+                                          uint32_t M, uint32_t sub) {
+    const bool vec = (reinterpret_cast<uintptr_t>(cbk) & 15u) == 0u;
+    if (vec && sub == 8u) {
+        build_lut_vec<2>(lut, cbk, res, M);
+    } else if (vec && sub == 4u) {
+        build_lut_vec<1>(lut, cbk, res, M);
+    } else if (vec && sub == 16u) {
+        build_lut_vec<4>(lut, cbk, res, M);
+    } else if (vec && sub == 32u) {
+        build_lut_vec<8>(lut, cbk, res, M);
+    } else {
+        const uint32_t j = threadIdx.x;
+        for (uint32_t m = 0; m < M; ++m) {
+            const float* cb = cbk + ((size_t)m * 256u + j) * sub;
+            const float* r = res + m * sub;
+            float dist = 0.0f;
+            for (uint32_t s = 0; s < sub; ++s) {
+                float diff = r[s] - cb[s];
+                dist = __builtin_fmaf(diff, diff, dist);
+            }
+            lut[m * 256u + j] = dist;
+        }
+    }
 }
 
 // Persistent: each workgroup repeatedly takes the next query from `order`
 // (heaviest first) through a global atomic counter, so queries of very uneven
 // cost (Zipf lists, per-query nprobe) are balanced dynamically.
-template <bool LISTORDER, bool GBUF, int METRIC>
+//
+// M16 (16 sub-quantizers): the LUT is a static LDS array, the kernel's only
+// static LDS object and so at LDS address 0, which lets each lookup's byte
+// extract and scale fold into one SDWA shift (the address is code*4 plus an
+// immediate). Other M keep the LUT at the start of dynamic LDS.
+//
+// Candidates: each check of kEntriesPerCheck entries computes its keys into
+// registers and every wave publishes how many are below the threshold. Only
+// if appending them would overflow the buffer is it cut to the S best
+// (block_select) and the threshold lowered (and the waves recount). Keys are
+// then appended at positions from the per-wave counts, so the buffer fill and
+// the threshold are uniform registers, identical in every thread.
+template <bool LISTORDER, bool GBUF, int METRIC, bool M16>
 __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(
     IvfPqGpuTables t, const uint8_t* __restrict__ codes_lo, const float* __restrict__ Qm,
     uint32_t nq, const uint32_t* __restrict__ order, uint32_t* __restrict__ next_q,
@@ -91,53 +278,49 @@ __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(
     uint32_t k_u, uint32_t shortlist_u, uint32_t cap, int do_rerank,
     const uint32_t* __restrict__ mask, uint64_t* __restrict__ gbuf,
     GpuSearchResult* __restrict__ out, uint32_t out_stride) {
+    // Dynamic LDS: LUT [M][256] f32 (unless static), candidate buffer (unless
+    // in global memory), control words, the probed list's residual [dim].
+    __shared__ float s_lut16[M16 ? 16 * 256 : 1];
     extern __shared__ uint8_t s_dyn[];
-    float* lut = reinterpret_cast<float*>(s_dyn);
-    uint64_t* buf = GBUF ? gbuf + (size_t)blockIdx.x * cap
-                         : reinterpret_cast<uint64_t*>(s_dyn + (size_t)t.m_pq * 256 * sizeof(float));
-    __shared__ uint32_t s_cnt;
-    __shared__ uint64_t s_thr;
-    __shared__ uint32_t s_item;
+    float* lut = M16 ? s_lut16 : reinterpret_cast<float*>(s_dyn);
+    const size_t lut_bytes = M16 ? 0 : (size_t)t.m_pq * 256 * sizeof(float);
+    uint64_t* buf = GBUF ? gbuf + (size_t)blockIdx.x * cap : reinterpret_cast<uint64_t*>(s_dyn + lut_bytes);
+    uint8_t* ctl_base = s_dyn + lut_bytes + (GBUF ? (size_t)0 : (size_t)cap * 8);
+    IvfCtl& ctl = *reinterpret_cast<IvfCtl*>(ctl_base);
+    float* res = reinterpret_cast<float*>(ctl_base + kIvfCtlBytes);
+    constexpr uint32_t kW = kIvfThreads / kWave;
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
 
   for (;;) {
     __syncthreads();
-    if (threadIdx.x == 0) s_item = atomicAdd(next_q, 1u);
+    if (threadIdx.x == 0) ctl.item = atomicAdd(next_q, 1u);
     __syncthreads();
-    if (s_item >= nq) return;
-    const uint32_t q = order[s_item];
+    if (ctl.item >= nq) return;
+    const uint32_t q = order[ctl.item];
     const uint32_t M = t.m_pq, sub = t.sub_dim, dim = t.dim;
     const float* qv = Qm + (size_t)q * dim;
     uint32_t k = k_q ? k_q[q] : k_u;
     if (k > out_stride) k = out_stride;
     uint32_t S = shortlist_u < k ? k : shortlist_u;
     if (S > cap - kEntriesPerCheck) S = cap - kEntriesPerCheck;
-    if (threadIdx.x == 0) {
-        s_cnt = 0;
-        s_thr = kEmptyKey;
-    }
-    __syncthreads();
+    if (S == 0) S = 1;
+    uint32_t cnt = 0;         // keys in the candidate buffer
+    uint64_t thr = kEmptyKey; // admission threshold: the S-th best key so far
+    uint32_t ph = 0;          // which wn[] row the current check uses
 
     for (uint32_t p = 0; p < max_nprobe; ++p) {
         const uint32_t c = probes[(size_t)q * max_nprobe + p];
         if (c >= t.nlist) break;  // probes are packed; the rest are empty
         const float* cen = t.d_centroids + (size_t)c * dim;
-        __syncthreads();  // previous list's lookups are done with the LUT
-        for (uint32_t idx = threadIdx.x; idx < M * 256u; idx += blockDim.x) {
-            const uint32_t m = idx >> 8, j = idx & 255u;
-            const float* cb = t.d_pq_codebooks + ((size_t)m * 256u + j) * sub;
-            float dist = 0.0f;
-            for (uint32_t s = 0; s < sub; ++s) {
-                float r = qv[m * sub + s] - cen[m * sub + s];
-                float diff = r - cb[s];
-                dist = __builtin_fmaf(diff, diff, dist);
-            }
-            lut[idx] = dist;
-        }
+        // The previous list's LUT build is done with res (barrier after it).
+        for (uint32_t d = threadIdx.x; d < dim; d += kIvfThreads) res[d] = qv[d] - cen[d];
+        __syncthreads();  // res visible; the previous list's lookups are done with the LUT
+        build_lut(lut, t.d_pq_codebooks, res, M, sub);
         __syncthreads();
         const uint32_t beg = t.d_ivf_offsets[c], end = t.d_ivf_offsets[c + 1];
         for (uint32_t base = beg; base < end; base += kEntriesPerCheck) {
-            const uint64_t thr = s_thr;
-            if (M == 16u) {
+            uint64_t kk[kPassesPerCheck];
+            if (M16) {
                 // Issue the id and code loads of all passes first so that each
                 // wave keeps kPassesPerCheck 128-bit loads in flight.
                 uint32_t ids[kPassesPerCheck];
@@ -167,47 +350,81 @@ __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(
                 for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
                     const uint32_t e = base + pass * kIvfThreads + threadIdx.x;
                     const uint32_t id = ids[pass];
+                    kk[pass] = kEmptyKey;
                     if (e >= end || !mask_allows(mask, id)) continue;
                     const uint32_t ww[4] = {cw[pass].x, cw[pass].y, cw[pass].z, cw[pass].w};
                     float adc = 0.0f;
 #pragma unroll
                     for (int b = 0; b < 16; ++b) {
                         const uint32_t code = (ww[b >> 2] >> (8 * (b & 3))) & 255u;
-                        adc = adc + lut[b * 256u + code];
+                        adc = adc + s_lut16[b * 256u + code];
                     }
-                    if (adc < kAdmit) {
-                        const uint64_t key = make_key(adc, id);
-                        if (key < thr) buf[atomicAdd(&s_cnt, 1u)] = key;
-                    }
+                    if (adc < kAdmit) kk[pass] = make_key(adc, id);
                 }
             } else {
-#pragma unroll 1
+#pragma unroll
                 for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
                     const uint32_t e = base + pass * kIvfThreads + threadIdx.x;
-                    if (e >= end) break;
+                    kk[pass] = kEmptyKey;
+                    if (e >= end) continue;
                     const uint32_t id = t.d_ivf_vec_ids[e];
                     if (!mask_allows(mask, id)) continue;
                     const uint8_t* cp = LISTORDER ? codes_lo + (size_t)e * M : t.d_pq_codes + (size_t)id * M;
                     float adc = 0.0f;
                     for (uint32_t m = 0; m < M; ++m) adc = adc + lut[m * 256u + cp[m]];
-                    if (adc < kAdmit) {
-                        const uint64_t key = make_key(adc, id);
-                        if (key < thr) buf[atomicAdd(&s_cnt, 1u)] = key;
-                    }
+                    if (adc < kAdmit) kk[pass] = make_key(adc, id);
                 }
             }
+            // wn[ph] row: written before this check's barrier, read after it
+            // and before the next one; the next write to it is two checks on.
+            uint32_t wn = 0;
+#pragma unroll
+            for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass)
+                wn += (uint32_t)__popcll(__ballot(kk[pass] < thr));
+            if (lane == 0) ctl.wn[ph][wave] = wn;
             __syncthreads();
-            if (s_cnt + kEntriesPerCheck > cap) {
-                const uint64_t nt = block_prune(buf, &s_cnt, S);
-                if (threadIdx.x == 0 && nt < s_thr) s_thr = nt;
+            uint32_t row = ph, total = 0;
+#pragma unroll
+            for (uint32_t w = 0; w < kW; ++w) total += ctl.wn[ph][w];
+            if (cnt + total > cap) {
+                thr = block_select(buf, cnt, S, ctl);
+                cnt = S;
+                wn = 0;
+#pragma unroll
+                for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass)
+                    wn += (uint32_t)__popcll(__ballot(kk[pass] < thr));
+                if (lane == 0) ctl.wn[2][wave] = wn;
                 __syncthreads();
+                row = 2;
+                total = 0;
+#pragma unroll
+                for (uint32_t w = 0; w < kW; ++w) total += ctl.wn[2][w];
             }
+            uint32_t pos = cnt;
+#pragma unroll
+            for (uint32_t w = 0; w < kW; ++w) pos += w < wave ? ctl.wn[row][w] : 0u;
+#pragma unroll
+            for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
+                const bool in = kk[pass] < thr;
+                const uint64_t bal = __ballot(in);
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                           __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                if (in) buf[pos + below] = kk[pass];
+                pos += (uint32_t)__popcll(bal);
+            }
+            cnt += total;
+            ph ^= 1u;
         }
     }
     __syncthreads();
-    block_prune(buf, &s_cnt, S);
-    const uint32_t kept = s_cnt;
+    uint32_t kept = cnt;
+    if (kept > S) {
+        block_select(buf, kept, S, ctl);
+        kept = S;
+        __syncthreads();
+    }
     // Exact rerank of the shortlist, in place.
+    const uint32_t n = next_pow2(kept < 2u ? 2u : kept);
     if (do_rerank && t.d_vectors) {
         for (uint32_t i = threadIdx.x; i < kept; i += blockDim.x) {
             const uint32_t id = key_slot(buf[i]);
@@ -216,11 +433,10 @@ __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(
                 : FV_EMPTY_DIST;
             buf[i] = d < kAdmit ? make_key(d, id) : kEmptyKey;
         }
-        const uint32_t n = next_pow2(kept < 2u ? 2u : kept);
-        for (uint32_t i = kept + threadIdx.x; i < n; i += blockDim.x) buf[i] = kEmptyKey;
-        __syncthreads();
-        block_bitonic_sort(buf, n);
     }
+    for (uint32_t i = kept + threadIdx.x; i < n; i += blockDim.x) buf[i] = kEmptyKey;
+    __syncthreads();
+    block_bitonic_sort(buf, n);
     for (uint32_t i = threadIdx.x; i < out_stride; i += blockDim.x) {
         GpuSearchResult r;
         const uint64_t key = (i < k && i < kept) ? buf[i] : kEmptyKey;
@@ -308,7 +524,9 @@ IvfPqPlan plan_ivfpq(const IvfPqGpuTables* t, uint32_t nq, uint32_t max_nprobe,
     p.npow = next_pow2(t->nlist < 2u ? 2u : t->nlist);
     const size_t lut_bytes = (size_t)t->m_pq * 256 * sizeof(float);
     p.gbuf = lut_bytes + (size_t)p.cap * 8 > 64 * 1024;
-    p.lds_scan = lut_bytes + (p.gbuf ? 0 : (size_t)p.cap * 8);
+    // Dynamic part only: with 16 sub-quantizers the LUT is static LDS.
+    p.lds_scan = (t->m_pq == 16u ? 0 : lut_bytes) + (p.gbuf ? 0 : (size_t)p.cap * 8) +
+                 kIvfCtlBytes + (size_t)t->dim * sizeof(float);
     p.grid = nq < kMaxScanGrid ? nq : kMaxScanGrid;
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -331,7 +549,7 @@ IvfPqPlan plan_ivfpq(const IvfPqGpuTables* t, uint32_t nq, uint32_t max_nprobe,
     return p;
 }
 
-template <bool LO, bool GB>
+template <bool LO, bool GB, bool M16>
 static void launch_scan_m(int metric, dim3 g, size_t lds, hipStream_t s, const IvfPqGpuTables& t,
                           const uint8_t* lo, const float* Qm, uint32_t nq, const uint32_t* order,
                           uint32_t* next_q, const uint32_t* probes, uint32_t mnp,
@@ -339,11 +557,24 @@ static void launch_scan_m(int metric, dim3 g, size_t lds, hipStream_t s, const I
                           const uint32_t* mask, uint64_t* gbuf, GpuSearchResult* out,
                           uint32_t stride) {
     if (metric == METRIC_INNER_PRODUCT)
-        ivf_scan_kernel<LO, GB, METRIC_INNER_PRODUCT><<<g, kIvfThreads, lds, s>>>(t, lo, Qm, nq, order, next_q, probes, mnp, k_q, k_u, S, cap, rr, mask, gbuf, out, stride);
+        ivf_scan_kernel<LO, GB, METRIC_INNER_PRODUCT, M16><<<g, kIvfThreads, lds, s>>>(t, lo, Qm, nq, order, next_q, probes, mnp, k_q, k_u, S, cap, rr, mask, gbuf, out, stride);
     else if (metric == METRIC_COSINE)
-        ivf_scan_kernel<LO, GB, METRIC_COSINE><<<g, kIvfThreads, lds, s>>>(t, lo, Qm, nq, order, next_q, probes, mnp, k_q, k_u, S, cap, rr, mask, gbuf, out, stride);
+        ivf_scan_kernel<LO, GB, METRIC_COSINE, M16><<<g, kIvfThreads, lds, s>>>(t, lo, Qm, nq, order, next_q, probes, mnp, k_q, k_u, S, cap, rr, mask, gbuf, out, stride);
     else
-        ivf_scan_kernel<LO, GB, METRIC_L2><<<g, kIvfThreads, lds, s>>>(t, lo, Qm, nq, order, next_q, probes, mnp, k_q, k_u, S, cap, rr, mask, gbuf, out, stride);
+        ivf_scan_kernel<LO, GB, METRIC_L2, M16><<<g, kIvfThreads, lds, s>>>(t, lo, Qm, nq, order, next_q, probes, mnp, k_q, k_u, S, cap, rr, mask, gbuf, out, stride);
+}
+
+template <bool LO, bool GB>
+static void launch_scan_lo_gb(bool m16, int metric, dim3 g, size_t lds, hipStream_t s,
+                              const IvfPqGpuTables& t, const uint8_t* lo, const float* Qm,
+                              uint32_t nq, const uint32_t* order, uint32_t* next_q,
+                              const uint32_t* probes, uint32_t mnp, const uint32_t* k_q,
+                              uint32_t k_u, uint32_t S, uint32_t cap, int rr, const uint32_t* mask,
+                              uint64_t* gbuf, GpuSearchResult* out, uint32_t stride) {
+    if (m16)
+        launch_scan_m<LO, GB, true>(metric, g, lds, s, t, lo, Qm, nq, order, next_q, probes, mnp, k_q, k_u, S, cap, rr, mask, gbuf, out, stride);
+    else
+        launch_scan_m<LO, GB, false>(metric, g, lds, s, t, lo, Qm, nq, order, next_q, probes, mnp, k_q, k_u, S, cap, rr, mask, gbuf, out, stride);
 }
 
 hipError_t launch_ivfpq(const IvfPqGpuTables* t, const uint8_t* codes_lo, const float* Qm,
@@ -351,6 +582,8 @@ hipError_t launch_ivfpq(const IvfPqGpuTables* t, const uint8_t* codes_lo, const 
                         uint32_t nprobe_u, int do_rerank, int metric, const uint32_t* mask,
                         GpuSearchResult* out, uint32_t out_stride, void* workspace,
                         const IvfPqPlan& p, hipStream_t stream) {
+    // LUT (static with 16 sub-quantizers) + dynamic LDS must fit one CU's LDS.
+    if ((t->m_pq == 16u ? 16384u : 0u) + p.lds_scan > 160u * 1024u) return hipErrorInvalidValue;
     char* ws = static_cast<char*>(workspace);
     uint32_t* probes = reinterpret_cast<uint32_t*>(ws + p.off_probes);
     uint64_t* gbuf = p.gbuf ? reinterpret_cast<uint64_t*>(ws + p.off_gbuf) : nullptr;
@@ -381,14 +614,15 @@ hipError_t launch_ivfpq(const IvfPqGpuTables* t, const uint8_t* codes_lo, const 
     void* timer = timer_begin(stream);
     const dim3 g(p.grid);
     const bool lo = codes_lo != nullptr;
+    const bool m16 = t->m_pq == 16u;
     if (lo && p.gbuf)
-        launch_scan_m<true, true>(metric, g, p.lds_scan, stream, *t, codes_lo, Qm, nq, order, next_q, probes, p.max_nprobe, k_q, k_u, p.shortlist, p.cap, do_rerank, mask, gbuf, out, out_stride);
+        launch_scan_lo_gb<true, true>(m16, metric, g, p.lds_scan, stream, *t, codes_lo, Qm, nq, order, next_q, probes, p.max_nprobe, k_q, k_u, p.shortlist, p.cap, do_rerank, mask, gbuf, out, out_stride);
     else if (lo)
-        launch_scan_m<true, false>(metric, g, p.lds_scan, stream, *t, codes_lo, Qm, nq, order, next_q, probes, p.max_nprobe, k_q, k_u, p.shortlist, p.cap, do_rerank, mask, gbuf, out, out_stride);
+        launch_scan_lo_gb<true, false>(m16, metric, g, p.lds_scan, stream, *t, codes_lo, Qm, nq, order, next_q, probes, p.max_nprobe, k_q, k_u, p.shortlist, p.cap, do_rerank, mask, gbuf, out, out_stride);
     else if (p.gbuf)
-        launch_scan_m<false, true>(metric, g, p.lds_scan, stream, *t, codes_lo, Qm, nq, order, next_q, probes, p.max_nprobe, k_q, k_u, p.shortlist, p.cap, do_rerank, mask, gbuf, out, out_stride);
+        launch_scan_lo_gb<false, true>(m16, metric, g, p.lds_scan, stream, *t, codes_lo, Qm, nq, order, next_q, probes, p.max_nprobe, k_q, k_u, p.shortlist, p.cap, do_rerank, mask, gbuf, out, out_stride);
     else
-        launch_scan_m<false, false>(metric, g, p.lds_scan, stream, *t, codes_lo, Qm, nq, order, next_q, probes, p.max_nprobe, k_q, k_u, p.shortlist, p.cap, do_rerank, mask, gbuf, out, out_stride);
+        launch_scan_lo_gb<false, false>(m16, metric, g, p.lds_scan, stream, *t, codes_lo, Qm, nq, order, next_q, probes, p.max_nprobe, k_q, k_u, p.shortlist, p.cap, do_rerank, mask, gbuf, out, out_stride);
     timer_end(timer, stream);
     return hipGetLastError();
 }

@@ -101,6 +101,26 @@ def test_integer_codebooks_order_independent(torch):
     assert_same(got, O.ivf_pq_search_batch(host_tables(t), qs, 15, 5, 1))
 
 
+@pytest.mark.parametrize("dim,m_pq", [
+    (128, 16),   # sub 8, the 10M-row workload's shape
+    (64, 4),     # sub 16
+    (96, 3),     # sub 32
+    (36, 12),    # sub 3: scalar LUT build, LUT in dynamic LDS
+    (48, 16),    # sub 3 with the static 16-sub-quantizer LUT
+])
+def test_lut_shapes_vs_oracle(torch, dim, m_pq):
+    """Every LUT build shape (vector loads of 1, 2, 4 or 8 float4 per
+    codeword, or the scalar loop) and both LUT placements, bit-exact, with
+    shortlists shallow enough that the candidate buffer is cut many times."""
+    db, qs = S.flat_workload(6000, 40, dim, 24, seeds=(61, 62, 63))
+    t = build_ivfpq(db, 16, m_pq, seed=64, iters=3)
+    idx = F.IvfPqIndex.from_host(t)
+    h = host_tables(t)
+    for k, nprobe, rr in [(10, 6, 40), (7, 16, 1), (25, 3, 600)]:
+        got = L.split_results(idx.search_batch(dev(torch, qs), k, nprobe, rr))
+        assert_same(got, O.ivf_pq_search_batch(h, qs, k, nprobe, rr))
+
+
 def test_exhaustive_probe_matches_oracle_and_flat(torch, case):
     db, qs, t, idx, h = case
     got = L.split_results(idx.search_batch(dev(torch, qs[:20]), 10, t.nlist, 10240))
