@@ -76,6 +76,8 @@ struct IvfCtl {
     uint32_t hist[2][256];                     // digit histograms of consecutive rounds
 };
 constexpr size_t kIvfCtlBytes = (sizeof(IvfCtl) + 15) / 16 * 16;
+// Distance bits of an admitted key are never all ones (ord of a NaN).
+constexpr uint64_t kUnadmittedKey = 0xFFFFFFFF00000000ull;
 static_assert(kIvfThreads == 256, "one thread per histogram bin and per PQ codeword");
 
 // Keeps the `keep` smallest of the c distinct keys buf[0..c) (c > keep >= 1),
@@ -423,24 +425,43 @@ __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(
         kept = S;
         __syncthreads();
     }
-    // Exact rerank of the shortlist, in place.
-    const uint32_t n = next_pow2(kept < 2u ? 2u : kept);
+    // Exact rerank of the shortlist, in place. Rows that are not admitted get
+    // distinct keys above every admitted one (distance bits all ones).
     if (do_rerank && t.d_vectors) {
         for (uint32_t i = threadIdx.x; i < kept; i += blockDim.x) {
             const uint32_t id = key_slot(buf[i]);
-            float d = id < t.num_vectors
-                ? exact_distance<METRIC>(qv, t.d_vectors + (size_t)id * dim, dim)
+            const float d = id < t.num_vectors
+                ? exact_distance_vec<METRIC>(qv, t.d_vectors + (size_t)id * dim, dim)
                 : FV_EMPTY_DIST;
-            buf[i] = d < kAdmit ? make_key(d, id) : kEmptyKey;
+            buf[i] = d < kAdmit ? make_key(d, id) : (kUnadmittedKey | i);
         }
+        __syncthreads();
     }
-    for (uint32_t i = kept + threadIdx.x; i < n; i += blockDim.x) buf[i] = kEmptyKey;
-    __syncthreads();
-    block_bitonic_sort(buf, n);
+    // The k best, in (distance, id) order: cut to k, then each key is written
+    // at its rank among the k (or, for k above one key per thread, sorted).
+    const uint32_t kout = k < kept ? k : kept;
+    if (kept > kout && kout > 0u) {
+        block_select(buf, kept, kout, ctl);
+        __syncthreads();
+    }
+    if (kout <= (uint32_t)kIvfThreads) {
+        const uint64_t mine = threadIdx.x < kout ? buf[threadIdx.x] : kEmptyKey;
+        uint32_t rank = 0;
+        if (threadIdx.x < kout)
+            for (uint32_t j = 0; j < kout; ++j) rank += buf[j] < mine ? 1u : 0u;
+        __syncthreads();
+        if (threadIdx.x < kout) buf[rank] = mine;
+        __syncthreads();
+    } else {
+        const uint32_t n = next_pow2(kout);
+        for (uint32_t i = kout + threadIdx.x; i < n; i += blockDim.x) buf[i] = kEmptyKey;
+        __syncthreads();
+        block_bitonic_sort(buf, n);
+    }
     for (uint32_t i = threadIdx.x; i < out_stride; i += blockDim.x) {
         GpuSearchResult r;
-        const uint64_t key = (i < k && i < kept) ? buf[i] : kEmptyKey;
-        if (key == kEmptyKey) {
+        const uint64_t key = i < kout ? buf[i] : kEmptyKey;
+        if ((key >> 32) == 0xFFFFFFFFull) {
             r.id = FV_EMPTY_ID;
             r.distance = FV_EMPTY_DIST;
         } else {
