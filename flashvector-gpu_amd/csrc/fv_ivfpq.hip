@@ -318,8 +318,23 @@ __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(
         for (uint32_t d = threadIdx.x; d < dim; d += kIvfThreads) res[d] = qv[d] - cen[d];
         __syncthreads();  // res visible; the previous list's lookups are done with the LUT
         build_lut(lut, t.d_pq_codebooks, res, M, sub);
-        __syncthreads();
         const uint32_t beg = t.d_ivf_offsets[c], end = t.d_ivf_offsets[c + 1];
+        // List-ordered codes, 16 sub-quantizers: the ids and codes of the next
+        // check are loaded while the current one is scored, so each wave
+        // keeps up to 2 * kPassesPerCheck loads in flight across barriers.
+        constexpr bool kPrefetch = M16 && LISTORDER;
+        uint32_t nids[kPassesPerCheck];
+        uint4 ncw[kPassesPerCheck];
+        if (kPrefetch && beg < end) {
+#pragma unroll
+            for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
+                const uint32_t e = beg + pass * kIvfThreads + threadIdx.x;
+                const uint32_t ec = e < end ? e : end - 1;
+                nids[pass] = t.d_ivf_vec_ids[ec];
+                ncw[pass] = *reinterpret_cast<const uint4*>(codes_lo + (size_t)ec * 16u);
+            }
+        }
+        __syncthreads();
         for (uint32_t base = beg; base < end; base += kEntriesPerCheck) {
             uint64_t kk[kPassesPerCheck];
             if (M16) {
@@ -327,12 +342,30 @@ __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(
                 // wave keeps kPassesPerCheck 128-bit loads in flight.
                 uint32_t ids[kPassesPerCheck];
                 uint4 cw[kPassesPerCheck];
+                if (kPrefetch) {
 #pragma unroll
-                for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
-                    const uint32_t e = base + pass * kIvfThreads + threadIdx.x;
-                    const uint32_t ec = e < end ? e : end - 1;
-                    ids[pass] = t.d_ivf_vec_ids[ec];
-                    if (LISTORDER) cw[pass] = *reinterpret_cast<const uint4*>(codes_lo + (size_t)ec * 16u);
+                    for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
+                        ids[pass] = nids[pass];
+                        cw[pass] = ncw[pass];
+                    }
+                    const uint32_t nb = base + kEntriesPerCheck;
+                    if (nb < end) {
+#pragma unroll
+                        for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
+                            const uint32_t e = nb + pass * kIvfThreads + threadIdx.x;
+                            const uint32_t ec = e < end ? e : end - 1;
+                            nids[pass] = t.d_ivf_vec_ids[ec];
+                            ncw[pass] = *reinterpret_cast<const uint4*>(codes_lo + (size_t)ec * 16u);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
+                        const uint32_t e = base + pass * kIvfThreads + threadIdx.x;
+                        const uint32_t ec = e < end ? e : end - 1;
+                        ids[pass] = t.d_ivf_vec_ids[ec];
+                        if (LISTORDER) cw[pass] = *reinterpret_cast<const uint4*>(codes_lo + (size_t)ec * 16u);
+                    }
                 }
                 if (!LISTORDER) {
 #pragma unroll
