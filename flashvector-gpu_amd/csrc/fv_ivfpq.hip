@@ -19,12 +19,13 @@
 //      with the reference's id-indexed codes it gathers them by vector id.
 //      ADC = sequential FP32 adds over m = 0..M-1. An entry is kept if its
 //      (adc, id) key is below the running threshold key; keys go to a
-//      candidate buffer (LDS, or global memory for very deep shortlists) that
-//      the workgroup sorts and truncates to S when it nears capacity.
+//      candidate buffer (LDS, or global memory for very deep shortlists)
+//      that is cut to the S best by radix selection whenever the next keys
+//      would not fit.
 //    finally the S best are reranked with the exact FP32 distance (the
-//    reference CPU path's order), sorted by (distance, id) and the first k
-//    are written.
-// No MFMA: the scan is bandwidth- and LDS-lookup-bound.
+//    reference CPU path's order), cut to the k best and written in
+//    (distance, id) order.
+// No MFMA: the scan is bound by its LUT lookups in LDS and by list bytes.
 #include "fv_ivfpq.hpp"
 
 namespace fv {
