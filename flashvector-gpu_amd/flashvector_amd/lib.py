@@ -156,6 +156,11 @@ SIGNATURES = {
     "fvh_stats": (I, [P, P, P]),
     "fv_topk_ip_backward": (I, [P, U, P, U, U, P, U, P, P, ctypes.c_float, ctypes.c_float, P, P,
                                 P, P, P]),
+    "fv_kmeans_assign": (I, [P, U, U, U, P, U, P, P, U, P]),
+    "fv_kmeans_workspace_size": (SZ, [U, U, U]),
+    "fv_kmeans_fit": (I, [P, U, U, U, U, U, ctypes.c_uint64, P, P, P, P, P, P, P, SZ, P]),
+    "fv_ivf_pq_build_workspace_size": (SZ, [U, U, U, U]),
+    "fv_ivf_pq_build": (I, [P, U, U, U, U, U, U, ctypes.c_uint64, P, P, P, P, P, P, P, SZ, P]),
     "fv_kernel_timer_enable": (I, [I]),
     "fv_kernel_timer_read": (I, [P, P, I]),
 }

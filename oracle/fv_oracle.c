@@ -5,6 +5,7 @@
  */
 #include "fv_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -567,5 +568,231 @@ int fvo_flat_topk_f64(const float* vectors, uint32_t n, uint32_t dim, const floa
         }
         free(all);
     }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* IVF-PQ index construction                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* splitmix64 (Steele, Lea, Flood 2014; Vigna's public-domain constants). */
+static uint64_t sm64_next(uint64_t* s) {
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static uint32_t sm64_below(uint64_t* s, uint32_t m) { return (uint32_t)(sm64_next(s) % m); }
+static double sm64_unit(uint64_t* s) { return (double)(sm64_next(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+/* KMeans::l2_sq (cluster.rs:18-26): sequential, non-fused. */
+static float km_l2(const float* a, const float* b, uint32_t dim) {
+    float acc = 0.0f;
+    for (uint32_t d = 0; d < dim; ++d) {
+        float t = a[d] - b[d];
+        float sq = t * t;
+        acc = acc + sq;
+    }
+    return acc;
+}
+
+/* Assign step (cluster.rs:97-123): first strict minimum, best 0 if none. */
+void fvo_kmeans_assign(const float* x, uint32_t n, uint32_t dim, uint32_t ld,
+                       const float* centroids, uint32_t k, uint32_t* assign) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const float* v = x + (size_t)i * ld;
+        float best = INFINITY;
+        uint32_t bc = 0;
+        for (uint32_t c = 0; c < k; ++c) {
+            float d = km_l2(v, centroids + (size_t)c * dim, dim);
+            if (d < best) {
+                best = d;
+                bc = c;
+            }
+        }
+        assign[i] = bc;
+    }
+}
+
+#define KM_SEED_CHUNK 256u
+
+/* k-means++ over a bounded sample (cluster.rs:46-93), deterministic. */
+static void km_seed(const float* x, uint32_t n, uint32_t dim, uint32_t ld, uint32_t k,
+                    uint64_t* rng, float* cent) {
+    uint32_t pool = k * 64u < n ? k * 64u : n;
+    if (pool == 0) pool = 1;
+    uint32_t* idx = (uint32_t*)malloc((size_t)n * sizeof(uint32_t));
+    for (uint32_t i = 0; i < n; ++i) idx[i] = i;
+    for (uint32_t i = 0; i < pool; ++i) {
+        uint32_t j = i + sm64_below(rng, n - i);
+        uint32_t t = idx[i]; idx[i] = idx[j]; idx[j] = t;
+    }
+    double* u = (double*)malloc((size_t)(k > 1 ? k : 1) * sizeof(double));
+    for (uint32_t c = 1; c < k; ++c) u[c] = sm64_unit(rng);
+    float* closest = (float*)malloc((size_t)pool * sizeof(float));
+    uint32_t nch = (pool + KM_SEED_CHUNK - 1) / KM_SEED_CHUNK;
+    double* cs = (double*)malloc((size_t)nch * sizeof(double));
+    for (uint32_t i = 0; i < pool; ++i) closest[i] = INFINITY;
+    memcpy(cent, x + (size_t)idx[0] * ld, dim * sizeof(float));
+    for (uint32_t c = 1; c < k; ++c) {
+        const float* last = cent + (size_t)(c - 1) * dim;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < (int64_t)pool; ++i) {
+            float cand = km_l2(x + (size_t)idx[i] * ld, last, dim);
+            if (cand < closest[i]) closest[i] = cand;
+        }
+        double total = 0.0;
+        for (uint32_t j = 0; j < nch; ++j) {
+            double s = 0.0;
+            uint32_t e = (j + 1) * KM_SEED_CHUNK < pool ? (j + 1) * KM_SEED_CHUNK : pool;
+            for (uint32_t i = j * KM_SEED_CHUNK; i < e; ++i) s = s + (double)closest[i];
+            cs[j] = s;
+            total = total + s;
+        }
+        uint32_t chosen = idx[pool - 1];
+        if (total > 0.0) {
+            double target = u[c] * total, run = 0.0;
+            for (uint32_t j = 0; j < nch; ++j) {
+                uint32_t e = (j + 1) * KM_SEED_CHUNK < pool ? (j + 1) * KM_SEED_CHUNK : pool;
+                if (run + cs[j] > target) {
+                    double r = run;
+                    chosen = idx[e - 1];
+                    for (uint32_t i = j * KM_SEED_CHUNK; i < e; ++i) {
+                        r = r + (double)closest[i];
+                        if (r > target) {
+                            chosen = idx[i];
+                            break;
+                        }
+                    }
+                    break;
+                }
+                run = run + cs[j];
+            }
+        }
+        memcpy(cent + (size_t)c * dim, x + (size_t)chosen * ld, dim * sizeof(float));
+    }
+    free(cs);
+    free(closest);
+    free(u);
+    free(idx);
+}
+
+void fvo_kmeans_fit(const float* x, uint32_t n, uint32_t dim, uint32_t ld, uint32_t k,
+                    uint32_t max_iters, uint64_t seed, float* cent, uint32_t* assign,
+                    uint32_t* sizes, uint32_t* iters_run) {
+    uint64_t rng = seed;
+    uint32_t it_done = 0;
+    km_seed(x, n, dim, ld, k, &rng, cent);
+    memset(assign, 0, (size_t)n * sizeof(uint32_t));
+    memset(sizes, 0, (size_t)k * sizeof(uint32_t));
+    float* mean = NULL;
+    float* ncent = (float*)malloc((size_t)k * dim * sizeof(float));
+    for (uint32_t it = 0; it < max_iters; ++it) {
+        fvo_kmeans_assign(x, n, dim, ld, cent, k, assign);
+        /* Update step (cluster.rs:125-171): sums in vector-index order. */
+        memset(ncent, 0, (size_t)k * dim * sizeof(float));
+        memset(sizes, 0, (size_t)k * sizeof(uint32_t));
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t c = assign[i];
+            sizes[c]++;
+            const float* v = x + (size_t)i * ld;
+            float* s = ncent + (size_t)c * dim;
+            for (uint32_t d = 0; d < dim; ++d) s[d] = s[d] + v[d];
+        }
+        float max_delta = 0.0f;
+        for (uint32_t c = 0; c < k; ++c) {
+            float* s = ncent + (size_t)c * dim;
+            if (sizes[c] > 0) {
+                float inv = 1.0f / (float)sizes[c];
+                const float* old = cent + (size_t)c * dim;
+                float delta = 0.0f;
+                for (uint32_t d = 0; d < dim; ++d) {
+                    s[d] = s[d] * inv;
+                    float diff = s[d] - old[d];
+                    float sq = diff * diff;
+                    delta = delta + sq;
+                }
+                if (delta > max_delta) max_delta = delta;
+            } else {
+                /* Empty-cluster recovery (cluster.rs:151-168): the row furthest
+                 * from the data mean among min(n, 500) drawn rows. */
+                if (!mean) {
+                    mean = (float*)calloc(dim, sizeof(float));
+                    for (uint32_t i = 0; i < n; ++i) {
+                        const float* v = x + (size_t)i * ld;
+                        for (uint32_t d = 0; d < dim; ++d) mean[d] = mean[d] + v[d];
+                    }
+                    float inv_n = 1.0f / (float)n;
+                    for (uint32_t d = 0; d < dim; ++d) mean[d] = mean[d] * inv_n;
+                }
+                uint32_t far = sm64_below(&rng, n);
+                float max_d = 0.0f;
+                uint32_t draws = n < 500u ? n : 500u;
+                for (uint32_t t = 0; t < draws; ++t) {
+                    uint32_t si = sm64_below(&rng, n);
+                    float d = km_l2(x + (size_t)si * ld, mean, dim);
+                    if (d > max_d) {
+                        max_d = d;
+                        far = si;
+                    }
+                }
+                memcpy(s, x + (size_t)far * ld, dim * sizeof(float));
+            }
+        }
+        memcpy(cent, ncent, (size_t)k * dim * sizeof(float));
+        it_done = it + 1;
+        if (max_delta < 1e-4f) break;
+    }
+    if (iters_run) *iters_run = it_done;
+    free(ncent);
+    free(mean);
+}
+
+/* GpuVectorIndex::build, IVF-PQ part (index.rs:405-458): coarse k-means over
+ * all rows, residuals against the assigned centroid, one k-means per
+ * sub-space on the residuals (ProductQuantizer::train, cluster.rs:197-244;
+ * unused codewords FLT_MAX), encode (cluster.rs:246-286) and posting lists
+ * ascending by vector id. Sub-quantizer m draws from seed + (m+1)*golden. */
+int fvo_ivf_pq_build(const float* x, uint32_t n, uint32_t dim, uint32_t nlist, uint32_t m_pq,
+                     uint32_t coarse_iters, uint32_t pq_iters, uint64_t seed, float* centroids,
+                     float* codebooks, uint8_t* codes, uint32_t* offsets, uint32_t* vec_ids,
+                     uint32_t* iters_run) {
+    if (n == 0 || dim == 0 || m_pq == 0 || dim % m_pq) return -1;
+    if (nlist > n) nlist = n;
+    if (nlist == 0) nlist = 1;
+    const uint32_t sub = dim / m_pq;
+    uint32_t* assign = (uint32_t*)malloc((size_t)n * sizeof(uint32_t));
+    uint32_t* sizes = (uint32_t*)malloc((size_t)(nlist > 256 ? nlist : 256) * sizeof(uint32_t));
+    fvo_kmeans_fit(x, n, dim, dim, nlist, coarse_iters, seed, centroids, assign, sizes,
+                   iters_run ? &iters_run[0] : NULL);
+    float* res = (float*)malloc((size_t)n * dim * sizeof(float));
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        const float* c = centroids + (size_t)assign[i] * dim;
+        for (uint32_t d = 0; d < dim; ++d) res[(size_t)i * dim + d] = x[(size_t)i * dim + d] - c[d];
+    }
+    const uint32_t kpq = n < 256u ? n : 256u;
+    uint32_t* sub_assign = (uint32_t*)malloc((size_t)n * sizeof(uint32_t));
+    for (uint32_t m = 0; m < m_pq; ++m) {
+        float* cb = codebooks + (size_t)m * 256 * sub;
+        fvo_kmeans_fit(res + (size_t)m * sub, n, sub, dim, kpq, pq_iters,
+                       seed + (uint64_t)(m + 1) * 0x9E3779B97F4A7C15ull, cb, sub_assign, sizes,
+                       iters_run ? &iters_run[1 + m] : NULL);
+        for (uint32_t c = kpq; c < 256; ++c)
+            for (uint32_t d = 0; d < sub; ++d) cb[(size_t)c * sub + d] = FLT_MAX;
+        fvo_kmeans_assign(res + (size_t)m * sub, n, sub, dim, cb, 256, sub_assign);
+        for (uint32_t i = 0; i < n; ++i) codes[(size_t)i * m_pq + m] = (uint8_t)sub_assign[i];
+    }
+    memset(offsets, 0, (size_t)(nlist + 1) * sizeof(uint32_t));
+    for (uint32_t i = 0; i < n; ++i) offsets[assign[i] + 1]++;
+    for (uint32_t c = 0; c < nlist; ++c) offsets[c + 1] += offsets[c];
+    uint32_t* fill = (uint32_t*)calloc(nlist, sizeof(uint32_t));
+    for (uint32_t i = 0; i < n; ++i) vec_ids[offsets[assign[i]] + fill[assign[i]]++] = i;
+    free(fill);
+    free(sub_assign);
+    free(res);
+    free(sizes);
+    free(assign);
     return 0;
 }

@@ -144,6 +144,32 @@ int fvo_flat_topk_f64(const float* vectors, uint32_t n, uint32_t dim, const floa
                       uint32_t nq, uint32_t top_k, int metric, const uint32_t* valid_mask,
                       uint32_t* out_ids, double* out_dist, int nthreads);
 
+/* ---- IVF-PQ index construction -------------------------------------------
+ * Deterministic restatement of KMeans::fit (crates/engine/src/cluster.rs:28-182),
+ * ProductQuantizer::train/encode (cluster.rs:197-287) and the IVF-PQ part of
+ * GpuVectorIndex::build (crates/engine/src/index.rs:405-458). The reference
+ * draws from an unseeded thread RNG; here every draw comes from splitmix64 in
+ * a fixed order (pool shuffle of min(n, 64k) candidates, k-1 seeding targets,
+ * then empty-cluster recovery in ascending cluster order) and the k-means++
+ * target is found by f64 sums over 256-candidate chunks (first candidate
+ * whose running sum exceeds u * total). FP32 steps follow the reference:
+ * l2 = sequential non-fused sum of (x - c)^2; assignment = first strict
+ * minimum (best starts at +inf, cluster 0 if none); centroid = sum in
+ * vector-index order times (1.0f / count); stop when the largest squared move
+ * is below 1e-4f. Rows are x + i*ld (ld >= dim). Parity pins: closed forms
+ * (integer data with power-of-two cluster sizes, separated blobs, codeword
+ * identity encode) and brute force in tests/test_oracle.py; the seeding
+ * search order is pinned only through invariants (parity partly unpinned). */
+void fvo_kmeans_assign(const float* x, uint32_t n, uint32_t dim, uint32_t ld,
+                       const float* centroids, uint32_t k, uint32_t* assign);
+void fvo_kmeans_fit(const float* x, uint32_t n, uint32_t dim, uint32_t ld, uint32_t k,
+                    uint32_t max_iters, uint64_t seed, float* centroids, uint32_t* assign,
+                    uint32_t* sizes, uint32_t* iters_run);
+int fvo_ivf_pq_build(const float* x, uint32_t n, uint32_t dim, uint32_t nlist, uint32_t m_pq,
+                     uint32_t coarse_iters, uint32_t pq_iters, uint64_t seed, float* centroids,
+                     float* codebooks, uint8_t* codes, uint32_t* offsets, uint32_t* vec_ids,
+                     uint32_t* iters_run);
+
 #ifdef __cplusplus
 }
 #endif

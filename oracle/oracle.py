@@ -82,6 +82,9 @@ def lib():
         L.fvo_e4m3_to_f32.restype = ctypes.c_float
         L.fvo_encode_fp8_rows.argtypes = [P, U, U, U, ctypes.c_float, P, P]
         L.fvo_flat_topk_f64.argtypes = [P, U, U, P, U, U, I, P, P, P, I]
+        L.fvo_kmeans_assign.argtypes = [P, U, U, U, P, U, P]
+        L.fvo_kmeans_fit.argtypes = [P, U, U, U, U, U, ctypes.c_uint64, P, P, P, P]
+        L.fvo_ivf_pq_build.argtypes = [P, U, U, U, U, U, U, ctypes.c_uint64, P, P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -294,3 +297,46 @@ def flat_topk_f64(vectors, queries, top_k, metric="l2", valid_mask=None, nthread
                                  _p(ids), _p(dist), nthreads)
     assert rc == 0
     return ids, dist
+
+
+def kmeans_assign(x, centroids, dim=None, ld=None):
+    """Nearest centroid of every row (first strict minimum). x is [n, ld];
+    the first `dim` columns are the row."""
+    x, c = _f32(x), _f32(centroids)
+    n, ld_ = x.shape
+    dim = dim or c.shape[1]
+    out = np.empty(n, np.uint32)
+    lib().fvo_kmeans_assign(_p(x), n, dim, ld or ld_, _p(c), c.shape[0], _p(out))
+    return out
+
+
+def kmeans_fit(x, k, max_iters, seed, dim=None):
+    """(centroids [k, dim], assignments [n], sizes [k], iterations run)."""
+    x = _f32(x)
+    n, ld = x.shape
+    dim = dim or ld
+    cent = np.empty((k, dim), np.float32)
+    assign = np.empty(n, np.uint32)
+    sizes = np.empty(k, np.uint32)
+    it = np.zeros(1, np.uint32)
+    lib().fvo_kmeans_fit(_p(x), n, dim, ld, k, max_iters, seed, _p(cent), _p(assign),
+                         _p(sizes), _p(it))
+    return cent, assign, sizes, int(it[0])
+
+
+def ivf_pq_build(x, nlist, m_pq, coarse_iters, pq_iters, seed):
+    """dict of centroids, codebooks [m, 256*sub], codes [n, m], offsets, vec_ids, iters."""
+    x = _f32(x)
+    n, dim = x.shape
+    nl = max(1, min(nlist, n))
+    sub = dim // m_pq
+    out = dict(centroids=np.empty((nl, dim), np.float32),
+               codebooks=np.empty((m_pq, 256 * sub), np.float32),
+               codes=np.empty((n, m_pq), np.uint8), offsets=np.empty(nl + 1, np.uint32),
+               vec_ids=np.empty(n, np.uint32), iters=np.zeros(1 + m_pq, np.uint32))
+    rc = lib().fvo_ivf_pq_build(_p(x), n, dim, nlist, m_pq, coarse_iters, pq_iters, seed,
+                                _p(out["centroids"]), _p(out["codebooks"]), _p(out["codes"]),
+                                _p(out["offsets"]), _p(out["vec_ids"]), _p(out["iters"]))
+    if rc != 0:
+        raise ValueError(f"fvo_ivf_pq_build: {rc}")
+    return out

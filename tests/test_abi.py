@@ -11,7 +11,7 @@ from flashvector_amd import lib as L
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADERS = [os.path.join(ROOT, "include", "flashvector", h)
-           for h in ("cuda_bridge.h", "fv_ext.h", "fv_host.h")]
+           for h in ("cuda_bridge.h", "fv_ext.h", "fv_host.h", "fv_build.h")]
 
 
 def declared_symbols():
@@ -62,6 +62,18 @@ def test_null_pointers_rejected(so):
 def test_workspace_sizes_are_host_only(so):
     assert so.fv_flat_exact_workspace_size(1000, 128, 10, 10, 0) > 0
     assert so.fv_flat_exact_workspace_size(1000, 128, 10, 2000, 0) == 0   # k above the limit
+    assert so.fv_kmeans_workspace_size(10000, 128, 64) > 10000 * 16
+    assert so.fv_kmeans_workspace_size(10, 128, 64) == 0                   # k above n
+    assert so.fv_ivf_pq_build_workspace_size(10000, 128, 64, 16) > 10000 * 128 * 4
+    assert so.fv_ivf_pq_build_workspace_size(10000, 130, 64, 16) == 0      # dim % m_pq
+
+
+def test_index_build_rejects_null(so):
+    assert so.fv_kmeans_assign(None, 1, 1, 1, None, 1, None, None, 0, None) == -1
+    assert so.fv_kmeans_fit(None, 1, 1, 1, 1, 1, 0, None, None, None, None, None, None, None,
+                            0, None) == -1
+    assert so.fv_ivf_pq_build(None, 1, 1, 1, 1, 1, 1, 0, None, None, None, None, None, None,
+                              None, 0, None) == -1
 
 
 def test_host_engine_rejects_null_and_bad_config(so):
