@@ -75,8 +75,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--alt-valid-frac", type=float, default=0.95,
                     help="second mask density reported beside the headline (0 = skip)")
-    ap.add_argument("--workload", choices=["flat", "ivfpq", "fp8", "backward"], default="flat",
-                    help="flat: the headline matrix-core scan; ivfpq: 10M-row IVF-PQ uneven batch")
+    ap.add_argument("--workload", choices=["flat", "ivfpq", "fp8", "backward", "build"],
+                    default="flat",
+                    help="flat: the headline matrix-core scan; ivfpq: 10M-row IVF-PQ uneven batch; "
+                         "build: IVF-PQ table construction on the GPU")
+    ap.add_argument("--cpu-rows", type=int, default=50_000,
+                    help="build workload: rows of the CPU oracle's timed sample")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the built-in plan instead of the autotuned one")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -95,6 +99,8 @@ def main():
         return bench_fp8(args)
     if args.workload == "backward":
         return bench_backward(args)
+    if args.workload == "build":
+        return bench_build(args)
     args.n = args.n or 1_000_000
     args.nq = args.nq or 10_000
 
@@ -377,6 +383,70 @@ def bench_ivfpq(args):
                      "traffic": measured_traffic("ivf_scan_kernel/ivfpq_uneven_10m"),
                      "kernel": "ivf_scan_kernel", "kernel_avg_ms": round(scan_avg, 4),
                      "algorithmic_bytes_per_launch": nbytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+
+
+def bench_build(args):
+    """IVF-PQ table construction on the GPU: N=1M x 128 Gaussian-mixture rows
+    (integers 0-255), nlist 256, M=16, 25 coarse and 25 PQ Lloyd iterations at
+    most (the reference's SIFT1M build settings, README.md:188, index.rs:414,428),
+    seed 1. One build is the step: k-means++ seeding, Lloyd iterations,
+    residuals, 16 sub-space k-means, encoding and posting lists."""
+    import numpy as np
+    import torch
+
+    from flashvector_amd import gpu_build as G
+    from flashvector_amd import lib as L
+    from flashvector_amd import synthetic as S
+
+    torch.cuda.set_device(0)
+    n, dim, nlist, m_pq, iters = args.n or 1_000_000, 128, 256, 16, 25
+    log(f"generating {n} x {dim} rows")
+    db, _ = S.flat_workload(n, 1, dim, 256, seeds=S.SEEDS["flat_mixed_1m"])
+    x = torch.from_numpy(db).cuda()
+    G.build_ivfpq(x[:20000].contiguous(), 64, m_pq, 2, 2, seed=1)   # loads the kernels
+    torch.cuda.synchronize()
+    log("timed build")
+    with L.KernelTimer() as kt:
+        t0 = time.perf_counter()
+        t = G.build_ivfpq(x, nlist, m_pq, iters, iters, seed=1)
+        torch.cuda.synchronize()
+        build_s = time.perf_counter() - t0
+        assign_ms, launches = kt.read()
+    sizes = np.diff(t.offsets.cpu().numpy().astype(np.int64))
+    assign_avg = assign_ms / max(launches, 1)
+    flops = 3.0 * n * nlist * dim          # sub, mul, add per (row, centroid, dim)
+    achieved = flops / (assign_avg / 1e3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        O.build()
+        ns = min(args.cpu_rows, n)
+        cores = len(os.sched_getaffinity(0))
+        log(f"CPU oracle build on the first {ns} rows")
+        t0 = time.perf_counter()
+        o = O.ivf_pq_build(db[:ns], nlist, m_pq, iters, iters, 1)
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": round(ns / cpu_s, 1), "unit": "rows/s", "cores": cores, "kind": "port",
+               "sample": f"first {ns} of the {n} rows, same nlist/M/iterations/seed "
+                         f"(oracle/fv_oracle.c, OpenMP over rows in the assign steps): "
+                         f"{cpu_s:.2f} s, iterations {o['iters'][0]} coarse"}
+    res = {
+        "metric": "IVF-PQ table build time (1M x 128, nlist 256, M=16, 25+25 iterations)",
+        "value": round(build_s, 3), "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(build_s * 1e3, 1), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded Gaussian mixture)",
+        "config": {"workload": "ivfpq_build_1m", "n": n, "dim": dim, "nlist": nlist, "m_pq": m_pq,
+                   "coarse_iters_max": iters, "pq_iters_max": iters, "seed": 1},
+        "rows_per_s": round(n / build_s, 1), "iterations": t.iterations,
+        "list_sizes": {"min": int(sizes.min()), "max": int(sizes.max())},
+        "roofline": {"bound": "fp32-valu", "achieved": round(achieved, 2),
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel": "km_assign_kernel (coarse)", "kernel_avg_ms": round(assign_avg, 4),
+                     "launches": launches, "algorithmic_flops_per_launch": flops},
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)

@@ -21,6 +21,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cfloat>
+#include <thread>
 #include <vector>
 
 #include "../../include/flashvector/fv_build.h"
@@ -148,6 +149,10 @@ __global__ void km_offsets_kernel(const uint32_t* __restrict__ skeys, uint32_t n
 }
 
 // Sum of each (cluster, dim) over the cluster's rows in row order, from 0.
+// The adds form one dependent chain per (cluster, dim); kChainLoads row
+// loads are issued ahead of their adds so the chain waits on memory once per
+// group rather than once per row.
+constexpr int kChainLoads = 32;
 __global__ __launch_bounds__(64) void km_update_kernel(
     const float* __restrict__ X, uint32_t dim, uint32_t ld, const uint32_t* __restrict__ members,
     const uint32_t* __restrict__ offsets, float* __restrict__ sums) {
@@ -157,12 +162,12 @@ __global__ __launch_bounds__(64) void km_update_kernel(
     const uint32_t beg = offsets[c], end = offsets[c + 1];
     float acc = 0.0f;
     uint32_t j = beg;
-    for (; j + 8u <= end; j += 8u) {
-        float v[8];
+    for (; j + kChainLoads <= end; j += kChainLoads) {
+        float v[kChainLoads];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = X[(size_t)members[j + u] * ld + d];
+        for (int u = 0; u < kChainLoads; ++u) v[u] = X[(size_t)members[j + u] * ld + d];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc = acc + v[u];
+        for (int u = 0; u < kChainLoads; ++u) acc = acc + v[u];
     }
     for (; j < end; ++j) acc = acc + X[(size_t)members[j] * ld + d];
     sums[(size_t)c * dim + d] = acc;
@@ -230,40 +235,86 @@ __global__ __launch_bounds__(256) void km_seed_update_kernel(
 // k-means++ step, part 2 (one workgroup): total = chunk sums in order; the
 // chosen row is the first whose running sum exceeds u * total (the pool's last
 // row when total is 0 or rounding leaves none); its row becomes centroid c.
-__global__ __launch_bounds__(64) void km_seed_pick_kernel(
+// The sums run in one thread over values the workgroup staged in LDS.
+constexpr uint32_t kPickTile = 1024;
+__global__ __launch_bounds__(256) void km_seed_pick_kernel(
     const double* __restrict__ chunk_sum, uint32_t nch, const float* __restrict__ closest,
     uint32_t pool, const uint32_t* __restrict__ pool_ix, const double* __restrict__ u,
     const float* __restrict__ X, uint32_t dim, uint32_t ld, float* __restrict__ dst) {
+    __shared__ double s_cs[kPickTile];
+    __shared__ float s_cl[kSeedChunk];
+    __shared__ double s_run;
+    __shared__ int s_hit;  // chunk holding the target, -1 while none
     __shared__ uint32_t s_row;
-    if (threadIdx.x == 0) {
-        double total = 0.0;
-        for (uint32_t j = 0; j < nch; ++j) total = total + chunk_sum[j];
-        uint32_t chosen = pool_ix[pool - 1];
-        if (total > 0.0) {
-            const double target = (*u) * total;
-            double run = 0.0;
-            for (uint32_t j = 0; j < nch; ++j) {
-                const uint32_t e = min((j + 1) * kSeedChunk, pool);
-                if (run + chunk_sum[j] > target) {
-                    double r = run;
-                    chosen = pool_ix[e - 1];
-                    for (uint32_t i = j * kSeedChunk; i < e; ++i) {
-                        r = r + (double)closest[i];
-                        if (r > target) {
-                            chosen = pool_ix[i];
-                            break;
-                        }
-                    }
-                    break;
-                }
-                run = run + chunk_sum[j];
-            }
+    const uint32_t tid = threadIdx.x;
+    // Pass 1: total over all chunk sums, tile by tile in order.
+    if (tid == 0) s_run = 0.0;
+    for (uint32_t t0 = 0; t0 < nch; t0 += kPickTile) {
+        const uint32_t m = min(kPickTile, nch - t0);
+        __syncthreads();
+        for (uint32_t j = tid; j < m; j += blockDim.x) s_cs[j] = chunk_sum[t0 + j];
+        __syncthreads();
+        if (tid == 0) {
+            double run = s_run;
+            for (uint32_t j = 0; j < m; ++j) run = run + s_cs[j];
+            s_run = run;
         }
-        s_row = chosen;
     }
     __syncthreads();
-    const uint32_t r = s_row;
-    for (uint32_t d = threadIdx.x; d < dim; d += blockDim.x) dst[d] = X[(size_t)r * ld + d];
+    const double total = s_run;
+    uint32_t chosen = pool_ix[pool - 1];
+    __syncthreads();  // every thread has read s_run before it is reset
+    if (total > 0.0) {
+        const double target = (*u) * total;
+        // Pass 2: the first chunk whose running sum exceeds the target.
+        if (tid == 0) {
+            s_run = 0.0;
+            s_hit = -1;
+        }
+        double before = 0.0;
+        for (uint32_t t0 = 0; t0 < nch; t0 += kPickTile) {
+            const uint32_t m = min(kPickTile, nch - t0);
+            __syncthreads();
+            if (s_hit >= 0) break;
+            for (uint32_t j = tid; j < m; j += blockDim.x) s_cs[j] = chunk_sum[t0 + j];
+            __syncthreads();
+            if (tid == 0) {
+                double run = s_run;
+                for (uint32_t j = 0; j < m; ++j) {
+                    if (run + s_cs[j] > target) {
+                        s_hit = (int)(t0 + j);
+                        break;
+                    }
+                    run = run + s_cs[j];
+                }
+                s_run = run;
+            }
+        }
+        __syncthreads();
+        const int hit = s_hit;
+        before = s_run;
+        if (hit >= 0) {
+            // Pass 3: the first row of that chunk whose running sum exceeds it.
+            const uint32_t b = (uint32_t)hit * kSeedChunk, e = min(b + kSeedChunk, pool);
+            for (uint32_t i = b + tid; i < e; i += blockDim.x) s_cl[i - b] = closest[i];
+            __syncthreads();
+            if (tid == 0) {
+                uint32_t pick = pool_ix[e - 1];
+                double r = before;
+                for (uint32_t i = b; i < e; ++i) {
+                    r = r + (double)s_cl[i - b];
+                    if (r > target) {
+                        pick = pool_ix[i];
+                        break;
+                    }
+                }
+                s_row = pick;
+            }
+            __syncthreads();
+            chosen = s_row;
+        }
+    }
+    for (uint32_t d = tid; d < dim; d += blockDim.x) dst[d] = X[(size_t)chosen * ld + d];
 }
 
 // Mean of all rows per dim: sequential over rows, then times 1.0f / n.
@@ -273,12 +324,12 @@ __global__ void km_mean_kernel(const float* __restrict__ X, uint32_t n, uint32_t
     if (d >= dim) return;
     float acc = 0.0f;
     uint32_t i = 0;
-    for (; i + 8u <= n; i += 8u) {
-        float v[8];
+    for (; i + kChainLoads <= n; i += kChainLoads) {
+        float v[kChainLoads];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = X[(size_t)(i + u) * ld + d];
+        for (int u = 0; u < kChainLoads; ++u) v[u] = X[(size_t)(i + u) * ld + d];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc = acc + v[u];
+        for (int u = 0; u < kChainLoads; ++u) acc = acc + v[u];
     }
     for (; i < n; ++i) acc = acc + X[(size_t)i * ld + d];
     mean[d] = acc * (1.0f / (float)n);
@@ -416,10 +467,12 @@ hipError_t group_by_cluster(const uint32_t* assign, uint32_t n, uint32_t k, char
     return hipGetLastError();
 }
 
+// `timed`: bracket the assign launches with the kernel timer (the dominant
+// kernel of a coarse k-means).
 hipError_t kmeans_fit(const float* X, uint32_t n, uint32_t dim, uint32_t ld, uint32_t k,
                       uint32_t max_iters, uint64_t seed, float* cent, uint32_t* assign,
                       uint32_t* sizes, uint32_t* offsets_out, uint32_t* members_out,
-                      uint32_t* iters_run, char* ws, hipStream_t s) {
+                      uint32_t* iters_run, char* ws, hipStream_t s, bool timed) {
     const KmLayout L = km_layout(n, dim, k);
     Rng rng{seed};
     // ---- k-means++ seeding over the pool (cluster.rs:46-93) ----
@@ -445,7 +498,7 @@ hipError_t kmeans_fit(const float* X, uint32_t n, uint32_t dim, uint32_t ld, uin
         km_seed_update_kernel<<<nch, kSeedChunk, 0, s>>>(X, dim, ld, pool_ix, pool,
                                                          cent + (size_t)(c - 1) * dim, closest,
                                                          chunk_sum);
-        km_seed_pick_kernel<<<1, 64, 0, s>>>(chunk_sum, nch, closest, pool, pool_ix, d_u + c, X,
+        km_seed_pick_kernel<<<1, 256, 0, s>>>(chunk_sum, nch, closest, pool, pool_ix, d_u + c, X,
                                              dim, ld, cent + (size_t)c * dim);
     }
     FV_TRY(hipGetLastError());
@@ -463,7 +516,9 @@ hipError_t kmeans_fit(const float* X, uint32_t n, uint32_t dim, uint32_t ld, uin
     bool have_mean = false;
     uint32_t done = 0;
     for (uint32_t it = 0; it < max_iters; ++it) {
+        void* timer = timed ? timer_begin(s) : nullptr;
         FV_TRY(assign_launch(X, n, dim, ld, cent, k, assign, nullptr, 0, s));
+        if (timer) timer_end(timer, s);
         FV_TRY(group_by_cluster(assign, n, k, ws, L, members, offsets, s));
         km_update_kernel<<<dim3(k, (dim + 63) / 64), 64, 0, s>>>(X, dim, ld, members, offsets,
                                                                  sums);
@@ -510,23 +565,33 @@ hipError_t kmeans_fit(const float* X, uint32_t n, uint32_t dim, uint32_t ld, uin
     return hipSuccess;
 }
 
+// The sub-space k-means runs are independent: up to kPqWorkers run at once,
+// each from its own host thread on its own stream and workspace slot.
+constexpr uint32_t kPqWorkers = 4;
+
 struct BuildLayout {
-    size_t km, residual, assign, sub_assign, sizes, total;
+    size_t km, residual, assign, sizes, pq_slot, pq_slot_bytes, pq_km, pq_assign, pq_sizes, total;
+    uint32_t workers;
 };
 
 BuildLayout build_layout(uint32_t n, uint32_t dim, uint32_t nlist, uint32_t m_pq) {
     BuildLayout B{};
     const uint32_t nl = nlist > n ? n : (nlist ? nlist : 1u);
     const uint32_t kpq = n < 256u ? n : 256u;
-    const KmLayout a = km_layout(n, dim, nl);
-    const KmLayout b = km_layout(n, dim / m_pq, kpq);
     size_t off = 0;
     auto take = [&](size_t bytes) { const size_t o = off; off += align256(bytes); return o; };
-    B.km = take(a.total > b.total ? a.total : b.total);
+    B.km = take(km_layout(n, dim, nl).total);
     B.residual = take((size_t)n * dim * 4);
     B.assign = take((size_t)n * 4);
-    B.sub_assign = take((size_t)n * 4);
-    B.sizes = take((size_t)(nl > 256u ? nl : 256u) * 4);
+    B.sizes = take((size_t)nl * 4);
+    B.workers = m_pq < kPqWorkers ? m_pq : kPqWorkers;
+    size_t so = 0;
+    auto slot = [&](size_t bytes) { const size_t o = so; so += align256(bytes); return o; };
+    B.pq_km = slot(km_layout(n, dim / m_pq, kpq).total);
+    B.pq_assign = slot((size_t)n * 4);
+    B.pq_sizes = slot(256 * 4);
+    B.pq_slot_bytes = so;
+    B.pq_slot = take(so * B.workers);
     B.total = off;
     return B;
 }
@@ -562,7 +627,7 @@ int fv_kmeans_fit(const float* d_x, uint32_t n, uint32_t dim, uint32_t ld, uint3
     if (workspace_bytes < km_layout(n, dim, k).total) return (int)hipErrorInvalidValue;
     return (int)kmeans_fit(d_x, n, dim, ld, k, max_iters, seed, d_centroids, d_assign, d_sizes,
                            d_offsets, d_members, iters_run, static_cast<char*>(d_workspace),
-                           (hipStream_t)stream);
+                           (hipStream_t)stream, true);
 }
 
 size_t fv_ivf_pq_build_workspace_size(uint32_t n, uint32_t dim, uint32_t nlist, uint32_t m_pq) {
@@ -587,30 +652,55 @@ int fv_ivf_pq_build(const float* d_vectors, uint32_t n, uint32_t dim, uint32_t n
     const uint32_t sub = dim / m_pq;
     const uint32_t kpq = n < 256u ? n : 256u;
     uint32_t* assign = reinterpret_cast<uint32_t*>(ws + B.assign);
-    uint32_t* sub_assign = reinterpret_cast<uint32_t*>(ws + B.sub_assign);
     uint32_t* sizes = reinterpret_cast<uint32_t*>(ws + B.sizes);
     float* res = reinterpret_cast<float*>(ws + B.residual);
     hipError_t e = kmeans_fit(d_vectors, n, dim, dim, nl, coarse_iters, seed, d_centroids, assign,
-                              sizes, d_ivf_offsets, d_ivf_vec_ids, iters_run, ws + B.km, s);
+                              sizes, d_ivf_offsets, d_ivf_vec_ids, iters_run, ws + B.km, s, true);
     if (e != hipSuccess) return (int)e;
     const size_t total = (size_t)n * dim;
     km_residual_kernel<<<blocks_for(total, 256), 256, 0, s>>>(d_vectors, d_centroids, assign, n,
                                                               dim, res);
-    for (uint32_t m = 0; m < m_pq; ++m) {
-        float* cb = d_codebooks + (size_t)m * 256 * sub;
-        e = kmeans_fit(res + (size_t)m * sub, n, sub, dim, kpq, pq_iters,
-                       seed + (uint64_t)(m + 1) * 0x9E3779B97F4A7C15ull, cb, sub_assign, sizes,
-                       nullptr, nullptr, iters_run ? iters_run + 1 + m : nullptr, ws + B.km, s);
-        if (e != hipSuccess) return (int)e;
-        if (kpq < 256u)
-            km_fill_f32_kernel<<<blocks_for((size_t)(256 - kpq) * sub, 256), 256, 0, s>>>(
-                cb + (size_t)kpq * sub, (size_t)(256 - kpq) * sub, FLT_MAX);
-        e = assign_launch(res + (size_t)m * sub, n, sub, dim, cb, 256, nullptr, d_codes + m,
-                          m_pq, s);
-        if (e != hipSuccess) return (int)e;
-    }
-    e = hipStreamSynchronize(s);
-    return (int)e;
+    e = hipStreamSynchronize(s);  // residuals ready for the worker streams
+    if (e != hipSuccess) return (int)e;
+    int device = 0;
+    e = hipGetDevice(&device);
+    if (e != hipSuccess) return (int)e;
+    std::vector<hipError_t> errs(B.workers, hipSuccess);
+    auto worker = [&](uint32_t w) {
+        hipError_t err = hipSetDevice(device);
+        hipStream_t ws_stream = nullptr;
+        if (err == hipSuccess) err = hipStreamCreateWithFlags(&ws_stream, hipStreamNonBlocking);
+        char* slot = ws + B.pq_slot + (size_t)w * B.pq_slot_bytes;
+        uint32_t* sub_assign = reinterpret_cast<uint32_t*>(slot + B.pq_assign);
+        uint32_t* sub_sizes = reinterpret_cast<uint32_t*>(slot + B.pq_sizes);
+        for (uint32_t m = w; err == hipSuccess && m < m_pq; m += B.workers) {
+            float* cb = d_codebooks + (size_t)m * 256 * sub;
+            err = kmeans_fit(res + (size_t)m * sub, n, sub, dim, kpq, pq_iters,
+                             seed + (uint64_t)(m + 1) * 0x9E3779B97F4A7C15ull, cb, sub_assign,
+                             sub_sizes, nullptr, nullptr, iters_run ? iters_run + 1 + m : nullptr,
+                             slot + B.pq_km, ws_stream, false);
+            if (err != hipSuccess) break;
+            if (kpq < 256u)
+                km_fill_f32_kernel<<<blocks_for((size_t)(256 - kpq) * sub, 256), 256, 0,
+                                     ws_stream>>>(cb + (size_t)kpq * sub,
+                                                  (size_t)(256 - kpq) * sub, FLT_MAX);
+            err = assign_launch(res + (size_t)m * sub, n, sub, dim, cb, 256, nullptr,
+                                d_codes + m, m_pq, ws_stream);
+        }
+        if (ws_stream) {
+            const hipError_t se = hipStreamSynchronize(ws_stream);
+            if (err == hipSuccess) err = se;
+            (void)hipStreamDestroy(ws_stream);
+        }
+        errs[w] = err;
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t w = 1; w < B.workers; ++w) pool.emplace_back(worker, w);
+    worker(0);
+    for (auto& t : pool) t.join();
+    for (hipError_t we : errs)
+        if (we != hipSuccess) return (int)we;
+    return 0;
 }
 
 }  // extern "C"
