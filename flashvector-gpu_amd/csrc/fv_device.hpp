@@ -325,6 +325,32 @@ __device__ __forceinline__ double exact_distance_f64(const float* __restrict__ q
 
 // OCP FP8 E4M3 (bias 7, no infinities, largest finite 448): round to nearest
 // even, saturating to +-448; NaN maps to the NaN code 0x7f.
+// FP16 bits of a stored row element, by the reference's host encoder
+// (crates/engine/src/quant.rs:237-273): normal results round half away from
+// zero on the magnitude (mantissa + 0x1000, then truncate), subnormal
+// results truncate, overflow gives infinity, NaN becomes the quiet 0x7e00.
+// Stored codes must equal the reference's tables bit for bit; queries are
+// converted with round-to-nearest-even, as the reference's kernel does.
+__host__ __device__ __forceinline__ uint16_t f32_to_f16_rows(float x) {
+    uint32_t bits;
+    __builtin_memcpy(&bits, &x, 4);
+    const uint16_t sign = (uint16_t)((bits >> 16) & 0x8000u);
+    const int32_t exp = (int32_t)((bits >> 23) & 0xffu);
+    const uint32_t mant = bits & 0x007fffffu;
+    if (exp == 255) return (uint16_t)(sign | (mant == 0 ? 0x7c00u : 0x7e00u));
+    const int32_t e16 = exp - 112;
+    if (e16 >= 31) return (uint16_t)(sign | 0x7c00u);
+    if (e16 <= 0) {
+        if (e16 < -10) return sign;
+        return (uint16_t)(sign | (uint16_t)((mant | 0x00800000u) >> (14 - e16)));
+    }
+    const uint32_t r = mant + 0x1000u;
+    if (r & 0x00800000u) {
+        return e16 + 1 >= 31 ? (uint16_t)(sign | 0x7c00u) : (uint16_t)(sign | (uint16_t)((e16 + 1) << 10));
+    }
+    return (uint16_t)(sign | (uint16_t)(e16 << 10) | (uint16_t)(r >> 13));
+}
+
 __host__ __device__ __forceinline__ uint8_t f32_to_e4m3(float f) {
     uint32_t u = __builtin_bit_cast(uint32_t, f);
     const uint8_t sign = (uint8_t)((u >> 24) & 0x80u);

@@ -17,8 +17,7 @@ __global__ void encode_16bit_kernel(const float* __restrict__ x, uint32_t n, uin
     float v = d < dim ? x[(size_t)r * dim + d] : 0.0f;
     uint16_t h;
     if (dtype == FV_DTYPE_F16) {
-        _Float16 f = (_Float16)v;
-        h = __builtin_bit_cast(uint16_t, f);
+        h = f32_to_f16_rows(v);
     } else {
         uint32_t u = __float_as_uint(v);
         u += 0x7fffu + ((u >> 16) & 1u);

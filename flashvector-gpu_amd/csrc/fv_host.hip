@@ -564,3 +564,570 @@ int fvh_stats(const FvEngine* e, uint64_t* queries, double* total_ms) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// FLSHVEC1 snapshots (crates/engine/src/persistence.rs:3-200,
+// crates/engine/src/index.rs:845-1258).
+// ---------------------------------------------------------------------------
+
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <cstdio>
+#include <map>
+#include <string>
+
+namespace fvh {
+
+constexpr size_t kSnapAlign = 64;
+constexpr char kSnapMagic[8] = {'F', 'L', 'S', 'H', 'V', 'E', 'C', '1'};
+
+static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+static size_t dtype_size(const std::string& t) {
+    if (t == "f32" || t == "u32") return 4;
+    if (t == "u16") return 2;
+    if (t == "u8") return 1;
+    return 0;
+}
+
+static void json_str(std::string& o, const char* s) {
+    o += '"';
+    for (; *s; ++s) {
+        if (*s == '"' || *s == '\\') o += '\\';
+        o += *s;
+    }
+    o += '"';
+}
+
+// Minimal JSON reader for the manifest: objects, arrays, strings, numbers
+// (kept as text), true/false/null.
+struct JVal {
+    enum Kind { Null, Bool, Num, Str, Arr, Obj } kind = Null;
+    std::string text;  // number or string
+    std::vector<JVal> arr;
+    std::vector<std::pair<std::string, JVal>> obj;
+    const JVal* get(const char* k) const {
+        for (const auto& kv : obj)
+            if (kv.first == k) return &kv.second;
+        return nullptr;
+    }
+};
+
+struct JParser {
+    const char* p;
+    const char* end;
+    int depth = 0;
+    void ws() {
+        while (p < end && (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r')) ++p;
+    }
+    bool str(std::string& out) {
+        if (p >= end || *p != '"') return false;
+        ++p;
+        while (p < end && *p != '"') {
+            if (*p == '\\') {
+                if (++p >= end) return false;
+                const char c = *p;
+                if (c == 'n') out += '\n';
+                else if (c == 't') out += '\t';
+                else if (c == 'r') out += '\r';
+                else if (c == 'b') out += '\b';
+                else if (c == 'f') out += '\f';
+                else if (c == 'u') {
+                    if (end - p < 5) return false;
+                    out += '?';  // names here are ASCII; escapes are not interpreted
+                    p += 4;
+                } else out += c;
+                ++p;
+            } else {
+                out += *p++;
+            }
+        }
+        if (p >= end) return false;
+        ++p;
+        return true;
+    }
+    bool value(JVal& v) {
+        if (++depth > 64) return false;
+        ws();
+        if (p >= end) return false;
+        bool ok = true;
+        if (*p == '{') {
+            v.kind = JVal::Obj;
+            ++p;
+            ws();
+            if (p < end && *p == '}') {
+                ++p;
+            } else {
+                for (;;) {
+                    ws();
+                    std::string k;
+                    JVal x;
+                    if (!str(k)) { ok = false; break; }
+                    ws();
+                    if (p >= end || *p != ':') { ok = false; break; }
+                    ++p;
+                    if (!value(x)) { ok = false; break; }
+                    v.obj.emplace_back(std::move(k), std::move(x));
+                    ws();
+                    if (p < end && *p == ',') { ++p; continue; }
+                    if (p < end && *p == '}') { ++p; break; }
+                    ok = false;
+                    break;
+                }
+            }
+        } else if (*p == '[') {
+            v.kind = JVal::Arr;
+            ++p;
+            ws();
+            if (p < end && *p == ']') {
+                ++p;
+            } else {
+                for (;;) {
+                    JVal x;
+                    if (!value(x)) { ok = false; break; }
+                    v.arr.push_back(std::move(x));
+                    ws();
+                    if (p < end && *p == ',') { ++p; continue; }
+                    if (p < end && *p == ']') { ++p; break; }
+                    ok = false;
+                    break;
+                }
+            }
+        } else if (*p == '"') {
+            v.kind = JVal::Str;
+            ok = str(v.text);
+        } else if (*p == '-' || (*p >= '0' && *p <= '9')) {
+            v.kind = JVal::Num;
+            while (p < end && (*p == '-' || *p == '+' || *p == '.' || *p == 'e' || *p == 'E' ||
+                               (*p >= '0' && *p <= '9')))
+                v.text += *p++;
+        } else if (end - p >= 4 && !strncmp(p, "true", 4)) {
+            v.kind = JVal::Bool; v.text = "true"; p += 4;
+        } else if (end - p >= 5 && !strncmp(p, "false", 5)) {
+            v.kind = JVal::Bool; v.text = "false"; p += 5;
+        } else if (end - p >= 4 && !strncmp(p, "null", 4)) {
+            v.kind = JVal::Null; p += 4;
+        } else {
+            ok = false;
+        }
+        --depth;
+        return ok;
+    }
+};
+
+// Unsigned integer field; false when absent or not a plain unsigned integer.
+static bool juint(const JVal* v, uint64_t& out) {
+    if (!v || v->kind != JVal::Num || v->text.empty()) return false;
+    uint64_t x = 0;
+    for (char c : v->text) {
+        if (c < '0' || c > '9') return false;
+        const uint64_t d = (uint64_t)(c - '0');
+        if (x > (UINT64_MAX - d) / 10) return false;
+        x = x * 10 + d;
+    }
+    out = x;
+    return true;
+}
+
+}  // namespace fvh
+
+struct FvSnapshot {
+    void* map = nullptr;
+    size_t size = 0;
+    size_t data_start = 0;
+    struct Sec { size_t offset, byte_len, len; std::string dtype; };
+    std::map<std::string, Sec> sections;
+    ~FvSnapshot() {
+        if (map) munmap(map, size);
+    }
+};
+
+extern "C" {
+
+int fvh_snapshot_write(const char* path, const FvSnapshotInfo* info,
+                       const FvSnapshotSection* sections, uint32_t num_sections) {
+    using namespace fvh;
+    if (!path || !info || (num_sections && !sections)) return -1;
+    struct Meta { size_t offset, byte_len; };
+    std::map<std::string, std::pair<Meta, const FvSnapshotSection*>> sorted;
+    std::vector<Meta> metas(num_sections);
+    size_t cursor = 0;
+    for (uint32_t i = 0; i < num_sections; ++i) {
+        const FvSnapshotSection& s = sections[i];
+        if (!s.name || !s.dtype) return -1;
+        const size_t es = dtype_size(s.dtype);
+        if (!es || (s.len && !s.data)) return FVH_ERR_FORMAT;
+        cursor = align_up(cursor, kSnapAlign);
+        metas[i] = Meta{cursor, (size_t)s.len * es};
+        sorted[s.name] = {metas[i], &s};  // a repeated name keeps its last entry
+        cursor += metas[i].byte_len;
+    }
+    // serde_json's compact form of SnapshotManifest, fields in declaration order.
+    std::string j = "{\"version\":" + std::to_string(info->version) + ",\"config\":{\"dim\":" +
+                    std::to_string(info->dim) + ",\"max_elements\":" +
+                    std::to_string(info->max_elements) + ",\"m\":" + std::to_string(info->m) +
+                    ",\"ef_construction\":" + std::to_string(info->ef_construction) +
+                    ",\"nlist\":" + std::to_string(info->nlist) + ",\"m_pq\":" +
+                    std::to_string(info->m_pq) + ",\"metric\":" + std::to_string(info->metric) +
+                    "},\"num_vectors\":" + std::to_string(info->num_vectors) +
+                    ",\"entry_point\":" + std::to_string(info->entry_point) +
+                    ",\"cagra_degree\":" + std::to_string(info->cagra_degree) +
+                    ",\"cagra_seeds\":[],\"rabitq_padded_dim\":" +
+                    std::to_string(info->rabitq_padded_dim) + ",\"fp16_padded_dim\":" +
+                    std::to_string(info->fp16_padded_dim) + ",\"sparse_num_docs\":" +
+                    std::to_string(info->sparse_num_docs) + ",\"sparse_nnz\":" +
+                    std::to_string(info->sparse_nnz) + ",\"next_logical_id\":" +
+                    std::to_string(info->next_logical_id) + ",\"sections\":{";
+    bool first = true;
+    for (const auto& kv : sorted) {
+        if (!first) j += ',';
+        first = false;
+        json_str(j, kv.first.c_str());
+        j += ":{\"offset\":" + std::to_string(kv.second.first.offset) + ",\"byte_len\":" +
+             std::to_string(kv.second.first.byte_len) + ",\"dtype\":";
+        json_str(j, kv.second.second->dtype);
+        j += ",\"len\":" + std::to_string(kv.second.second->len) + "}";
+    }
+    j += "}}";
+    FILE* f = fopen(path, "wb");
+    if (!f) return FVH_ERR_IO;
+    bool ok = fwrite(kSnapMagic, 1, 8, f) == 8;
+    uint8_t hl[8];
+    for (int b = 0; b < 8; ++b) hl[b] = (uint8_t)((uint64_t)j.size() >> (8 * b));
+    ok = ok && fwrite(hl, 1, 8, f) == 8 && fwrite(j.data(), 1, j.size(), f) == j.size();
+    static const uint8_t zeros[kSnapAlign] = {0};
+    const size_t header_end = 16 + j.size();
+    const size_t pad = align_up(header_end, kSnapAlign) - header_end;
+    ok = ok && (pad == 0 || fwrite(zeros, 1, pad, f) == pad);
+    size_t written = 0;
+    for (uint32_t i = 0; ok && i < num_sections; ++i) {
+        const size_t gap = metas[i].offset - written;
+        ok = gap == 0 || fwrite(zeros, 1, gap, f) == gap;
+        if (ok && metas[i].byte_len)
+            ok = fwrite(sections[i].data, 1, metas[i].byte_len, f) == metas[i].byte_len;
+        written = metas[i].offset + metas[i].byte_len;
+    }
+    if (fclose(f) != 0) ok = false;
+    return ok ? 0 : FVH_ERR_IO;
+}
+
+int fvh_snapshot_open(const char* path, FvSnapshot** out, FvSnapshotInfo* info) {
+    using namespace fvh;
+    if (!path || !out) return -1;
+    *out = nullptr;
+    const int fd = open(path, O_RDONLY);
+    if (fd < 0) return FVH_ERR_IO;
+    struct stat st;
+    if (fstat(fd, &st) != 0) {
+        close(fd);
+        return FVH_ERR_IO;
+    }
+    std::unique_ptr<FvSnapshot> s(new FvSnapshot());
+    s->size = (size_t)st.st_size;
+    if (s->size < 16) {
+        close(fd);
+        return FVH_ERR_FORMAT;
+    }
+    s->map = mmap(nullptr, s->size, PROT_READ, MAP_PRIVATE, fd, 0);
+    close(fd);
+    if (s->map == MAP_FAILED) {
+        s->map = nullptr;
+        return FVH_ERR_IO;
+    }
+    const uint8_t* b = static_cast<const uint8_t*>(s->map);
+    if (memcmp(b, kSnapMagic, 8) != 0) return FVH_ERR_FORMAT;
+    uint64_t hlen = 0;
+    for (int i = 0; i < 8; ++i) hlen |= (uint64_t)b[8 + i] << (8 * i);
+    if (hlen > s->size - 16) return FVH_ERR_FORMAT;   // truncated header
+    const size_t header_end = 16 + (size_t)hlen;
+    JParser jp{reinterpret_cast<const char*>(b + 16), reinterpret_cast<const char*>(b + header_end)};
+    JVal root;
+    if (!jp.value(root) || root.kind != JVal::Obj) return FVH_ERR_FORMAT;
+    jp.ws();
+    if (jp.p != jp.end) return FVH_ERR_FORMAT;
+    s->data_start = align_up(header_end, kSnapAlign);
+    if (s->data_start > s->size) return FVH_ERR_FORMAT;  // truncated payload
+    FvSnapshotInfo I{};
+    const JVal* cfg = root.get("config");
+    const JVal* secs = root.get("sections");
+    uint64_t v = 0;
+    auto u32f = [&](const JVal* o, const char* k, uint32_t& dst) {
+        if (!juint(o ? o->get(k) : nullptr, v) || v > UINT32_MAX) return false;
+        dst = (uint32_t)v;
+        return true;
+    };
+    auto u64f = [&](const char* k, uint64_t& dst) { return juint(root.get(k), dst); };
+    const JVal* seeds = root.get("cagra_seeds");
+    if (!cfg || cfg->kind != JVal::Obj || !secs || secs->kind != JVal::Obj || !seeds ||
+        seeds->kind != JVal::Arr || !u32f(&root, "version", I.version) ||
+        !u32f(cfg, "dim", I.dim) || !u32f(cfg, "max_elements", I.max_elements) ||
+        !u32f(cfg, "m", I.m) || !u32f(cfg, "ef_construction", I.ef_construction) ||
+        !u32f(cfg, "nlist", I.nlist) || !u32f(cfg, "m_pq", I.m_pq) ||
+        !u32f(cfg, "metric", I.metric) || !u64f("num_vectors", I.num_vectors) ||
+        !u32f(&root, "entry_point", I.entry_point) || !u64f("cagra_degree", I.cagra_degree) ||
+        !u64f("rabitq_padded_dim", I.rabitq_padded_dim) ||
+        !u64f("fp16_padded_dim", I.fp16_padded_dim) ||
+        !u64f("sparse_num_docs", I.sparse_num_docs) || !u64f("sparse_nnz", I.sparse_nnz) ||
+        !u32f(&root, "next_logical_id", I.next_logical_id))
+        return FVH_ERR_FORMAT;
+    I.num_cagra_seeds = (uint32_t)seeds->arr.size();
+    for (const auto& kv : secs->obj) {
+        if (kv.second.kind != JVal::Obj) return FVH_ERR_FORMAT;
+        FvSnapshot::Sec sec;
+        uint64_t off, bl, len;
+        const JVal* dt = kv.second.get("dtype");
+        if (!juint(kv.second.get("offset"), off) || !juint(kv.second.get("byte_len"), bl) ||
+            !juint(kv.second.get("len"), len) || !dt || dt->kind != JVal::Str)
+            return FVH_ERR_FORMAT;
+        // Every section must lie inside the file (persistence.rs:150-158).
+        if (off > s->size || bl > s->size || s->data_start + off > s->size - bl)
+            return FVH_ERR_FORMAT;
+        sec.offset = (size_t)off;
+        sec.byte_len = (size_t)bl;
+        sec.len = (size_t)len;
+        sec.dtype = dt->text;
+        s->sections[kv.first] = sec;
+    }
+    if (info) *info = I;
+    *out = s.release();
+    return 0;
+}
+
+int fvh_snapshot_section(const FvSnapshot* s, const char* name, const void** data,
+                         uint64_t* byte_len, uint64_t* len, char* dtype8) {
+    if (!s || !name) return -1;
+    auto it = s->sections.find(name);
+    if (it == s->sections.end()) return 1;
+    const auto& sec = it->second;
+    if (data) *data = static_cast<const uint8_t*>(s->map) + s->data_start + sec.offset;
+    if (byte_len) *byte_len = sec.byte_len;
+    if (len) *len = sec.len;
+    if (dtype8) {
+        const size_t n = std::min<size_t>(sec.dtype.size(), 7);
+        memcpy(dtype8, sec.dtype.data(), n);
+        dtype8[n] = 0;
+    }
+    return 0;
+}
+
+void fvh_snapshot_close(FvSnapshot* s) { delete s; }
+
+}  // extern "C"
+
+namespace fvh {
+
+template <typename T>
+static hipError_t download(std::vector<T>& dst, const T* src, size_t count) {
+    dst.resize(count);
+    if (count == 0) return hipSuccess;
+    return hipMemcpy(dst.data(), src, count * sizeof(T), hipMemcpyDeviceToHost);
+}
+
+// Typed view of a section: its bytes must be a whole number of T and the
+// pointer aligned (persistence.rs:173-200); absent sections give nullptr.
+template <typename T>
+static int section_view(const FvSnapshot* s, const char* name, const T** p, size_t* count) {
+    const void* d = nullptr;
+    uint64_t bl = 0;
+    *p = nullptr;
+    *count = 0;
+    const int rc = fvh_snapshot_section(s, name, &d, &bl, nullptr, nullptr);
+    if (rc == 1) return 1;
+    if (rc) return rc;
+    if (bl % sizeof(T) || (reinterpret_cast<uintptr_t>(d) % alignof(T))) return FVH_ERR_FORMAT;
+    *p = static_cast<const T*>(d);
+    *count = (size_t)(bl / sizeof(T));
+    return 0;
+}
+
+}  // namespace fvh
+
+extern "C" {
+
+int fvh_save_snapshot(FvEngine* e, const char* path) {
+    using namespace fvh;
+    if (!e || !path) return -1;
+    if (e->mode == 0) return -1;
+    const uint32_t dim = e->cfg.dim, n = e->n_main;
+    hipError_t he = hipDeviceSynchronize();
+    if (he != hipSuccess) return (int)he;
+    std::vector<float> rows, centroids, codebooks, norms, scales, delta_rows;
+    std::vector<uint8_t> codes, pq_codes;
+    std::vector<uint32_t> offsets, vec_ids, lids(n), tomb;
+    std::vector<uint16_t> f16;
+    if ((he = download(rows, e->rows.p, (size_t)n * dim)) != hipSuccess) return (int)he;
+    for (uint32_t s = 0; s < n; ++s) lids[s] = e->h_slot2lid.empty() ? s : e->h_slot2lid[s];
+    FvSnapshotInfo I{};
+    I.version = 1;
+    I.dim = dim;
+    I.max_elements = n;
+    I.m = 32;
+    I.ef_construction = 128;
+    I.metric = e->cfg.metric;
+    I.num_vectors = n;
+    uint32_t nlist = 0, m_pq = 0;
+    const char* extra_name = nullptr;
+    if (e->mode == 2) {
+        nlist = e->ivf.nlist;
+        m_pq = e->ivf.m_pq;
+        if ((he = download(centroids, e->centroids.p, (size_t)nlist * dim)) != hipSuccess ||
+            (he = download(codebooks, e->codebooks.p, (size_t)m_pq * 256 * (dim / m_pq))) != hipSuccess ||
+            (he = download(pq_codes, e->pq_codes.p, (size_t)n * m_pq)) != hipSuccess ||
+            (he = download(offsets, e->offsets.p, (size_t)nlist + 1)) != hipSuccess ||
+            (he = download(vec_ids, e->vec_ids.p, (size_t)n)) != hipSuccess)
+            return (int)he;
+    } else {
+        const FvFlatTables& t = e->flat;
+        const size_t esz = t.dtype == FV_DTYPE_FP8_E4M3 ? 1 : 2;
+        if ((he = download(codes, static_cast<const uint8_t*>(t.d_codes), (size_t)n * t.padded_dim * esz)) != hipSuccess ||
+            (he = download(norms, e->norms.p, (size_t)n)) != hipSuccess)
+            return (int)he;
+        if (t.dtype == FV_DTYPE_F16) {
+            // The reference pads FP16 rows to a multiple of 16 (quant.rs:306-327);
+            // the padding is zero either way.
+            const uint32_t pd16 = (dim + 15) / 16 * 16;
+            f16.resize((size_t)n * pd16);
+            const uint16_t* src = reinterpret_cast<const uint16_t*>(codes.data());
+            for (uint32_t r = 0; r < n; ++r)
+                memcpy(&f16[(size_t)r * pd16], src + (size_t)r * t.padded_dim, pd16 * 2);
+            I.fp16_padded_dim = pd16;
+        } else {
+            extra_name = t.dtype == FV_DTYPE_BF16 ? "vectors_bf16" : "vectors_fp8";
+            if (t.dtype == FV_DTYPE_FP8_E4M3 &&
+                (he = download(scales, e->scales.p, (size_t)n)) != hipSuccess)
+                return (int)he;
+        }
+    }
+    I.nlist = nlist;
+    I.m_pq = m_pq;
+    // Tombstones: logical ids of dead main slots and dead delta rows, ascending.
+    for (uint32_t s = 0; s < n; ++s)
+        if (!((e->h_valid[s >> 5] >> (s & 31u)) & 1u)) tomb.push_back(lids[s]);
+    for (uint32_t j = 0; j < e->n_delta; ++j)
+        if (!((e->h_delta_alive[j >> 5] >> (j & 31u)) & 1u)) tomb.push_back(e->h_delta_ids[j]);
+    std::sort(tomb.begin(), tomb.end());
+    tomb.erase(std::unique(tomb.begin(), tomb.end()), tomb.end());
+    uint64_t next = 0;
+    for (uint32_t l : lids) next = std::max<uint64_t>(next, (uint64_t)l + 1);
+    for (uint32_t l : e->h_delta_ids) next = std::max<uint64_t>(next, (uint64_t)l + 1);
+    I.next_logical_id = (uint32_t)std::min<uint64_t>(next, UINT32_MAX);
+    if (e->n_delta &&
+        (he = download(delta_rows, e->delta_rows.p, (size_t)e->n_delta * dim)) != hipSuccess)
+        return (int)he;
+    // index.rs:866-1028 section order.
+    std::vector<FvSnapshotSection> S = {
+        {"vectors", "f32", rows.size(), rows.data()},
+        {"logical_ids", "u32", lids.size(), lids.data()},
+        {"main_valid_words", "u32", e->h_valid.size(), e->h_valid.data()},
+        {"hnsw_adjacency", "u32", 0, nullptr},
+        {"hnsw_degrees", "u32", 0, nullptr},
+        {"cagra_adjacency", "u32", 0, nullptr},
+        {"ivf_centroids", "f32", centroids.size(), centroids.data()},
+        {"pq_codebooks", "f32", codebooks.size(), codebooks.data()},
+        {"pq_codes", "u8", pq_codes.size(), pq_codes.data()},
+        {"ivf_offsets", "u32", offsets.size(), offsets.data()},
+        {"ivf_vec_ids", "u32", vec_ids.size(), vec_ids.data()},
+        {"sq_mins", "f32", 0, nullptr},
+        {"sq_scales", "f32", 0, nullptr},
+        {"sq_codes", "u8", 0, nullptr},
+        {"rabitq_signs", "f32", 0, nullptr},
+        {"rabitq_centroids", "f32", 0, nullptr},
+        {"rabitq_bits", "u32", 0, nullptr},
+        {"rabitq_meta", "f32", 0, nullptr},
+        {"vectors_fp16", "u16", f16.size(), f16.data()},
+        {"vec_norms_sq", "f32", norms.size(), norms.data()},
+    };
+    if (extra_name) {
+        const bool is8 = e->flat.dtype == FV_DTYPE_FP8_E4M3;
+        S.push_back({extra_name, is8 ? "u8" : "u16", is8 ? codes.size() : codes.size() / 2,
+                     codes.data()});
+        if (is8) S.push_back({"row_scales", "f32", scales.size(), scales.data()});
+    }
+    if (!tomb.empty()) S.push_back({"tombstones", "u32", tomb.size(), tomb.data()});
+    if (e->n_delta) {
+        S.push_back({"delta_vectors", "f32", delta_rows.size(), delta_rows.data()});
+        S.push_back({"delta_ids", "u32", e->h_delta_ids.size(), e->h_delta_ids.data()});
+    }
+    return fvh_snapshot_write(path, &I, S.data(), (uint32_t)S.size());
+}
+
+int fvh_load_snapshot(FvEngine* e, const char* path, uint32_t flat_dtype) {
+    using namespace fvh;
+    if (!e || !path) return -1;
+    FvSnapshot* raw = nullptr;
+    FvSnapshotInfo I{};
+    int rc = fvh_snapshot_open(path, &raw, &I);
+    if (rc) return rc;
+    std::unique_ptr<FvSnapshot> s(raw);
+    const uint32_t dim = e->cfg.dim;
+    if (I.dim != dim || I.metric != e->cfg.metric) return (int)hipErrorInvalidValue;
+    const uint64_t n64 = I.num_vectors;
+    if (n64 == 0 || n64 > UINT32_MAX) return FVH_ERR_FORMAT;
+    const uint32_t n = (uint32_t)n64;
+    const float* rows; const uint32_t* lids; const uint32_t* valid;
+    size_t c_rows, c_lids, c_valid;
+    if (section_view(s.get(), "vectors", &rows, &c_rows) ||
+        section_view(s.get(), "logical_ids", &lids, &c_lids) ||
+        section_view(s.get(), "main_valid_words", &valid, &c_valid))
+        return FVH_ERR_FORMAT;
+    if (c_rows != (size_t)n * dim || c_lids != n || c_valid < words_for(n)) return FVH_ERR_FORMAT;
+    // Identity ids load as "no remap" (index.rs:1187-1190 id_remapped).
+    bool remapped = false;
+    for (uint32_t sl = 0; sl < n && !remapped; ++sl) remapped = lids[sl] != sl;
+    const uint32_t* use_lids = remapped ? lids : nullptr;
+    if (flat_dtype == 0) {
+        const float* cent; const float* books; const uint8_t* codes; const uint32_t* off;
+        const uint32_t* vids;
+        size_t c_cent, c_books, c_codes, c_off, c_vids;
+        if (section_view(s.get(), "ivf_centroids", &cent, &c_cent) ||
+            section_view(s.get(), "pq_codebooks", &books, &c_books) ||
+            section_view(s.get(), "pq_codes", &codes, &c_codes) ||
+            section_view(s.get(), "ivf_offsets", &off, &c_off) ||
+            section_view(s.get(), "ivf_vec_ids", &vids, &c_vids))
+            return FVH_ERR_FORMAT;
+        if (c_off < 2 || c_cent != (c_off - 1) * dim || c_vids != n || c_codes % n ||
+            c_codes == 0)
+            return FVH_ERR_FORMAT;
+        const uint32_t nlist = (uint32_t)(c_off - 1), m_pq = (uint32_t)(c_codes / n);
+        if (m_pq == 0 || dim % m_pq || c_books != (size_t)m_pq * 256 * (dim / m_pq) ||
+            off[nlist] != n)
+            return FVH_ERR_FORMAT;
+        for (size_t i = 0; i < c_vids; ++i)
+            if (vids[i] >= n) return FVH_ERR_FORMAT;
+        for (uint32_t c = 0; c < nlist; ++c)
+            if (off[c] > off[c + 1]) return FVH_ERR_FORMAT;
+        rc = fvh_load_ivfpq(e, cent, nlist, books, m_pq, codes, off, vids, rows, n, use_lids);
+    } else {
+        rc = fvh_load_flat(e, rows, n, flat_dtype, use_lids);
+    }
+    if (rc) return rc;
+    const float* drows; const uint32_t* dids;
+    size_t c_drows, c_dids;
+    const int a = section_view(s.get(), "delta_vectors", &drows, &c_drows);
+    const int b = section_view(s.get(), "delta_ids", &dids, &c_dids);
+    if ((a && a != 1) || (b && b != 1)) return FVH_ERR_FORMAT;
+    if (a == 0 && b == 0 && c_dids) {
+        if (c_drows != c_dids * dim) return FVH_ERR_FORMAT;
+        if ((rc = fvh_add_delta(e, drows, dids, (uint32_t)c_dids))) return rc;
+    }
+    const uint32_t* tomb;
+    size_t c_tomb;
+    const int t = section_view(s.get(), "tombstones", &tomb, &c_tomb);
+    if (t == 0) {
+        if (c_tomb) rc = fvh_delete(e, tomb, (uint32_t)c_tomb);
+    } else if (t == 1) {
+        // index.rs:1195-1206: without tombstones, cleared validity bits.
+        std::vector<uint32_t> dead;
+        for (uint32_t sl = 0; sl < n; ++sl)
+            if (!((valid[sl >> 5] >> (sl & 31u)) & 1u)) dead.push_back(lids[sl]);
+        if (!dead.empty()) rc = fvh_delete(e, dead.data(), (uint32_t)dead.size());
+    } else {
+        return FVH_ERR_FORMAT;
+    }
+    return rc;
+}
+
+}  // extern "C"

@@ -133,6 +133,18 @@ class Engine:
                                             L.ptr(out), L.stream_handle()), "fvh_search_device")
         return out
 
+    def save_snapshot(self, path):
+        """FLSHVEC1 file of this engine's tables, ids, tombstones and delta
+        (GpuVectorIndex::save_snapshot, index.rs:845-1031)."""
+        L.check(self._lib.fvh_save_snapshot(self._h, str(path).encode()), "fvh_save_snapshot")
+
+    def load_snapshot(self, path, dtype=None):
+        """Loads a FLSHVEC1 file (index.rs:1033-1258): its IVF-PQ tables when
+        dtype is None, else a flat index of `dtype` encoded from its rows."""
+        L.check(self._lib.fvh_load_snapshot(self._h, str(path).encode(),
+                                            0 if dtype is None else DTYPES[dtype]),
+                "fvh_load_snapshot")
+
     def stats(self):
         n = ctypes.c_uint64(0)
         ms = ctypes.c_double(0.0)

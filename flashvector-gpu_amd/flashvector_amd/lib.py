@@ -106,6 +106,26 @@ class FvFlatTuning(ctypes.Structure):
     ]
 
 
+class FvSnapshotInfo(ctypes.Structure):
+    """include/flashvector/fv_host.h (the SnapshotManifest scalars of
+    crates/engine/src/persistence.rs:35-48)."""
+
+    _fields_ = [("version", ctypes.c_uint32), ("dim", ctypes.c_uint32),
+                ("max_elements", ctypes.c_uint32), ("m", ctypes.c_uint32),
+                ("ef_construction", ctypes.c_uint32), ("nlist", ctypes.c_uint32),
+                ("m_pq", ctypes.c_uint32), ("metric", ctypes.c_uint32),
+                ("num_vectors", ctypes.c_uint64), ("entry_point", ctypes.c_uint32),
+                ("next_logical_id", ctypes.c_uint32), ("cagra_degree", ctypes.c_uint64),
+                ("rabitq_padded_dim", ctypes.c_uint64), ("fp16_padded_dim", ctypes.c_uint64),
+                ("sparse_num_docs", ctypes.c_uint64), ("sparse_nnz", ctypes.c_uint64),
+                ("num_cagra_seeds", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class FvSnapshotSection(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("dtype", ctypes.c_char_p),
+                ("len", ctypes.c_uint64), ("data", ctypes.c_void_p)]
+
+
 DTYPES = {"f32": 0, "f16": 1, "bf16": 2, "fp8": 3}
 
 P, U, I, SZ = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_size_t
@@ -154,6 +174,12 @@ SIGNATURES = {
     "fvh_search": (I, [P, P, U, U, P, U, P]),
     "fvh_search_device": (I, [P, P, U, U, P, U, P, P]),
     "fvh_stats": (I, [P, P, P]),
+    "fvh_snapshot_write": (I, [ctypes.c_char_p, P, P, U]),
+    "fvh_snapshot_open": (I, [ctypes.c_char_p, P, P]),
+    "fvh_snapshot_section": (I, [P, ctypes.c_char_p, P, P, P, P]),
+    "fvh_snapshot_close": (None, [P]),
+    "fvh_save_snapshot": (I, [P, ctypes.c_char_p]),
+    "fvh_load_snapshot": (I, [P, ctypes.c_char_p, U]),
     "fv_topk_ip_backward": (I, [P, U, P, U, U, P, U, P, P, ctypes.c_float, ctypes.c_float, P, P,
                                 P, P, P]),
     "fv_kmeans_assign": (I, [P, U, U, U, P, U, P, P, U, P]),

@@ -31,7 +31,9 @@ int fv_flat_exact_search(const float* d_vectors, uint32_t num_vectors, uint32_t 
                          size_t workspace_bytes, void* stream);
 
 /* ---- reduced-precision flat scan on the matrix cores --------------------
- * Rows are stored as 16-bit codes (FP16 or BF16, RNE from FP32) or as FP8
+ * Rows are stored as 16-bit codes (FP16 by the reference's host conversion,
+ * crates/engine/src/quant.rs:237-273, i.e. halves rounded away from zero and
+ * subnormals truncated; BF16 round-to-nearest-even) or as FP8
  * e4m3 codes with one FP32 scale per row (s = max|x|/448, code =
  * RNE-saturating e4m3(x/s)), padded with zeros to padded_dim (a multiple of
  * 16, at most 768). The scan scores every row on MFMA with FP32 accumulation

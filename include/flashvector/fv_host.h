@@ -91,6 +91,66 @@ int fvh_search_device(FvEngine* e, const float* d_queries, uint32_t nq, uint32_t
 /* Queries served and summed wall time of fvh_search* calls (metrics.rs). */
 int fvh_stats(const FvEngine* e, uint64_t* queries, double* total_ms);
 
+/* ---- FLSHVEC1 snapshots ---------------------------------------------------
+ * The reference's on-disk format (crates/engine/src/persistence.rs:3-200):
+ * magic "FLSHVEC1", little-endian u64 manifest length H, H bytes of compact
+ * JSON (SnapshotManifest: version, config {dim, max_elements, m,
+ * ef_construction, nlist, m_pq, metric}, num_vectors, entry_point,
+ * cagra_degree, cagra_seeds, rabitq_padded_dim, fp16_padded_dim,
+ * sparse_num_docs, sparse_nnz, next_logical_id, sections {name: {offset,
+ * byte_len, dtype, len}} sorted by name), zeros to a 64-byte boundary, then
+ * the sections in the order given, each starting 64-byte aligned (offsets
+ * relative to that boundary, gaps zero-filled). Host-only; errors: -1 NULL
+ * argument, FVH_ERR_IO file error, FVH_ERR_FORMAT invalid snapshot. */
+#define FVH_ERR_IO (-2)
+#define FVH_ERR_FORMAT (-3)
+
+typedef struct {
+    uint32_t version;
+    uint32_t dim, max_elements, m, ef_construction, nlist, m_pq, metric; /* IndexConfig */
+    uint64_t num_vectors;
+    uint32_t entry_point;
+    uint32_t next_logical_id;
+    uint64_t cagra_degree, rabitq_padded_dim, fp16_padded_dim, sparse_num_docs, sparse_nnz;
+    uint32_t num_cagra_seeds; /* written as []; reported when reading */
+    uint32_t reserved;
+} FvSnapshotInfo;
+
+typedef struct {
+    const char* name;    /* section name, e.g. "ivf_centroids" */
+    const char* dtype;   /* "f32", "u32", "u16" or "u8" */
+    uint64_t len;        /* elements */
+    const void* data;    /* len * element-size bytes (may be NULL when len is 0) */
+} FvSnapshotSection;
+
+typedef struct FvSnapshot FvSnapshot;
+
+/* write_aligned_snapshot (persistence.rs:51-109). */
+int fvh_snapshot_write(const char* path, const FvSnapshotInfo* info,
+                       const FvSnapshotSection* sections, uint32_t num_sections);
+/* MmapSnapshot::open (persistence.rs:127-162): maps the file read-only and
+ * checks magic, manifest bounds and JSON, and that every section lies inside
+ * the file. */
+int fvh_snapshot_open(const char* path, FvSnapshot** out, FvSnapshotInfo* info);
+/* Section bytes inside the mapping (valid until fvh_snapshot_close); returns 1
+ * when the section is absent. dtype receives up to 7 characters. */
+int fvh_snapshot_section(const FvSnapshot* s, const char* name, const void** data,
+                         uint64_t* byte_len, uint64_t* len, char* dtype8);
+void fvh_snapshot_close(FvSnapshot* s);
+
+/* Engine state as a snapshot (index.rs:845-1031): FP32 rows, logical ids,
+ * slot validity words, IVF-PQ tables, FP16 rows padded to 16 and their
+ * norms (flat FP16 engines; BF16 / FP8 codes go to the extra sections
+ * "vectors_bf16" / "vectors_fp8" + "row_scales"), tombstoned logical ids,
+ * delta rows and ids. Graph, SQ8 and RaBitQ sections are written empty. */
+int fvh_save_snapshot(FvEngine* e, const char* path);
+/* Loads a snapshot into an engine created with the same dim and metric
+ * (index.rs:1033-1258 for the sections above): flat_dtype 0 loads the IVF-PQ
+ * tables (which must be present), FvDtype 1..3 a flat index encoded from the
+ * FP32 rows; then the delta rows, then the tombstones (or, without a
+ * tombstones section, the cleared bits of main_valid_words). */
+int fvh_load_snapshot(FvEngine* e, const char* path, uint32_t flat_dtype);
+
 #ifdef __cplusplus
 }
 #endif
