@@ -152,6 +152,160 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
     return v;
 }
 
+// ---- Workgroup radix selection (256 threads) -------------------------------
+constexpr uint32_t kSelThreads = 256;
+constexpr uint32_t kSelWaves = kSelThreads / kWave;
+
+// LDS scratch of the selection helpers below (one per workgroup).
+struct BlockSelectLds {
+    uint64_t thr;                   // key stored by block_compact's keep == 2
+    uint64_t red[2 * kSelWaves];    // per-wave smallest / largest key
+    uint32_t sel[3];                // radix round: digit, keys below it, keys in it
+    uint32_t wsum[2][kSelWaves];    // per-wave survivor counts
+    uint32_t hist[2][256];          // digit histograms of consecutive rounds
+};
+
+// Radix selection over the c distinct keys buf[0..c) (c > keep >= 1): the
+// bits above the highest bit in which the smallest and largest key differ
+// are common to all keys; below it, 8-bit digits are histogrammed one round
+// at a time over the keys that share the prefix found so far, until the
+// digit holding the keep-th smallest holds no other key (or every bit is
+// fixed). On return exactly `keep` keys satisfy (key & pmask) <= prefix, and
+// the keep-th smallest is the one with (key & pmask) == prefix. All threads
+// call it after a barrier; buf is only read.
+__device__ inline void block_radix_kth(const uint64_t* buf, uint32_t c, uint32_t keep,
+                                       BlockSelectLds& L, uint64_t& pmask_out,
+                                       uint64_t& prefix_out) {
+    const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    unsigned long long lo = ~0ull, hi = 0;
+    for (uint32_t i = tid; i < c; i += kSelThreads) {
+        const unsigned long long k = buf[i];
+        lo = k < lo ? k : lo;
+        hi = k > hi ? k : hi;
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const unsigned long long a = __shfl_xor(lo, off), b = __shfl_xor(hi, off);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+    if (lane == 0) {
+        L.red[wave] = lo;
+        L.red[kSelWaves + wave] = hi;
+    }
+    L.hist[0][tid] = 0;
+    __syncthreads();
+    lo = L.red[0];
+    hi = L.red[kSelWaves];
+    for (uint32_t w = 1; w < kSelWaves; ++w) {
+        lo = L.red[w] < lo ? L.red[w] : lo;
+        hi = L.red[kSelWaves + w] > hi ? L.red[kSelWaves + w] : hi;
+    }
+    uint32_t hb = 64u - (uint32_t)__clzll((long long)(lo ^ hi));  // undetermined low bits
+    uint64_t pmask = hb >= 64u ? 0ull : ~0ull << hb;
+    uint64_t prefix = lo & pmask;
+    uint32_t need = keep;  // rank of the wanted key among those sharing the prefix
+    for (uint32_t r = 0;; ++r) {
+        const uint32_t nb = hb < 8u ? hb : 8u, shift = hb - nb;
+        const uint64_t dmask = (1ull << nb) - 1ull;
+        uint32_t* h = L.hist[r & 1u];
+        for (uint32_t i = tid; i < c; i += kSelThreads) {
+            const uint64_t k = buf[i];
+            if ((k & pmask) == prefix) atomicAdd(&h[(uint32_t)((k >> shift) & dmask)], 1u);
+        }
+        L.hist[(r + 1u) & 1u][tid] = 0;
+        __syncthreads();
+        if (wave == 0) {
+            const uint32_t h0 = h[4 * lane], h1 = h[4 * lane + 1], h2 = h[4 * lane + 2],
+                           h3 = h[4 * lane + 3];
+            const uint32_t sum = h0 + h1 + h2 + h3;
+            const uint32_t incl = wave_incl_scan(sum, lane), excl = incl - sum;
+            if (excl < need && need <= incl) {
+                uint32_t b = 4 * lane, before = excl, cnt = h0;
+                if (before + h0 < need) {
+                    before += h0; ++b; cnt = h1;
+                    if (before + h1 < need) {
+                        before += h1; ++b; cnt = h2;
+                        if (before + h2 < need) {
+                            before += h2; ++b; cnt = h3;
+                        }
+                    }
+                }
+                L.sel[0] = b;
+                L.sel[1] = before;
+                L.sel[2] = cnt;
+            }
+        }
+        __syncthreads();
+        const uint32_t b = L.sel[0], cnt = L.sel[2];
+        need -= L.sel[1];
+        prefix |= (uint64_t)b << shift;
+        pmask |= dmask << shift;
+        hb = shift;
+        if (cnt == 1u || hb == 0u) break;
+    }
+    pmask_out = pmask;
+    prefix_out = prefix;
+}
+
+// In-place compaction of buf[0..c) to the keys for which keepf(key) is
+// nonzero, in no particular order; a return of 2 also stores the key in
+// L.thr. Returns the number kept (the same in every thread). 8 keys per
+// thread per chunk: every key of a chunk is read before the chunk's barrier
+// and survivors are written after it, to positions below the number of keys
+// read so far. All threads call it after a barrier; callers barrier before
+// other threads read the survivors.
+template <class F>
+__device__ inline uint32_t block_compact(uint64_t* buf, uint32_t c, F keepf, BlockSelectLds& L) {
+    const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    uint32_t running = 0;
+    for (uint32_t base = 0, ch = 0; base < c; base += 8u * kSelThreads, ++ch) {
+        uint64_t v[8];
+        uint32_t keepmask = 0, n = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < 8u; ++u) {
+            const uint32_t i = base + u * kSelThreads + tid;
+            v[u] = i < c ? buf[i] : ~0ull;
+            if (i < c) {
+                const int f = keepf(v[u]);
+                if (f) {
+                    keepmask |= 1u << u;
+                    ++n;
+                    if (f == 2) L.thr = v[u];
+                }
+            }
+        }
+        const uint32_t incl = wave_incl_scan(n, lane);
+        if (lane == kWave - 1) L.wsum[ch & 1u][wave] = incl;
+        __syncthreads();
+        uint32_t off = running + incl - n;
+        for (uint32_t w = 0; w < kSelWaves; ++w) {
+            const uint32_t ws = L.wsum[ch & 1u][w];
+            off += w < wave ? ws : 0u;
+            running += ws;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 8u; ++u)
+            if (keepmask & (1u << u)) buf[off++] = v[u];
+    }
+    return running;
+}
+
+// Keeps the `keep` smallest of the c distinct keys buf[0..c) (c > keep >= 1),
+// compacted into buf[0..keep) in no particular order, and returns the keep-th
+// smallest to every thread (block_radix_kth, then block_compact). Same call
+// and barrier rules as block_compact.
+__device__ inline uint64_t block_select(uint64_t* buf, uint32_t c, uint32_t keep,
+                                        BlockSelectLds& L) {
+    uint64_t pmask, prefix;
+    block_radix_kth(buf, c, keep, L, pmask, prefix);
+    block_compact(buf, c, [pmask, prefix](uint64_t k) {
+        const uint64_t mp = k & pmask;
+        return mp < prefix ? 1 : (mp == prefix ? 2 : 0);
+    }, L);
+    return L.thr;
+}
+
 // Prune of one query's candidate buffer by one wavefront: sorts the `count`
 // buffered keys through LDS scratch (capacity >= next_pow2(count)), writes the
 // `keep` smallest back in ascending order and returns how many remain.

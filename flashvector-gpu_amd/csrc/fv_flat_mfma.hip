@@ -88,6 +88,11 @@ __global__ __launch_bounds__(256) void bound_select_kernel(const float* __restri
     if (lane == 0) atomicMin(&gthr[q], hi);
 }
 
+// Keys of rows whose distance is not admitted (>= 1e30 or NaN): the row in
+// the low word keeps them distinct, and they order after every admitted key.
+__device__ __forceinline__ uint64_t unadmitted_key(uint32_t row) { return 0xFFFFFFFF00000000ull | row; }
+__device__ __forceinline__ bool key_unadmitted(uint64_t k) { return (uint32_t)(k >> 32) == 0xFFFFFFFFu; }
+
 // One workgroup per query: streams the candidates of every chunk that are
 // within the final shared bound into an LDS top-S (sorted only when more
 // than S arrive) and writes the S best keys (approximate score, row) to
@@ -99,6 +104,7 @@ __global__ __launch_bounds__(256) void flat_select_kernel(
     extern __shared__ uint64_t s_k[];
     __shared__ uint32_t s_n;
     __shared__ uint32_t s_g;
+    __shared__ BlockSelectLds s_bs;
     const uint32_t q = blockIdx.x;
     uint32_t k, S;
     query_k_s(a, q, k, S);
@@ -140,26 +146,28 @@ __global__ __launch_bounds__(256) void flat_select_kernel(
             }
             __syncthreads();
             if (s_n + kSelectBatch > lcap) {
+                // Cut to the S smallest (radix selection, no sort) and lower
+                // the admission bound to the S-th smallest's distance.
                 const uint32_t n = s_n;
-                const uint32_t np = next_pow2(n);
-                for (uint32_t i = n + threadIdx.x; i < np; i += blockDim.x) s_k[i] = kEmptyKey;
-                __syncthreads();
-                block_bitonic_sort(s_k, np);
-                if (threadIdx.x == 0) {
-                    s_n = n < S ? n : S;
-                    if (n >= S) s_g = min(s_g, (uint32_t)(s_k[S - 1] >> 32));
+                if (n > S) {
+                    const uint64_t t = block_select(s_k, n, S, s_bs);
+                    __syncthreads();
+                    if (threadIdx.x == 0) {
+                        s_n = S;
+                        s_g = min(s_g, (uint32_t)(t >> 32));
+                    }
                 }
                 __syncthreads();
             }
         }
         __syncthreads();
     }
+    // The S smallest, in no particular order: the rerank re-scores them all
+    // and the finalize orders them.
     const uint32_t n = s_n;
     if (n > S) {
-        const uint32_t np = next_pow2(n);
-        for (uint32_t i = n + threadIdx.x; i < np; i += blockDim.x) s_k[i] = kEmptyKey;
+        block_select(s_k, n, S, s_bs);
         __syncthreads();
-        block_bitonic_sort(s_k, np);
     }
     const uint32_t have = n < S ? n : S;
     uint64_t* o = sel + (size_t)q * a.s_max;
@@ -335,7 +343,7 @@ __global__ __launch_bounds__(256) void flat_rerank_kernel(
         d32 = score_as_distance<METRIC>(key_dist(my_key), qv, dim);
     }
     if (act) {
-        sel[item] = d32 < kAdmit ? make_key(d32, s_row[tid]) : kEmptyKey;
+        sel[item] = d32 < kAdmit ? make_key(d32, s_row[tid]) : unadmitted_key(s_row[tid]);
         if (err) err[item] = e;
     }
 }
@@ -355,25 +363,84 @@ __global__ __launch_bounds__(256) void flat_finalize_kernel(
     GpuSearchResult* __restrict__ out) {
     extern __shared__ uint64_t s_k[];
     __shared__ uint32_t s_emax;
+    __shared__ BlockSelectLds s_bs;
+    __shared__ uint64_t s_succ[kSelWaves];
     const uint32_t q = blockIdx.x;
     uint32_t k, S;
     query_k_s(a, q, k, S);
-    const uint32_t have = sel_n[q];
+    const uint32_t have_in = sel_n[q];
     const uint64_t* src = sel + (size_t)q * a.s_max;
-    const uint32_t hp = next_pow2(have < 2u ? 2u : have);
-    for (uint32_t i = threadIdx.x; i < hp; i += blockDim.x) s_k[i] = i < have ? src[i] : kEmptyKey;
+    for (uint32_t i = threadIdx.x; i < have_in; i += blockDim.x) s_k[i] = src[i];
     const bool f64b = (a.flags & FV_SEARCH_F64_BOUNDARY) && X32 != nullptr && err != nullptr;
+    if (threadIdx.x == 0) s_emax = 0u;
+    __syncthreads();
     if (f64b) {
         // Largest FP32 error bound (positive floats order like their bits).
         float e = 0.0f;
-        for (uint32_t i = threadIdx.x; i < have; i += blockDim.x)
+        for (uint32_t i = threadIdx.x; i < have_in; i += blockDim.x)
             e = fmaxf(e, err[(size_t)q * a.s_max + i]);
-        if (threadIdx.x == 0) s_emax = 0u;
-        __syncthreads();
         atomicMax(&s_emax, __float_as_uint(e));
     }
     __syncthreads();
-    block_bitonic_sort(s_k, hp);
+    // Only the first k in (distance, row) order are written. Without the FP64
+    // boundary the k smallest are selected and only they are sorted. With it,
+    // every candidate within band of the k-th (all that can be re-scored) is
+    // kept, plus the next one above the band (the neighbour the band test
+    // looks at); a candidate further up cannot reach the first k, whose
+    // values (FP32, or FP64 within the error bound) stay below the band.
+    const uint32_t kq = k < have_in ? k : have_in;
+    uint32_t have = have_in;
+    if (have_in > kq && kq > 0) {
+        if (!(f64b && have_in > 1)) {
+            block_select(s_k, have_in, kq, s_bs);
+            have = kq;
+        } else {
+            uint64_t pmask, prefix;
+            block_radix_kth(s_k, have_in, kq, s_bs, pmask, prefix);
+            for (uint32_t i = threadIdx.x; i < have_in; i += blockDim.x)
+                if ((s_k[i] & pmask) == prefix) s_bs.thr = s_k[i];
+            __syncthreads();
+          if (!key_unadmitted(s_bs.thr)) {   // else fewer than k admitted: sort them all
+            const float lim = key_dist(s_bs.thr) + 2.0000002f * __uint_as_float(s_emax);
+            unsigned long long succ = kEmptyKey;
+            for (uint32_t i = threadIdx.x; i < have_in; i += blockDim.x) {
+                const uint64_t key = s_k[i];
+                if (key_dist(key) > lim && key < succ) succ = key;
+            }
+#pragma unroll
+            for (int off = kWave / 2; off > 0; off >>= 1) {
+                const unsigned long long o = __shfl_xor(succ, off);
+                succ = o < succ ? o : succ;
+            }
+            if ((threadIdx.x & (kWave - 1)) == 0) s_succ[threadIdx.x / kWave] = succ;
+            __syncthreads();
+            succ = s_succ[0];
+            for (uint32_t w = 1; w < kSelWaves; ++w) succ = s_succ[w] < succ ? s_succ[w] : succ;
+            const uint32_t m = block_compact(s_k, have_in, [lim](uint64_t key) {
+                return key_dist(key) <= lim ? 1 : 0;
+            }, s_bs);
+            __syncthreads();
+            if (succ != kEmptyKey && threadIdx.x == 0) s_k[m] = succ;
+            have = m + (succ != kEmptyKey ? 1u : 0u);
+          }
+        }
+        __syncthreads();
+    }
+    const uint32_t hp = next_pow2(have < 2u ? 2u : have);
+    for (uint32_t i = have + threadIdx.x; i < hp; i += blockDim.x) s_k[i] = kEmptyKey;
+    __syncthreads();
+    if (!f64b && have <= blockDim.x) {
+        // Each key to its rank (keys are distinct).
+        const uint64_t mine = threadIdx.x < have ? s_k[threadIdx.x] : kEmptyKey;
+        uint32_t rank = 0;
+        if (threadIdx.x < have)
+            for (uint32_t j = 0; j < have; ++j) rank += s_k[j] < mine ? 1u : 0u;
+        __syncthreads();
+        if (threadIdx.x < have) s_k[rank] = mine;
+        __syncthreads();
+    } else {
+        block_bitonic_sort(s_k, hp);
+    }
     const float* qv = Qm + (size_t)q * a.dim;
     if (f64b && have > 1) {
         // Candidates adjacent in FP32 order whose gap is at most twice the
@@ -398,11 +465,11 @@ __global__ __launch_bounds__(256) void flat_finalize_kernel(
             const uint64_t key = s_k[i];
             uint32_t r = 0xFFFFFFFFu;
             uint64_t v = ~0ull;
-            if (i < have && key != kEmptyKey) {
+            if (i < have && !key_unadmitted(key)) {
                 const float d = key_dist(key);
                 r = key_slot(key);
-                const bool near_prev = i > 0 && s_k[i - 1] != kEmptyKey && d - key_dist(s_k[i - 1]) <= band;
-                const bool near_nxt = i + 1 < have && s_k[i + 1] != kEmptyKey && key_dist(s_k[i + 1]) - d <= band;
+                const bool near_prev = i > 0 && !key_unadmitted(s_k[i - 1]) && d - key_dist(s_k[i - 1]) <= band;
+                const bool near_nxt = i + 1 < have && !key_unadmitted(s_k[i + 1]) && key_dist(s_k[i + 1]) - d <= band;
                 const uint64_t bits = __double_as_longlong((double)d);
                 v = (bits & 0x8000000000000000ull) ? ~bits : (bits | 0x8000000000000000ull);
                 if ((near_prev || near_nxt) && d <= lim) {
@@ -468,7 +535,9 @@ __global__ __launch_bounds__(256) void flat_finalize_kernel(
     for (uint32_t i = threadIdx.x; i < a.top_k; i += blockDim.x) {
         const uint64_t key = i < k && i < hp ? s_k[i] : kEmptyKey;
         GpuSearchResult r;
-        if (key == kEmptyKey) {
+        // Empty slots, rows whose distance is not admitted, and the FP64
+        // path's placeholders (row 0xFFFFFFFF) are written empty.
+        if (key_unadmitted(key) || key_slot(key) == FV_EMPTY_ID || !(key_dist(key) < kAdmit)) {
             r.id = FV_EMPTY_ID;
             r.distance = FV_EMPTY_DIST;
         } else {
@@ -564,11 +633,6 @@ __device__ __forceinline__ void wave_key_ranks(const uint64_t* keys, uint32_t n,
             if ((uint32_t)s < ns) rank[s] += ki < mine[s] ? 1u : 0u;
     }
 }
-
-// Keys of rows whose distance is not admitted (>= 1e30 or NaN): the row in
-// the low word keeps them distinct, and they order after every admitted key.
-__device__ __forceinline__ uint64_t unadmitted_key(uint32_t row) { return 0xFFFFFFFF00000000ull | row; }
-__device__ __forceinline__ bool key_unadmitted(uint64_t k) { return (uint32_t)(k >> 32) == 0xFFFFFFFFu; }
 
 template <int METRIC>
 __global__ __launch_bounds__(256) void flat_tail_kernel(

@@ -68,134 +68,14 @@ __global__ __launch_bounds__(256) void ivf_coarse_kernel(
 // Workgroup control words and selection scratch of the scan kernel, in
 // dynamic LDS after the LUT and the candidate buffer.
 struct IvfCtl {
-    uint64_t thr;                              // key returned by block_select
+    BlockSelectLds bs;                         // block_select scratch
     uint32_t item;                             // position of the current query in `order`
-    uint32_t sel[3];                           // radix round: digit, keys below it, keys in it
     uint32_t wn[3][kIvfThreads / kWave];       // per-wave keys to append: 2 checks, after a cut
-    uint64_t red[2 * (kIvfThreads / kWave)];   // per-wave smallest / largest key
-    uint32_t wsum[2][kIvfThreads / kWave];     // per-wave survivor counts
-    uint32_t hist[2][256];                     // digit histograms of consecutive rounds
 };
 constexpr size_t kIvfCtlBytes = (sizeof(IvfCtl) + 15) / 16 * 16;
 // Distance bits of an admitted key are never all ones (ord of a NaN).
 constexpr uint64_t kUnadmittedKey = 0xFFFFFFFF00000000ull;
-static_assert(kIvfThreads == 256, "one thread per histogram bin and per PQ codeword");
-
-// Keeps the `keep` smallest of the c distinct keys buf[0..c) (c > keep >= 1),
-// compacted into buf[0..keep) in no particular order, and returns the
-// keep-th smallest to every thread.
-//
-// Radix selection: the bits above the highest bit in which the smallest and
-// largest key differ are common to all keys; below it, 8-bit digits are
-// histogrammed one round at a time over the keys that share the prefix found
-// so far, until the digit holding the keep-th smallest holds no other key (or
-// every bit is fixed). The survivors are then exactly the keys whose prefix
-// is at most the final one. All threads call this after a barrier; it ends
-// with every read of buf done and the survivors written by their own
-// threads, so the caller must barrier before other threads read buf[0..keep).
-__device__ uint64_t block_select(uint64_t* buf, uint32_t c, uint32_t keep, IvfCtl& ctl) {
-    constexpr uint32_t kW = kIvfThreads / kWave;
-    const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-    unsigned long long lo = kEmptyKey, hi = 0;
-    for (uint32_t i = tid; i < c; i += kIvfThreads) {
-        const unsigned long long k = buf[i];
-        lo = k < lo ? k : lo;
-        hi = k > hi ? k : hi;
-    }
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        const unsigned long long a = __shfl_xor(lo, off), b = __shfl_xor(hi, off);
-        lo = a < lo ? a : lo;
-        hi = b > hi ? b : hi;
-    }
-    if (lane == 0) {
-        ctl.red[wave] = lo;
-        ctl.red[kW + wave] = hi;
-    }
-    ctl.hist[0][tid] = 0;
-    __syncthreads();
-    lo = ctl.red[0];
-    hi = ctl.red[kW];
-    for (uint32_t w = 1; w < kW; ++w) {
-        lo = ctl.red[w] < lo ? ctl.red[w] : lo;
-        hi = ctl.red[kW + w] > hi ? ctl.red[kW + w] : hi;
-    }
-    uint32_t hb = 64u - (uint32_t)__clzll((long long)(lo ^ hi));  // undetermined low bits
-    uint64_t pmask = hb >= 64u ? 0ull : ~0ull << hb;
-    uint64_t prefix = lo & pmask;
-    uint32_t need = keep;  // rank of the wanted key among those sharing the prefix
-    for (uint32_t r = 0;; ++r) {
-        const uint32_t nb = hb < 8u ? hb : 8u, shift = hb - nb;
-        const uint64_t dmask = (1ull << nb) - 1ull;
-        uint32_t* h = ctl.hist[r & 1u];
-        for (uint32_t i = tid; i < c; i += kIvfThreads) {
-            const uint64_t k = buf[i];
-            if ((k & pmask) == prefix) atomicAdd(&h[(uint32_t)((k >> shift) & dmask)], 1u);
-        }
-        ctl.hist[(r + 1u) & 1u][tid] = 0;
-        __syncthreads();
-        if (wave == 0) {
-            const uint32_t h0 = h[4 * lane], h1 = h[4 * lane + 1], h2 = h[4 * lane + 2],
-                           h3 = h[4 * lane + 3];
-            const uint32_t s = h0 + h1 + h2 + h3;
-            const uint32_t incl = wave_incl_scan(s, lane), excl = incl - s;
-            if (excl < need && need <= incl) {
-                uint32_t b = 4 * lane, before = excl, cnt = h0;
-                if (before + h0 < need) {
-                    before += h0; ++b; cnt = h1;
-                    if (before + h1 < need) {
-                        before += h1; ++b; cnt = h2;
-                        if (before + h2 < need) {
-                            before += h2; ++b; cnt = h3;
-                        }
-                    }
-                }
-                ctl.sel[0] = b;
-                ctl.sel[1] = before;
-                ctl.sel[2] = cnt;
-            }
-        }
-        __syncthreads();
-        const uint32_t b = ctl.sel[0], cnt = ctl.sel[2];
-        need -= ctl.sel[1];
-        prefix |= (uint64_t)b << shift;
-        pmask |= dmask << shift;
-        hb = shift;
-        if (cnt == 1u || hb == 0u) break;
-    }
-    // In-place compaction, 8 keys per thread per chunk: every key of a chunk
-    // is read before the chunk's barrier and survivors are written after it,
-    // to positions below the number of keys read so far.
-    uint32_t running = 0;
-    for (uint32_t base = 0, ch = 0; base < c; base += 8u * kIvfThreads, ++ch) {
-        uint64_t v[8];
-        uint32_t keepmask = 0, n = 0;
-#pragma unroll
-        for (uint32_t u = 0; u < 8u; ++u) {
-            const uint32_t i = base + u * kIvfThreads + tid;
-            v[u] = i < c ? buf[i] : kEmptyKey;
-            const uint64_t mp = v[u] & pmask;
-            if (i < c && mp <= prefix) {
-                keepmask |= 1u << u;
-                ++n;
-                if (mp == prefix) ctl.thr = v[u];
-            }
-        }
-        const uint32_t incl = wave_incl_scan(n, lane);
-        if (lane == kWave - 1) ctl.wsum[ch & 1u][wave] = incl;
-        __syncthreads();
-        uint32_t off = running + incl - n;
-        for (uint32_t w = 0; w < kW; ++w) {
-            const uint32_t ws = ctl.wsum[ch & 1u][w];
-            off += w < wave ? ws : 0u;
-            running += ws;
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < 8u; ++u)
-            if (keepmask & (1u << u)) buf[off++] = v[u];
-    }
-    return ctl.thr;
-}
+static_assert(kIvfThreads == (int)kSelThreads, "block_select and one thread per PQ codeword");
 
 // LUT [M][256] of one probed list: lut[m][j] = sum over s of
 // (res[m*SUB+s] - codebook[m][j][s])^2 as an fmaf chain in s. One codeword
@@ -423,7 +303,7 @@ __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(
 #pragma unroll
             for (uint32_t w = 0; w < kW; ++w) total += ctl.wn[ph][w];
             if (cnt + total > cap) {
-                thr = block_select(buf, cnt, S, ctl);
+                thr = block_select(buf, cnt, S, ctl.bs);
                 cnt = S;
                 wn = 0;
 #pragma unroll
@@ -455,7 +335,7 @@ __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(
     __syncthreads();
     uint32_t kept = cnt;
     if (kept > S) {
-        block_select(buf, kept, S, ctl);
+        block_select(buf, kept, S, ctl.bs);
         kept = S;
         __syncthreads();
     }
@@ -475,7 +355,7 @@ __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(
     // at its rank among the k (or, for k above one key per thread, sorted).
     const uint32_t kout = k < kept ? k : kept;
     if (kept > kout && kout > 0u) {
-        block_select(buf, kept, kout, ctl);
+        block_select(buf, kept, kout, ctl.bs);
         __syncthreads();
     }
     if (kout <= (uint32_t)kIvfThreads) {
