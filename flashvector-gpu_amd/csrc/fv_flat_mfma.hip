@@ -429,7 +429,7 @@ __global__ __launch_bounds__(256) void flat_finalize_kernel(
     const uint32_t hp = next_pow2(have < 2u ? 2u : have);
     for (uint32_t i = have + threadIdx.x; i < hp; i += blockDim.x) s_k[i] = kEmptyKey;
     __syncthreads();
-    if (!f64b && have <= blockDim.x) {
+    if (have <= blockDim.x) {
         // Each key to its rank (keys are distinct).
         const uint64_t mine = threadIdx.x < have ? s_k[threadIdx.x] : kEmptyKey;
         uint32_t rank = 0;
@@ -481,53 +481,149 @@ __global__ __launch_bounds__(256) void flat_finalize_kernel(
             s_r[i] = r;
         }
         __syncthreads();
-        // Pass 2: FP64 distances of the marked candidates, one wavefront per
-        // candidate, lanes splitting the dims.
+        // Pass 2: FP64 distances of the marked candidates. Wave w takes the
+        // entries w, w + W, w + 2W, ... (W waves), up to 4 marked ones at a
+        // time so that their row loads overlap; lanes split the dims and the
+        // sums are reduced across the wave.
         if (s_any) {
+            constexpr int kRows = 4;
             const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-            for (uint32_t i = wave; i < have; i += blockDim.x / kWave) {
-                const uint32_t r = s_r[i];
-                if (r == 0xFFFFFFFFu || !(r & 0x80000000u)) continue;
-                const float* x = X32 + (size_t)(r & 0x7FFFFFFFu) * a.dim;
-                double acc = 0.0, nqd = 0.0, nxd = 0.0;
-                for (uint32_t d = lane; d < a.dim; d += kWave) {
-                    const double qd = qv[d], xd = x[d];
-                    if (METRIC == METRIC_L2) {
-                        const double t = qd - xd;
-                        acc = __builtin_fma(t, t, acc);
-                    } else {
-                        acc = __builtin_fma(qd, xd, acc);
+            const uint32_t nw = blockDim.x / kWave;
+            uint32_t i = wave;
+            while (i < have) {
+                uint32_t idx[kRows], row[kRows];
+                int m = 0;
+                for (; i < have && m < kRows; i += nw) {
+                    const uint32_t r = s_r[i];
+                    if (r == 0xFFFFFFFFu || !(r & 0x80000000u)) continue;
+#pragma unroll
+                    for (int t = 0; t < kRows; ++t)
+                        if (t == m) {
+                            idx[t] = i;
+                            row[t] = r & 0x7FFFFFFFu;
+                        }
+                    ++m;
+                }
+                double acc[kRows], nqd[kRows], nxd[kRows];
+#pragma unroll
+                for (int t = 0; t < kRows; ++t) acc[t] = nqd[t] = nxd[t] = 0.0;
+                // Each lane sums its dims d = lane, lane + 64, ... in order;
+                // 4 of them are loaded per step so that the loads overlap.
+                auto fma_step = [&](double qd, const double (&xd)[kRows]) {
+#pragma unroll
+                    for (int t = 0; t < kRows; ++t) {
+                        if (METRIC == METRIC_L2) {
+                            const double u = qd - xd[t];
+                            acc[t] = __builtin_fma(u, u, acc[t]);
+                        } else {
+                            acc[t] = __builtin_fma(qd, xd[t], acc[t]);
+                            if (METRIC == METRIC_COSINE) {
+                                nqd[t] = __builtin_fma(qd, qd, nqd[t]);
+                                nxd[t] = __builtin_fma(xd[t], xd[t], nxd[t]);
+                            }
+                        }
+                    }
+                };
+                uint32_t d = lane;
+                for (; d + 3u * kWave < a.dim; d += 4u * kWave) {
+                    double qd[4], xd[4][kRows];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        qd[u] = qv[d + u * kWave];
+#pragma unroll
+                        for (int t = 0; t < kRows; ++t)
+                            xd[u][t] = t < m ? (double)X32[(size_t)row[t] * a.dim + d + u * kWave] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) fma_step(qd[u], xd[u]);
+                }
+                for (; d < a.dim; d += kWave) {
+                    double xd[kRows];
+#pragma unroll
+                    for (int t = 0; t < kRows; ++t)
+                        xd[t] = t < m ? (double)X32[(size_t)row[t] * a.dim + d] : 0.0;
+                    fma_step(qv[d], xd);
+                }
+#pragma unroll
+                for (int t = 0; t < kRows; ++t) {
+                    for (int off = 32; off > 0; off >>= 1) {
+                        acc[t] += __shfl_xor(acc[t], off);
                         if (METRIC == METRIC_COSINE) {
-                            nqd = __builtin_fma(qd, qd, nqd);
-                            nxd = __builtin_fma(xd, xd, nxd);
+                            nqd[t] += __shfl_xor(nqd[t], off);
+                            nxd[t] += __shfl_xor(nxd[t], off);
                         }
                     }
                 }
-                for (int off = 32; off > 0; off >>= 1) {
-                    acc += __shfl_xor(acc, off);
-                    if (METRIC == METRIC_COSINE) {
-                        nqd += __shfl_xor(nqd, off);
-                        nxd += __shfl_xor(nxd, off);
-                    }
-                }
                 if (lane == 0) {
-                    double dd;
-                    if (METRIC == METRIC_L2) dd = acc;
-                    else if (METRIC == METRIC_INNER_PRODUCT) dd = -acc;
-                    else dd = 1.0 - acc / (sqrt(nqd) * sqrt(nxd) + 1e-8);
-                    const uint64_t bits = __double_as_longlong(dd);
-                    s_v[i] = (bits & 0x8000000000000000ull) ? ~bits : (bits | 0x8000000000000000ull);
-                    s_r[i] = r & 0x7FFFFFFFu;
+#pragma unroll
+                    for (int t = 0; t < kRows; ++t) {
+                        if (t >= m) break;
+                        double dd;
+                        if (METRIC == METRIC_L2) dd = acc[t];
+                        else if (METRIC == METRIC_INNER_PRODUCT) dd = -acc[t];
+                        else dd = 1.0 - acc[t] / (sqrt(nqd[t]) * sqrt(nxd[t]) + 1e-8);
+                        const uint64_t bits = __double_as_longlong(dd);
+                        s_v[idx[t]] = (bits & 0x8000000000000000ull) ? ~bits : (bits | 0x8000000000000000ull);
+                    }
                 }
             }
         }
         __syncthreads();
         if (s_any) {
-            block_bitonic_sort_pairs(s_v, s_r, hp);
+            // Re-scored candidates (bit 31 of s_r, kept until written out)
+            // form runs of adjacent entries; an entry outside a run is more
+            // than the band away from both neighbours and every re-scored
+            // value is within the error bound of its FP32 value, so each run
+            // only reorders among itself (entries above lim, which never
+            // reach the first k, aside). Short runs are sorted by their first
+            // thread; a run longer than kMaxRun sends the workgroup to a
+            // full sort of (value, row) pairs.
+            constexpr uint32_t kMaxRun = 32;
+            __shared__ uint32_t s_maxrun;
+            auto marked = [&](uint32_t i) {
+                const uint32_t r = s_r[i];
+                return r != 0xFFFFFFFFu && (r & 0x80000000u) != 0u;
+            };
+            if (threadIdx.x == 0) s_maxrun = 0u;
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < have; i += blockDim.x) {
+                if (!marked(i) || (i > 0 && marked(i - 1))) continue;
+                uint32_t j = i + 1;
+                while (j < have && marked(j)) ++j;
+                atomicMax(&s_maxrun, j - i);
+            }
+            __syncthreads();
+            if (s_maxrun <= kMaxRun) {
+                for (uint32_t i = threadIdx.x; i < have; i += blockDim.x) {
+                    if (!marked(i) || (i > 0 && marked(i - 1))) continue;
+                    uint32_t j = i + 1;
+                    while (j < have && marked(j)) ++j;
+                    for (uint32_t a2 = i + 1; a2 < j; ++a2) {
+                        const uint64_t v = s_v[a2];
+                        const uint32_t r = s_r[a2];
+                        uint32_t b2 = a2;
+                        while (b2 > i && (s_v[b2 - 1] > v || (s_v[b2 - 1] == v &&
+                                                              (s_r[b2 - 1] & 0x7FFFFFFFu) > (r & 0x7FFFFFFFu)))) {
+                            s_v[b2] = s_v[b2 - 1];
+                            s_r[b2] = s_r[b2 - 1];
+                            --b2;
+                        }
+                        s_v[b2] = v;
+                        s_r[b2] = r;
+                    }
+                }
+            } else {
+                for (uint32_t i = threadIdx.x; i < hp; i += blockDim.x)
+                    if (s_r[i] != 0xFFFFFFFFu) s_r[i] &= 0x7FFFFFFFu;
+                __syncthreads();
+                block_bitonic_sort_pairs(s_v, s_r, hp);
+            }
+            __syncthreads();
             for (uint32_t i = threadIdx.x; i < kk; i += blockDim.x) {
                 const uint64_t v = s_v[i];
                 const uint64_t bits = (v & 0x8000000000000000ull) ? (v & 0x7fffffffffffffffull) : ~v;
-                s_k[i] = make_key((float)__longlong_as_double(bits), s_r[i]);
+                const uint32_t r = s_r[i] == 0xFFFFFFFFu ? 0xFFFFFFFFu : (s_r[i] & 0x7FFFFFFFu);
+                s_k[i] = make_key((float)__longlong_as_double(bits), r);
             }
             __syncthreads();
         }
