@@ -152,17 +152,18 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
     return v;
 }
 
-// ---- Workgroup radix selection (256 threads) -------------------------------
-constexpr uint32_t kSelThreads = 256;
-constexpr uint32_t kSelWaves = kSelThreads / kWave;
+// ---- Workgroup radix selection ---------------------------------------------
+// Workgroups of 256 to 1024 threads (whole wavefronts).
+constexpr uint32_t kSelThreads = 256;    // the IVF scan's workgroup
+constexpr uint32_t kSelMaxWaves = 16;
 
 // LDS scratch of the selection helpers below (one per workgroup).
 struct BlockSelectLds {
-    uint64_t thr;                   // key stored by block_compact's keep == 2
-    uint64_t red[2 * kSelWaves];    // per-wave smallest / largest key
-    uint32_t sel[3];                // radix round: digit, keys below it, keys in it
-    uint32_t wsum[2][kSelWaves];    // per-wave survivor counts
-    uint32_t hist[2][256];          // digit histograms of consecutive rounds
+    uint64_t thr;                     // key stored by block_compact's keep == 2
+    uint64_t red[2 * kSelMaxWaves];   // per-wave smallest / largest key
+    uint32_t sel[3];                  // radix round: digit, keys below it, keys in it
+    uint32_t wsum[2][kSelMaxWaves];   // per-wave survivor counts
+    uint32_t hist[2][256];            // digit histograms of consecutive rounds
 };
 
 // Radix selection over the c distinct keys buf[0..c) (c > keep >= 1): the
@@ -177,8 +178,9 @@ __device__ inline void block_radix_kth(const uint64_t* buf, uint32_t c, uint32_t
                                        BlockSelectLds& L, uint64_t& pmask_out,
                                        uint64_t& prefix_out) {
     const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const uint32_t nt = blockDim.x, nw = nt / kWave;
     unsigned long long lo = ~0ull, hi = 0;
-    for (uint32_t i = tid; i < c; i += kSelThreads) {
+    for (uint32_t i = tid; i < c; i += nt) {
         const unsigned long long k = buf[i];
         lo = k < lo ? k : lo;
         hi = k > hi ? k : hi;
@@ -191,15 +193,15 @@ __device__ inline void block_radix_kth(const uint64_t* buf, uint32_t c, uint32_t
     }
     if (lane == 0) {
         L.red[wave] = lo;
-        L.red[kSelWaves + wave] = hi;
+        L.red[kSelMaxWaves + wave] = hi;
     }
-    L.hist[0][tid] = 0;
+    for (uint32_t t = tid; t < 256u; t += nt) L.hist[0][t] = 0;
     __syncthreads();
     lo = L.red[0];
-    hi = L.red[kSelWaves];
-    for (uint32_t w = 1; w < kSelWaves; ++w) {
+    hi = L.red[kSelMaxWaves];
+    for (uint32_t w = 1; w < nw; ++w) {
         lo = L.red[w] < lo ? L.red[w] : lo;
-        hi = L.red[kSelWaves + w] > hi ? L.red[kSelWaves + w] : hi;
+        hi = L.red[kSelMaxWaves + w] > hi ? L.red[kSelMaxWaves + w] : hi;
     }
     uint32_t hb = 64u - (uint32_t)__clzll((long long)(lo ^ hi));  // undetermined low bits
     uint64_t pmask = hb >= 64u ? 0ull : ~0ull << hb;
@@ -209,11 +211,11 @@ __device__ inline void block_radix_kth(const uint64_t* buf, uint32_t c, uint32_t
         const uint32_t nb = hb < 8u ? hb : 8u, shift = hb - nb;
         const uint64_t dmask = (1ull << nb) - 1ull;
         uint32_t* h = L.hist[r & 1u];
-        for (uint32_t i = tid; i < c; i += kSelThreads) {
+        for (uint32_t i = tid; i < c; i += nt) {
             const uint64_t k = buf[i];
             if ((k & pmask) == prefix) atomicAdd(&h[(uint32_t)((k >> shift) & dmask)], 1u);
         }
-        L.hist[(r + 1u) & 1u][tid] = 0;
+        for (uint32_t t = tid; t < 256u; t += nt) L.hist[(r + 1u) & 1u][t] = 0;
         __syncthreads();
         if (wave == 0) {
             const uint32_t h0 = h[4 * lane], h1 = h[4 * lane + 1], h2 = h[4 * lane + 2],
@@ -258,13 +260,14 @@ __device__ inline void block_radix_kth(const uint64_t* buf, uint32_t c, uint32_t
 template <class F>
 __device__ inline uint32_t block_compact(uint64_t* buf, uint32_t c, F keepf, BlockSelectLds& L) {
     const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const uint32_t nt = blockDim.x, nw = nt / kWave;
     uint32_t running = 0;
-    for (uint32_t base = 0, ch = 0; base < c; base += 8u * kSelThreads, ++ch) {
+    for (uint32_t base = 0, ch = 0; base < c; base += 8u * nt, ++ch) {
         uint64_t v[8];
         uint32_t keepmask = 0, n = 0;
 #pragma unroll
         for (uint32_t u = 0; u < 8u; ++u) {
-            const uint32_t i = base + u * kSelThreads + tid;
+            const uint32_t i = base + u * nt + tid;
             v[u] = i < c ? buf[i] : ~0ull;
             if (i < c) {
                 const int f = keepf(v[u]);
@@ -279,7 +282,7 @@ __device__ inline uint32_t block_compact(uint64_t* buf, uint32_t c, F keepf, Blo
         if (lane == kWave - 1) L.wsum[ch & 1u][wave] = incl;
         __syncthreads();
         uint32_t off = running + incl - n;
-        for (uint32_t w = 0; w < kSelWaves; ++w) {
+        for (uint32_t w = 0; w < nw; ++w) {
             const uint32_t ws = L.wsum[ch & 1u][w];
             off += w < wave ? ws : 0u;
             running += ws;

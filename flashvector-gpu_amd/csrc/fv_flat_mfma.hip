@@ -355,8 +355,11 @@ __global__ __launch_bounds__(256) void flat_rerank_kernel(
 // in FP64 and reordered by (FP64 distance, row). Outside a cluster the FP32
 // order is the exact order, so only clusters that start before position k are
 // touched.
+// 16 wavefronts: the kernel's LDS allows one workgroup per CU, and the FP64
+// re-scoring's row reads need many waves in flight.
+constexpr int kFinalThreads = 1024;
 template <int METRIC>
-__global__ __launch_bounds__(256) void flat_finalize_kernel(
+__global__ __launch_bounds__(kFinalThreads) void flat_finalize_kernel(
     const uint64_t* __restrict__ sel, const uint32_t* __restrict__ sel_n,
     const float* __restrict__ err, uint32_t hcap, FlatSearchArgs a,
     const float* __restrict__ X32, const float* __restrict__ Qm,
@@ -364,7 +367,7 @@ __global__ __launch_bounds__(256) void flat_finalize_kernel(
     extern __shared__ uint64_t s_k[];
     __shared__ uint32_t s_emax;
     __shared__ BlockSelectLds s_bs;
-    __shared__ uint64_t s_succ[kSelWaves];
+    __shared__ uint64_t s_succ[kSelMaxWaves];
     const uint32_t q = blockIdx.x;
     uint32_t k, S;
     query_k_s(a, q, k, S);
@@ -415,7 +418,7 @@ __global__ __launch_bounds__(256) void flat_finalize_kernel(
             if ((threadIdx.x & (kWave - 1)) == 0) s_succ[threadIdx.x / kWave] = succ;
             __syncthreads();
             succ = s_succ[0];
-            for (uint32_t w = 1; w < kSelWaves; ++w) succ = s_succ[w] < succ ? s_succ[w] : succ;
+            for (uint32_t w = 1; w < blockDim.x / kWave; ++w) succ = s_succ[w] < succ ? s_succ[w] : succ;
             const uint32_t m = block_compact(s_k, have_in, [lim](uint64_t key) {
                 return key_dist(key) <= lim ? 1 : 0;
             }, s_bs);
@@ -1432,13 +1435,13 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
     const dim3 rg((unsigned)(((size_t)nq * p.s_max + kRerankRows - 1) / kRerankRows));
     if (a.metric == METRIC_L2) {
         flat_rerank_kernel<METRIC_L2><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, nq, p.s_max, sel_n, sel_off, sel_off + nq, sel, err);
-        flat_finalize_kernel<METRIC_L2><<<nq, 256, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
+        flat_finalize_kernel<METRIC_L2><<<nq, kFinalThreads, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
     } else if (a.metric == METRIC_INNER_PRODUCT) {
         flat_rerank_kernel<METRIC_INNER_PRODUCT><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, nq, p.s_max, sel_n, sel_off, sel_off + nq, sel, err);
-        flat_finalize_kernel<METRIC_INNER_PRODUCT><<<nq, 256, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
+        flat_finalize_kernel<METRIC_INNER_PRODUCT><<<nq, kFinalThreads, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
     } else {
         flat_rerank_kernel<METRIC_COSINE><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, nq, p.s_max, sel_n, sel_off, sel_off + nq, sel, err);
-        flat_finalize_kernel<METRIC_COSINE><<<nq, 256, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
+        flat_finalize_kernel<METRIC_COSINE><<<nq, kFinalThreads, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
     }
     return hipGetLastError();
 }
