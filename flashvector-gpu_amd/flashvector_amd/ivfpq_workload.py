@@ -43,40 +43,6 @@ def zipf_sizes(total: int, nlist: int, s: float, seed: int) -> np.ndarray:
     return sizes[rng.permutation(nlist)]
 
 
-def _nearest(x, c, chunk=1 << 18):
-    import torch
-    out = torch.empty(x.shape[0], dtype=torch.int64, device=x.device)
-    cn = (c * c).sum(1)
-    for s in range(0, x.shape[0], chunk):
-        xs = x[s:s + chunk]
-        d = cn[None, :] - 2.0 * (xs @ c.T)
-        out[s:s + chunk] = torch.argmin(d, dim=1)
-    return out
-
-
-def _nearest_exact(x, c, chunk=1 << 16):
-    """argmin of exact squared distances (first index on ties)."""
-    import torch
-    out = torch.empty(x.shape[0], dtype=torch.int64, device=x.device)
-    for s in range(0, x.shape[0], chunk):
-        d = ((x[s:s + chunk, None, :] - c[None, :, :]) ** 2).sum(-1)
-        out[s:s + chunk] = torch.argmin(d, dim=1)
-    return out
-
-
-def _kmeans(x, k, iters, gen):
-    import torch
-    idx = torch.randperm(x.shape[0], generator=gen, device=x.device)[:k]
-    c = x[idx].clone()
-    for _ in range(iters):
-        a = _nearest(x, c)
-        sums = torch.zeros_like(c).index_add_(0, a, x)
-        cnt = torch.bincount(a, minlength=k).to(x.dtype)
-        live = cnt > 0
-        c[live] = sums[live] / cnt[live, None]
-    return c
-
-
 def build(n=10_000_000, dim=128, nlist=4096, zipf_s=1.1, m_pq=16, nq=8192, sigma=8.0,
           nprobe_choices=(8, 16, 32, 64), k_choices=(10, 100), seed=21, train=262144,
           iters=10, device="cuda") -> IvfPqWorkload:
@@ -95,22 +61,26 @@ def build(n=10_000_000, dim=128, nlist=4096, zipf_s=1.1, m_pq=16, nq=8192, sigma
         e = min(n, s + (1 << 20))
         noise = torch.randn((e - s, dim), generator=g, device=device) * sigma
         vectors[s:e] = torch.clamp(torch.round(centers[labels[s:e]] + noise), 0.0, 255.0)
-    assign = _nearest(vectors, centers)
+    from . import gpu_build as G
+    # Exact nearest centers and deterministic k-means codebooks (fv_kmeans_*),
+    # so the tables are the same on every run.
+    assign = G.kmeans_assign(vectors, centers).long()
     sub = dim // m_pq
     g.manual_seed(seed + 3)
     samp = torch.randperm(n, generator=g, device=device)[:min(n, train)]
     resid_s = vectors[samp] - centers[assign[samp]]
     books = []
     for m in range(m_pq):
-        books.append(_kmeans(resid_s[:, m * sub:(m + 1) * sub].contiguous(), 256, iters, g))
+        books.append(G.kmeans_fit(resid_s[:, m * sub:(m + 1) * sub].contiguous(), 256, iters,
+                                  seed=seed + 3 + m).centroids)
     codebooks = torch.stack(books)                      # [m_pq, 256, sub]
     codes = torch.empty((n, m_pq), dtype=torch.uint8, device=device)
     for s in range(0, n, 1 << 20):
         e = min(n, s + (1 << 20))
         r = vectors[s:e] - centers[assign[s:e]]
         for m in range(m_pq):
-            codes[s:e, m] = _nearest_exact(r[:, m * sub:(m + 1) * sub].contiguous(),
-                                           codebooks[m]).to(torch.uint8)
+            codes[s:e, m] = G.kmeans_assign(r[:, m * sub:(m + 1) * sub].contiguous(),
+                                            codebooks[m]).to(torch.uint8)
     order = torch.argsort(assign * n + torch.arange(n, device=device))   # ascending id per list
     counts = torch.bincount(assign, minlength=nlist)
     offsets = torch.zeros(nlist + 1, dtype=torch.int64, device=device)
