@@ -32,7 +32,6 @@ namespace {
 
 constexpr int kAR = 64;         // rows per assign workgroup
 constexpr int kAC = 64;         // centroids per tile
-constexpr int kADK = 32;        // dims per LDS stage
 constexpr int kAStride = 68;    // padded LDS row (floats)
 constexpr uint32_t kSeedChunk = 256;
 constexpr uint32_t kEmptyDraws = 500;
@@ -48,10 +47,15 @@ __device__ __forceinline__ float km_l2(const float* __restrict__ a, const float*
     return acc;
 }
 
+// DK dims per LDS stage (8, 16 or 32): sub-space rows of 8 dims stage no
+// padding.
+template <int DK>
 __global__ __launch_bounds__(256) void km_assign_kernel(
     const float* __restrict__ X, uint32_t n, uint32_t dim, uint32_t ld,
     const float* __restrict__ C, uint32_t k, uint32_t* __restrict__ assign,
     uint8_t* __restrict__ codes, uint32_t code_stride) {
+    constexpr int kADK = DK;
+    constexpr int kRowsPerPass = 256 / (DK / 4);  // rows staged per pass of the loop below
     __shared__ __attribute__((aligned(16))) float sx[kADK * kAStride];
     __shared__ __attribute__((aligned(16))) float sc[kADK * kAStride];
     __shared__ uint64_t s_best[kAR][17];
@@ -69,9 +73,10 @@ __global__ __launch_bounds__(256) void km_assign_kernel(
             for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
         for (uint32_t d0 = 0; d0 < dim; d0 += kADK) {
             __syncthreads();
-            for (int it = 0; it < 2; ++it) {
-                const int qq = (tid >> 3) + it * 32;
-                const int dd = (tid & 7) * 4;
+            for (int it = 0; it * kRowsPerPass < kAR; ++it) {
+                const int qq = tid / (DK / 4) + it * kRowsPerPass;
+                const int dd = (tid % (DK / 4)) * 4;
+                if (qq >= kAR) continue;
                 const uint32_t rg = r0 + qq, cg = c0 + qq;
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
@@ -442,8 +447,13 @@ KmLayout km_layout(uint32_t n, uint32_t dim, uint32_t k) {
 hipError_t assign_launch(const float* X, uint32_t n, uint32_t dim, uint32_t ld, const float* C,
                          uint32_t k, uint32_t* assign, uint8_t* codes, uint32_t code_stride,
                          hipStream_t s) {
-    km_assign_kernel<<<blocks_for(n, kAR), 256, 0, s>>>(X, n, dim, ld, C, k, assign, codes,
-                                                         code_stride);
+    const unsigned g = blocks_for(n, kAR);
+    if (dim <= 8)
+        km_assign_kernel<8><<<g, 256, 0, s>>>(X, n, dim, ld, C, k, assign, codes, code_stride);
+    else if (dim <= 16)
+        km_assign_kernel<16><<<g, 256, 0, s>>>(X, n, dim, ld, C, k, assign, codes, code_stride);
+    else
+        km_assign_kernel<32><<<g, 256, 0, s>>>(X, n, dim, ld, C, k, assign, codes, code_stride);
     return hipGetLastError();
 }
 
