@@ -455,7 +455,11 @@ IvfPqPlan plan_ivfpq(const IvfPqGpuTables* t, uint32_t nq, uint32_t max_nprobe,
     uint32_t S = shortlist < max_k ? max_k : shortlist;
     p.shortlist = S;
     p.max_nprobe = max_nprobe;
-    p.cap = next_pow2(S + kEntriesPerCheck);
+    // Room for S kept keys plus one check's keys, and for the final sort of
+    // up to next_pow2(k) keys (k <= 1024).
+    p.cap = (S + kEntriesPerCheck + 63u) / 64u * 64u;
+    const uint32_t sort_room = next_pow2(S < 1024u ? S : 1024u);
+    if (p.cap < sort_room) p.cap = sort_room;
     p.npow = next_pow2(t->nlist < 2u ? 2u : t->nlist);
     const size_t lut_bytes = (size_t)t->m_pq * 256 * sizeof(float);
     p.gbuf = lut_bytes + (size_t)p.cap * 8 > 64 * 1024;
