@@ -1131,3 +1131,178 @@ int fvh_load_snapshot(FvEngine* e, const char* path, uint32_t flat_dtype) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Micro-batcher (crates/server/src/batcher.rs:80-262): one worker thread
+// coalesces single-query submissions into batched searches.
+// ---------------------------------------------------------------------------
+
+#include <condition_variable>
+#include <deque>
+#include <thread>
+
+namespace fvh {
+
+struct BatchRequest {
+    const float* query;  // caller's buffer; valid until its submit returns
+    uint32_t top_k;
+    GpuSearchResult* out;
+    double latency_us = 0.0;
+    int rc = 0;
+    bool done = false;
+};
+
+}  // namespace fvh
+
+struct FvBatcher {
+    FvBatchExecutor exec = nullptr;
+    void* user = nullptr;
+    uint32_t dim = 0;
+    uint32_t max_batch = 1;
+    std::chrono::microseconds max_wait{0};
+    std::mutex mu;
+    std::condition_variable work_cv;  // queue not empty, or stopping
+    std::condition_variable done_cv;  // some request finished
+    std::deque<fvh::BatchRequest*> queue;
+    bool stop = false;
+    std::atomic<uint64_t> total_requests{0};
+    std::atomic<uint64_t> total_batches{0};
+    std::thread worker;
+};
+
+namespace fvh {
+
+// Runs the queries of one launch key as one batch and fills their results.
+static void run_group(FvBatcher* b, const std::vector<BatchRequest*>& group,
+                      std::vector<float>& qbuf, std::vector<GpuSearchResult>& rbuf) {
+    const uint32_t nq = (uint32_t)group.size(), k = group[0]->top_k, dim = b->dim;
+    qbuf.resize((size_t)nq * dim);
+    rbuf.resize((size_t)nq * k);
+    for (uint32_t i = 0; i < nq; ++i)
+        std::memcpy(qbuf.data() + (size_t)i * dim, group[i]->query, (size_t)dim * sizeof(float));
+    b->total_requests.fetch_add(nq, std::memory_order_relaxed);
+    b->total_batches.fetch_add(1, std::memory_order_relaxed);
+    const auto t0 = std::chrono::steady_clock::now();
+    const int rc = b->exec(b->user, k, qbuf.data(), nq, rbuf.data());
+    const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+    for (uint32_t i = 0; i < nq; ++i) {
+        if (rc == 0)
+            std::memcpy(group[i]->out, rbuf.data() + (size_t)i * k, (size_t)k * sizeof(GpuSearchResult));
+        group[i]->rc = rc;
+        group[i]->latency_us = us;
+    }
+}
+
+static void batcher_loop(FvBatcher* b) {
+    std::vector<BatchRequest*> items;
+    std::vector<std::pair<uint32_t, std::vector<BatchRequest*>>> groups;
+    std::vector<float> qbuf;
+    std::vector<GpuSearchResult> rbuf;
+    std::unique_lock<std::mutex> lk(b->mu);
+    for (;;) {
+        b->work_cv.wait(lk, [b] { return b->stop || !b->queue.empty(); });
+        if (b->queue.empty()) return;  // stopping and drained
+        items.clear();
+        items.push_back(b->queue.front());
+        b->queue.pop_front();
+        const auto deadline = std::chrono::steady_clock::now() + b->max_wait;
+        while (items.size() < b->max_batch) {
+            if (!b->queue.empty()) {
+                items.push_back(b->queue.front());
+                b->queue.pop_front();
+                continue;
+            }
+            if (b->stop || std::chrono::steady_clock::now() >= deadline) break;
+            b->work_cv.wait_until(lk, deadline);
+        }
+        lk.unlock();
+        // Partition by launch key, groups in the order of their first query.
+        groups.clear();
+        for (BatchRequest* r : items) {
+            auto it = std::find_if(groups.begin(), groups.end(),
+                                   [r](const auto& g) { return g.first == r->top_k; });
+            if (it == groups.end()) groups.push_back({r->top_k, {r}});
+            else it->second.push_back(r);
+        }
+        for (const auto& g : groups) {
+            run_group(b, g.second, qbuf, rbuf);
+            lk.lock();
+            for (BatchRequest* r : g.second) r->done = true;
+            lk.unlock();
+            b->done_cv.notify_all();
+        }
+        lk.lock();
+    }
+}
+
+static int engine_executor(void* user, uint32_t top_k, const float* queries, uint32_t nq,
+                           GpuSearchResult* out) {
+    return fvh_search(static_cast<FvEngine*>(user), queries, nq, top_k, nullptr, 0, out);
+}
+
+}  // namespace fvh
+
+extern "C" {
+
+int fvh_batcher_create_with_executor(FvBatchExecutor exec, void* user, uint32_t dim,
+                                     uint32_t max_wait_us, uint32_t max_batch, FvBatcher** out) {
+    if (!exec || !out) return -1;
+    *out = nullptr;
+    if (dim == 0) return (int)hipErrorInvalidValue;
+    auto b = std::make_unique<FvBatcher>();
+    b->exec = exec;
+    b->user = user;
+    b->dim = dim;
+    b->max_batch = max_batch ? max_batch : 1u;  // batcher.rs:136
+    b->max_wait = std::chrono::microseconds(max_wait_us);
+    b->worker = std::thread(fvh::batcher_loop, b.get());
+    *out = b.release();
+    return 0;
+}
+
+int fvh_batcher_create(FvEngine* e, uint32_t max_wait_us, uint32_t max_batch, FvBatcher** out) {
+    if (!e || !out) return -1;
+    return fvh_batcher_create_with_executor(fvh::engine_executor, e, e->cfg.dim, max_wait_us,
+                                            max_batch, out);
+}
+
+int fvh_batcher_submit(FvBatcher* b, const float* h_query, uint32_t top_k, GpuSearchResult* h_out,
+                       double* latency_us) {
+    if (!b || !h_query || !h_out) return -1;
+    if (top_k == 0) return (int)hipErrorInvalidValue;
+    fvh::BatchRequest r;
+    r.query = h_query;
+    r.top_k = top_k;
+    r.out = h_out;
+    std::unique_lock<std::mutex> lk(b->mu);
+    if (b->stop) return FVH_ERR_SHUTDOWN;
+    b->queue.push_back(&r);
+    b->work_cv.notify_one();
+    b->done_cv.wait(lk, [&r] { return r.done; });
+    if (latency_us) *latency_us = r.latency_us;
+    return r.rc;
+}
+
+int fvh_batcher_stats(const FvBatcher* b, uint64_t* total_requests, uint64_t* total_batches,
+                      double* avg_batch_size) {
+    if (!b) return -1;
+    const uint64_t nr = b->total_requests.load(std::memory_order_relaxed);
+    const uint64_t nb = b->total_batches.load(std::memory_order_relaxed);
+    if (total_requests) *total_requests = nr;
+    if (total_batches) *total_batches = nb;
+    if (avg_batch_size) *avg_batch_size = nb ? (double)nr / (double)nb : 0.0;
+    return 0;
+}
+
+void fvh_batcher_destroy(FvBatcher* b) {
+    if (!b) return;
+    {
+        std::lock_guard<std::mutex> lk(b->mu);
+        b->stop = true;
+    }
+    b->work_cv.notify_all();
+    if (b->worker.joinable()) b->worker.join();
+    delete b;
+}
+
+}  // extern "C"

@@ -180,6 +180,11 @@ SIGNATURES = {
     "fvh_snapshot_close": (None, [P]),
     "fvh_save_snapshot": (I, [P, ctypes.c_char_p]),
     "fvh_load_snapshot": (I, [P, ctypes.c_char_p, U]),
+    "fvh_batcher_create": (I, [P, U, U, P]),
+    "fvh_batcher_create_with_executor": (I, [P, P, U, U, U, P]),
+    "fvh_batcher_submit": (I, [P, P, U, P, P]),
+    "fvh_batcher_stats": (I, [P, P, P, P]),
+    "fvh_batcher_destroy": (None, [P]),
     "fv_topk_ip_backward": (I, [P, U, P, U, U, P, U, P, P, ctypes.c_float, ctypes.c_float, P, P,
                                 P, P, P]),
     "fv_kmeans_assign": (I, [P, U, U, U, P, U, P, P, U, P]),

@@ -138,6 +138,51 @@ int fvh_snapshot_section(const FvSnapshot* s, const char* name, const void** dat
                          uint64_t* byte_len, uint64_t* len, char* dtype8);
 void fvh_snapshot_close(FvSnapshot* s);
 
+/* ---- Micro-batcher ---------------------------------------------------------
+ * Counterpart of the reference server's SearchBatcher
+ * (crates/server/src/batcher.rs:80-262): callers submit one query at a time
+ * from any number of threads; a worker thread takes the first waiting query,
+ * keeps collecting until max_batch queries are held or max_wait_us has passed
+ * since the first, splits what it holds by launch key (top_k here: the
+ * engine fixes metric, nprobe and shortlist; groups keep the order of their
+ * first query), runs each group as one batched search and hands every caller
+ * its own k results. A failed batch returns its error code to every query of
+ * the group. fvh_batcher_destroy serves the queries already queued, then
+ * stops the worker; it must not run concurrently with fvh_batcher_submit.
+ * Errors: -1 NULL argument, hipErrorInvalidValue for top_k 0,
+ * FVH_ERR_SHUTDOWN for a submit after destroy began, else the executor's. */
+#define FVH_ERR_SHUTDOWN (-4)
+
+typedef struct FvBatcher FvBatcher;
+
+/* Runs num_queries queries ([num_queries * dim] f32, host) at top_k and
+ * writes num_queries * top_k results (host); returns 0 or an error code. */
+typedef int (*FvBatchExecutor)(void* user, uint32_t top_k, const float* queries,
+                               uint32_t num_queries, GpuSearchResult* out);
+
+/* SearchBatcher::for_index (batcher.rs:89-124; the reference server uses
+ * max_wait 350 us, max_batch 64): the executor is fvh_search on `e`, which
+ * must outlive the batcher. max_batch 0 counts as 1. */
+int fvh_batcher_create(FvEngine* e, uint32_t max_wait_us, uint32_t max_batch, FvBatcher** out);
+
+/* SearchBatcher::with_executor (batcher.rs:126-229) for any executor of
+ * queries of `dim` floats. */
+int fvh_batcher_create_with_executor(FvBatchExecutor exec, void* user, uint32_t dim,
+                                     uint32_t max_wait_us, uint32_t max_batch, FvBatcher** out);
+
+/* SearchBatcher::submit (batcher.rs:231-247): blocks until the query's batch
+ * ran. h_query holds dim floats, h_out receives top_k results; latency_us
+ * (may be NULL) receives the batch's executor time in microseconds. */
+int fvh_batcher_submit(FvBatcher* b, const float* h_query, uint32_t top_k,
+                       GpuSearchResult* h_out, double* latency_us);
+
+/* SearchBatcher::stats (batcher.rs:249-262); avg_batch_size is 0 before the
+ * first batch. Any output may be NULL. */
+int fvh_batcher_stats(const FvBatcher* b, uint64_t* total_requests, uint64_t* total_batches,
+                      double* avg_batch_size);
+
+void fvh_batcher_destroy(FvBatcher* b);
+
 /* Engine state as a snapshot (index.rs:845-1031): FP32 rows, logical ids,
  * slot validity words, IVF-PQ tables, FP16 rows padded to 16 and their
  * norms (flat FP16 engines; BF16 / FP8 codes go to the extra sections
