@@ -75,7 +75,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--alt-valid-frac", type=float, default=0.95,
                     help="second mask density reported beside the headline (0 = skip)")
-    ap.add_argument("--workload", choices=["flat", "ivfpq", "fp8", "backward", "build"],
+    ap.add_argument("--workload", choices=["flat", "ivfpq", "fp8", "backward", "build", "exact"],
                     default="flat",
                     help="flat: the headline matrix-core scan; ivfpq: 10M-row IVF-PQ uneven batch; "
                          "build: IVF-PQ table construction on the GPU")
@@ -101,6 +101,8 @@ def main():
         return bench_backward(args)
     if args.workload == "build":
         return bench_build(args)
+    if args.workload == "exact":
+        return bench_exact(args)
     args.n = args.n or 1_000_000
     args.nq = args.nq or 10_000
 
@@ -386,6 +388,90 @@ def bench_ivfpq(args):
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
+
+
+def bench_exact(args):
+    """Exact FP32 flat scan through the reference's entry point
+    (cuda_flat_scan_batch): N=100,000 x d=128 integer mixture rows, Q=1,000
+    queries, k=10, L2, no mask. One step is one batched call over all Q
+    queries (its scratch comes from hipMallocAsync, as the reference's does).
+    Every query's ids and distances are checked bit-exact against the CPU
+    oracle, which is also the CPU baseline (all 1,000 queries)."""
+    import numpy as np
+    import torch
+
+    from flashvector_amd import lib as L
+    from flashvector_amd import search as F
+    from flashvector_amd import synthetic as S
+
+    torch.cuda.set_device(0)
+    n, nq, dim, k = args.n or 100_000, args.nq or 1_000, 128, args.k
+    log(f"generating workload n={n} nq={nq} d={dim}")
+    db, qs = S.flat_workload(n, nq, dim, 256, seeds=S.SEEDS["flat_fp32_100k"])
+    d_db = torch.from_numpy(db).cuda()
+    d_q = torch.from_numpy(qs).cuda()
+    out = L.result_tensor(nq, k, d_q.device)
+    if args.calibrate_copy:
+        src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        torch.empty_like(src).copy_(src)
+        del src
+
+    def search():
+        F.flat_scan_batch(d_db, d_q, k, "l2", out=out)
+
+    log("warmup")
+    for _ in range(args.warmup):
+        search()
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} steps")
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with L.KernelTimer() as kt:
+        torch.cuda.synchronize()
+        start.record()
+        for _ in range(args.steps):
+            search()
+        stop.record()
+        torch.cuda.synchronize()
+        scan_ms, launches = kt.read()
+    step_ms = start.elapsed_time(stop) / args.steps
+    scan_avg = scan_ms / max(launches, 1)
+    ops = 3.0 * nq * n * dim                # sub, mul, add per (query, row, dim), not fused
+    achieved = ops / (scan_avg / 1e3) / 1e12
+    ids, dists = L.split_results(out)
+
+    import oracle as O
+    O.build()
+    cores = len(os.sched_getaffinity(0))
+    log(f"CPU oracle on all {nq} queries, {cores} threads")
+    t0 = time.perf_counter()
+    ref_ids, ref_d = O.flat_scan_batch(db, qs, k, "l2")
+    cpu_s = time.perf_counter() - t0
+    exact = bool(np.array_equal(ids, ref_ids) and
+                 np.array_equal(dists.view(np.uint32), ref_d.view(np.uint32)))
+    res = {
+        "metric": "QPS, exact FP32 flat L2 top-10 (100k x 128, Q=1000), bit-exact vs CPU oracle",
+        "value": round(nq / (step_ms / 1e3), 1), "unit": "queries/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded integer Gaussian mixture, 256 clusters)",
+        "config": {"workload": "flat_l2_exact_fp32", "n": n, "dim": dim, "num_queries": nq,
+                   "top_k": k, "entry_point": "cuda_flat_scan_batch",
+                   "parallelism": "single_gpu"},
+        "bit_exact_vs_oracle": exact, "queries_checked": nq,
+        "roofline": {"bound": "fp32-valu", "achieved": round(achieved, 2),
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                     "traffic": measured_traffic("flat_exact_scan_kernel"),
+                     "kernel": "flat_exact_scan_kernel", "kernel_avg_ms": round(scan_avg, 4),
+                     "launches": launches, "algorithmic_ops_per_launch": ops},
+        "cpu_baseline": {"value": round(nq / cpu_s, 1), "unit": "queries/s", "cores": cores,
+                         "kind": "port",
+                         "sample": f"all {nq} queries against all {n} rows, exact FP32 "
+                                   f"(oracle/fv_oracle.c, OpenMP over queries), {cpu_s:.2f} s"},
+    }
+    print(json.dumps(res), flush=True)
+    if not exact:
+        sys.exit(1)
 
 
 def bench_build(args):

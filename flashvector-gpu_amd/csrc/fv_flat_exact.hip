@@ -6,8 +6,11 @@
 // every thread holds a 4x4 register tile of exact distances accumulated in the
 // reference CPU path's order (sequential over d, non-fused), and a candidate
 // that beats its query's running threshold is appended to a per-(chunk, query)
-// buffer. Full buffers are pruned by one wavefront (LDS bitonic sort). A second
-// kernel merges the per-chunk sorted lists of each query.
+// buffer. Tiles are staged through LDS 32 dims at a time; the next stage's
+// rows and queries are loaded into registers (128-bit loads) while the
+// current stage is computed. Buffers that could overflow are cut by one
+// wavefront (bisection over the distance, exact sort on heavy ties). A second
+// kernel merges the per-chunk lists of each query.
 #include "fv_flat_exact.hpp"
 
 namespace fv {
@@ -17,6 +20,10 @@ constexpr int kTN = 64;        // rows per tile
 constexpr int kDK = 32;        // dims per LDS stage
 constexpr int kLdsStride = 68; // padded LDS row (floats), keeps float4 alignment
 constexpr int kScanThreads = 256;
+// Static LDS of the scan kernel: staged queries and rows, counts, thresholds.
+constexpr uint32_t kExactStaticLds = 2u * kDK * kLdsStride * 4u + kTQ * 8u;
+constexpr uint32_t kCUs = 256;           // MI355X compute units
+constexpr uint32_t kLdsPerCU = 160u * 1024u;
 
 // Row norms (cosine): sequential non-fused sums, bit-identical to the sums the
 // oracle forms inside its per-pair loop.
@@ -34,6 +41,75 @@ __global__ void row_norms_kernel(const float* __restrict__ x, uint32_t n, uint32
     out[i] = acc;
 }
 
+// One stage (64 rows or queries x 32 dims) held in registers: thread t holds
+// dims 4*(t&7) .. +3 of tile rows t>>3 and (t>>3)+32, zero past the row range
+// or the dimension.
+struct ExactStage {
+    float x[2][4];
+    float q[2][4];
+};
+
+__device__ __forceinline__ void load_exact_stage(const float* __restrict__ X, const float* __restrict__ Qm,
+                                                 uint32_t dim, uint32_t nq, uint32_t q0, uint32_t r0,
+                                                 uint32_t r_end, uint32_t d0, bool vec, int tid,
+                                                 ExactStage& s) {
+    const uint32_t dg = d0 + (uint32_t)(tid & 7) * 4u;
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const uint32_t rr = (uint32_t)(tid >> 3) + (uint32_t)it * 32u;
+        const uint32_t rg = r0 + rr, qg = q0 + rr;
+        if (vec) {
+            // dim is a multiple of 4, so a vector that starts in range ends in range.
+            float4 xv = make_float4(0.0f, 0.0f, 0.0f, 0.0f), qv = xv;
+            if (rg < r_end && dg < dim) xv = *reinterpret_cast<const float4*>(X + (size_t)rg * dim + dg);
+            if (qg < nq && dg < dim) qv = *reinterpret_cast<const float4*>(Qm + (size_t)qg * dim + dg);
+            s.x[it][0] = xv.x; s.x[it][1] = xv.y; s.x[it][2] = xv.z; s.x[it][3] = xv.w;
+            s.q[it][0] = qv.x; s.q[it][1] = qv.y; s.q[it][2] = qv.z; s.q[it][3] = qv.w;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                s.x[it][c] = (rg < r_end && dg + c < dim) ? X[(size_t)rg * dim + dg + c] : 0.0f;
+                s.q[it][c] = (qg < nq && dg + c < dim) ? Qm[(size_t)qg * dim + dg + c] : 0.0f;
+            }
+        }
+    }
+}
+
+// Transposed into LDS: sx[d][row], sq[d][query].
+__device__ __forceinline__ void store_exact_stage(float* sq, float* sx, int tid, const ExactStage& s) {
+    const int dd = (tid & 7) * 4;
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const int rr = (tid >> 3) + it * 32;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            sx[(dd + c) * kLdsStride + rr] = s.x[it][c];
+            sq[(dd + c) * kLdsStride + rr] = s.q[it][c];
+        }
+    }
+}
+
+// Cuts one (chunk, query) buffer of c keys (c <= cap, cap >= keep + 128) so
+// that another tile of appends fits, by one wavefront: bisection keeps at
+// most keep + 64 keys at or below a bound; when ties at the bound keep more,
+// an exact sort keeps `keep`. *bound receives an ord(distance) that no later
+// row of the chunk may reach (0xFFFFFFFF: none yet).
+__device__ __forceinline__ uint32_t prune_exact_buffer(uint64_t* buf, uint32_t c, uint32_t keep,
+                                                       uint32_t cap, uint64_t* scratch, int lane,
+                                                       uint32_t* bound) {
+    uint32_t cnt = c;
+    *bound = 0xFFFFFFFFu;
+    if (cap <= 512u) {
+        cnt = cap <= 256u ? wave_bisect_prune<4>(buf, c, keep, lane, bound)
+                          : wave_bisect_prune<8>(buf, c, keep, lane, bound);
+        if (cnt + kTN <= cap) return cnt;
+        __threadfence_block();  // the compacted keys are read back by other lanes
+    }
+    const uint32_t kept = wave_prune(buf, cnt, keep, scratch, lane);
+    *bound = kept == keep ? (uint32_t)(scratch[keep - 1] >> 32) : 0xFFFFFFFFu;
+    return kept;
+}
+
 // FMA = true accumulates L2 as fmaf(diff, diff, acc), the order of the IVF
 // coarse quantizer (kernels/src/ivf_rerank.cuh:81-84); false is the reference
 // CPU path's non-fused order.
@@ -48,7 +124,6 @@ __global__ __launch_bounds__(kScanThreads) void flat_exact_scan_kernel(
     __shared__ uint32_t s_cnt[kTQ];
     __shared__ float s_thr[kTQ];
     extern __shared__ uint64_t s_scratch[];  // 4 waves x cap keys
-
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wave = tid / kWave;
@@ -59,6 +134,8 @@ __global__ __launch_bounds__(kScanThreads) void flat_exact_scan_kernel(
     const int tq = (tid & 15) * 4;
     const int tr = (tid >> 4) * 4;
     uint64_t* my_scratch = s_scratch + (size_t)wave * cap;
+    const bool vec = (dim & 3u) == 0u && (reinterpret_cast<uintptr_t>(X) & 15u) == 0u &&
+                     (reinterpret_cast<uintptr_t>(Qm) & 15u) == 0u;
 
     if (tid < kTQ) {
         s_cnt[tid] = 0;
@@ -68,32 +145,25 @@ __global__ __launch_bounds__(kScanThreads) void flat_exact_scan_kernel(
     if (METRIC == METRIC_COSINE) {
         for (int i = 0; i < 4; ++i) qn[i] = (q0 + tq + i < nq) ? qnorm[q0 + tq + i] : 1.0f;
     }
-
+    ExactStage st;
+    if (r_begin < r_end) load_exact_stage(X, Qm, dim, nq, q0, r_begin, r_end, 0u, vec, tid, st);
     for (uint32_t r0 = r_begin; r0 < r_end; r0 += kTN) {
         float acc[4][4];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
-
         for (uint32_t d0 = 0; d0 < dim; d0 += kDK) {
             __syncthreads();
-            // Stage 64 queries x 32 dims and 64 rows x 32 dims, transposed.
-            for (int it = 0; it < 2; ++it) {
-                int qq = (tid >> 3) + it * 32;
-                int dd = (tid & 7) * 4;
-                uint32_t qg = q0 + qq;
-                uint32_t rg = r0 + qq;
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    uint32_t dg = d0 + dd + c;
-                    sq[(dd + c) * kLdsStride + qq] =
-                        (qg < nq && dg < dim) ? Qm[(size_t)qg * dim + dg] : 0.0f;
-                    sx[(dd + c) * kLdsStride + qq] =
-                        (rg < r_end && dg < dim) ? X[(size_t)rg * dim + dg] : 0.0f;
-                }
-            }
+            store_exact_stage(sq, sx, tid, st);
             __syncthreads();
+            // Next stage: the next dims of this tile, else the next tile's first.
+            uint32_t nd0 = d0 + kDK, nr0 = r0;
+            if (nd0 >= dim) {
+                nd0 = 0;
+                nr0 = r0 + kTN;
+            }
+            if (nr0 < r_end) load_exact_stage(X, Qm, dim, nq, q0, nr0, r_end, nd0, vec, tid, st);
             const int dlim = min((int)kDK, (int)(dim - d0));
             for (int dd = 0; dd < dlim; ++dd) {
                 float4 qv = *reinterpret_cast<const float4*>(&sq[dd * kLdsStride + tq]);
@@ -118,7 +188,6 @@ __global__ __launch_bounds__(kScanThreads) void flat_exact_scan_kernel(
                     }
             }
         }
-
         // Filter against each query's threshold and append survivors.
         bool row_ok[4];
         float xn[4];
@@ -146,28 +215,36 @@ __global__ __launch_bounds__(kScanThreads) void flat_exact_scan_kernel(
             }
         }
         __syncthreads();
-        // Prune every buffer that could overflow during the next tile.
+        // Cut every buffer that could overflow during the next tile.
         for (int qq = wave; qq < kTQ; qq += 4) {
-            uint32_t qg = q0 + qq;
+            const uint32_t qg = q0 + qq;
             if (qg >= nq) break;
-            uint32_t c = s_cnt[qq];
-            if (c + kTN > cap) {
-                uint64_t* buf = cand + ((size_t)chunk * nq + qg) * cap;
-                uint32_t kept = wave_prune(buf, c, keep, my_scratch, lane);
-                if (lane == 0) {
-                    s_cnt[qq] = kept;
-                    if (kept == keep) s_thr[qq] = key_dist(my_scratch[keep - 1]);
-                }
+            const uint32_t c = s_cnt[qq];
+            if (c + kTN <= cap) continue;
+            uint64_t* buf = cand + ((size_t)chunk * nq + qg) * cap;
+            uint32_t t;
+            const uint32_t kept = prune_exact_buffer(buf, c, keep, cap, my_scratch, lane, &t);
+            if (lane == 0) {
+                s_cnt[qq] = kept;
+                if (t != 0xFFFFFFFFu) s_thr[qq] = fminf(s_thr[qq], unord_f32(t));
             }
+            wave_lds_sync();
         }
     }
     __syncthreads();
-    // Emit each query's sorted `keep` best of this chunk.
+    // Emit each query's sorted `keep` best of this chunk: bisection first, so
+    // that at most keep + 64 keys (or the tied ones) are sorted.
     for (int qq = wave; qq < kTQ; qq += 4) {
         uint32_t qg = q0 + qq;
         if (qg >= nq) break;
         uint32_t c = s_cnt[qq];
         uint64_t* buf = cand + ((size_t)chunk * nq + qg) * cap;
+        if (c > keep && cap <= 512u) {
+            uint32_t t;
+            c = cap <= 256u ? wave_bisect_prune<4>(buf, c, keep, lane, &t)
+                            : wave_bisect_prune<8>(buf, c, keep, lane, &t);
+            __threadfence_block();
+        }
         uint32_t kept = wave_prune(buf, c, keep, my_scratch, lane);
         uint64_t* out = partial + ((size_t)chunk * nq + qg) * keep;
         for (uint32_t i = lane; i < keep; i += kWave) out[i] = i < kept ? my_scratch[i] : kEmptyKey;
@@ -212,9 +289,21 @@ __global__ __launch_bounds__(256) void merge_partials_kernel(
 FlatExactPlan plan_flat_exact(uint32_t n, uint32_t dim, uint32_t nq, uint32_t top_k, int metric) {
     FlatExactPlan p{};
     p.keep = top_k;
-    p.cap = next_pow2(top_k + kTN) < 128u ? 128u : next_pow2(top_k + kTN);
+    // Bisection prunes leave up to keep + 64 keys; one more tile adds 64.
+    p.cap = next_pow2(top_k + 2u * kTN);
     uint32_t qblocks = (nq + kTQ - 1) / kTQ;
-    uint32_t want = (2048u + qblocks - 1) / qblocks;
+    // About 2048 workgroups, rounded to whole rounds of the workgroups the
+    // CUs hold at once (LDS-bound; at most 8 of 4 wavefronts per CU), so that
+    // the last round is not a sparse one.
+    const uint32_t lds = kExactStaticLds + 4u * p.cap * 8u;
+    uint32_t per_cu = kLdsPerCU / lds;
+    if (per_cu > 8u) per_cu = 8u;
+    if (per_cu < 1u) per_cu = 1u;
+    const uint32_t slots = kCUs * per_cu;
+    const uint32_t want_total = qblocks * ((2048u + qblocks - 1) / qblocks);
+    uint32_t rounds = (want_total + slots / 2) / slots;
+    if (rounds < 1u) rounds = 1u;
+    uint32_t want = rounds * slots / qblocks;
     uint32_t max_by_rows = (n + 255u) / 256u;
     uint32_t max_by_merge = 8192u / top_k;
     uint32_t chunks = want;

@@ -64,6 +64,17 @@ def test_flat_random_vs_oracle(torch, n, nq, dim, k, metric):
     assert_same(run_flat(torch, db, qs, k, metric), O.flat_scan_batch(db, qs, k, metric))
 
 
+def test_full_size_100k_q1000_bit_exact(torch):
+    """The CPU-runnable workload at full size (100,000 x 128 integer mixture
+    rows, 1,000 queries, k=10, seeds of flat_fp32_100k): every query's ids and
+    distances equal the oracle's; with a 50% mask and k=100 too."""
+    db, qs = S.flat_workload(100_000, 1_000, 128, 256, seeds=S.SEEDS["flat_fp32_100k"])
+    assert_same(run_flat(torch, db, qs, 10), O.flat_scan_batch(db, qs, 10))
+    mask = S.random_valid_mask(100_000, 0.5, S.SEEDS["flat_fp32_100k"][2])
+    assert_same(run_flat(torch, db, qs[:200], 100, mask=mask),
+                O.flat_scan_batch(db, qs[:200], 100, valid_mask=mask))
+
+
 def test_flat_non_integer_data_bit_exact(torch):
     rng = np.random.default_rng(4)
     db = rng.normal(size=(6000, 96)).astype(np.float32)
