@@ -21,7 +21,12 @@ constexpr int kDK = 32;        // dims per LDS stage
 constexpr int kLdsStride = 68; // padded LDS row (floats), keeps float4 alignment
 constexpr int kScanThreads = 256;
 // Static LDS of the scan kernel: staged queries and rows, counts, thresholds.
-constexpr uint32_t kExactStaticLds = 2u * kDK * kLdsStride * 4u + kTQ * 8u;
+constexpr uint32_t kExactStageFloats = 2u * kDK * kLdsStride;
+constexpr uint32_t kExactStaticLds = kExactStageFloats * 4u + kTQ * 8u;
+// Largest buffer capacity whose 4 wave-private sort scratches fit in the
+// staging area (they are used only between tiles, when it is idle).
+constexpr uint32_t kExactAliasCap = kExactStageFloats * 4u / (4u * 8u);
+constexpr uint32_t exact_dynamic_lds(uint32_t cap) { return cap <= kExactAliasCap ? 0u : 4u * cap * 8u; }
 constexpr uint32_t kCUs = 256;           // MI355X compute units
 constexpr uint32_t kLdsPerCU = 160u * 1024u;
 
@@ -119,11 +124,12 @@ __global__ __launch_bounds__(kScanThreads) void flat_exact_scan_kernel(
     uint32_t nq, const float* __restrict__ xnorm, const float* __restrict__ qnorm,
     const uint32_t* __restrict__ mask, uint32_t keep, uint32_t cap, uint32_t rows_per_chunk,
     uint64_t* __restrict__ cand, uint64_t* __restrict__ partial) {
-    __shared__ __attribute__((aligned(16))) float sq[kDK * kLdsStride];
-    __shared__ __attribute__((aligned(16))) float sx[kDK * kLdsStride];
+    __shared__ __attribute__((aligned(16))) float s_stage[kExactStageFloats];
     __shared__ uint32_t s_cnt[kTQ];
     __shared__ float s_thr[kTQ];
-    extern __shared__ uint64_t s_scratch[];  // 4 waves x cap keys
+    extern __shared__ uint64_t s_scratch[];  // 4 waves x cap keys when cap > kExactAliasCap
+    float* sq = s_stage;
+    float* sx = s_stage + kDK * kLdsStride;
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wave = tid / kWave;
@@ -133,7 +139,11 @@ __global__ __launch_bounds__(kScanThreads) void flat_exact_scan_kernel(
     const uint32_t r_end = min(n, r_begin + rows_per_chunk);
     const int tq = (tid & 15) * 4;
     const int tr = (tid >> 4) * 4;
-    uint64_t* my_scratch = s_scratch + (size_t)wave * cap;
+    // Sort scratch: the staging area between tiles (every wave has passed the
+    // barrier after the tile's last reads, and the next tile is stored after
+    // the next barrier), or dynamic LDS for large capacities.
+    uint64_t* my_scratch = (cap <= kExactAliasCap ? reinterpret_cast<uint64_t*>(s_stage) : s_scratch) +
+                           (size_t)wave * cap;
     const bool vec = (dim & 3u) == 0u && (reinterpret_cast<uintptr_t>(X) & 15u) == 0u &&
                      (reinterpret_cast<uintptr_t>(Qm) & 15u) == 0u;
 
@@ -295,7 +305,7 @@ FlatExactPlan plan_flat_exact(uint32_t n, uint32_t dim, uint32_t nq, uint32_t to
     // About 2048 workgroups, rounded to whole rounds of the workgroups the
     // CUs hold at once (LDS-bound; at most 8 of 4 wavefronts per CU), so that
     // the last round is not a sparse one.
-    const uint32_t lds = kExactStaticLds + 4u * p.cap * 8u;
+    const uint32_t lds = kExactStaticLds + exact_dynamic_lds(p.cap);
     uint32_t per_cu = kLdsPerCU / lds;
     if (per_cu > 8u) per_cu = 8u;
     if (per_cu < 1u) per_cu = 1u;
@@ -345,7 +355,7 @@ hipError_t launch_flat_exact(const float* X, const uint32_t* vec_ids, uint32_t n
         row_norms_kernel<<<(nq + 255) / 256, 256, 0, stream>>>(Qm, nq, dim, qnorm);
     }
     dim3 grid(p.qblocks, p.n_chunks);
-    size_t lds = (size_t)4 * p.cap * sizeof(uint64_t);
+    const size_t lds = exact_dynamic_lds(p.cap);
     void* timer = fma_l2 ? nullptr : timer_begin(stream);
     if (metric == METRIC_INNER_PRODUCT)
         flat_exact_scan_kernel<METRIC_INNER_PRODUCT, false><<<grid, kScanThreads, lds, stream>>>(
