@@ -46,22 +46,26 @@ __global__ void row_norms_kernel(const float* __restrict__ x, uint32_t n, uint32
     out[i] = acc;
 }
 
-// One stage (64 rows or queries x 32 dims) held in registers: thread t holds
-// dims 4*(t&7) .. +3 of tile rows t>>3 and (t>>3)+32, zero past the row range
-// or the dimension.
+// One stage (64 rows and 64 queries x kDK dims) held in registers: thread t
+// holds dims 4*(t % kSegs) .. +3 of tile rows t / kSegs + i * kRowsPerPass,
+// zero past the row range or the dimension.
+constexpr int kSegs = kDK / 4;
+constexpr int kRowsPerPass = kScanThreads / kSegs;
+constexpr int kPasses = kTN / kRowsPerPass;
+static_assert(kTN == kTQ && kPasses * kRowsPerPass == kTN, "stage shape");
 struct ExactStage {
-    float x[2][4];
-    float q[2][4];
+    float x[kPasses][4];
+    float q[kPasses][4];
 };
 
 __device__ __forceinline__ void load_exact_stage(const float* __restrict__ X, const float* __restrict__ Qm,
                                                  uint32_t dim, uint32_t nq, uint32_t q0, uint32_t r0,
                                                  uint32_t r_end, uint32_t d0, bool vec, int tid,
                                                  ExactStage& s) {
-    const uint32_t dg = d0 + (uint32_t)(tid & 7) * 4u;
+    const uint32_t dg = d0 + (uint32_t)(tid % kSegs) * 4u;
 #pragma unroll
-    for (int it = 0; it < 2; ++it) {
-        const uint32_t rr = (uint32_t)(tid >> 3) + (uint32_t)it * 32u;
+    for (int it = 0; it < kPasses; ++it) {
+        const uint32_t rr = (uint32_t)(tid / kSegs) + (uint32_t)(it * kRowsPerPass);
         const uint32_t rg = r0 + rr, qg = q0 + rr;
         if (vec) {
             // dim is a multiple of 4, so a vector that starts in range ends in range.
@@ -82,10 +86,10 @@ __device__ __forceinline__ void load_exact_stage(const float* __restrict__ X, co
 
 // Transposed into LDS: sx[d][row], sq[d][query].
 __device__ __forceinline__ void store_exact_stage(float* sq, float* sx, int tid, const ExactStage& s) {
-    const int dd = (tid & 7) * 4;
+    const int dd = (tid % kSegs) * 4;
 #pragma unroll
-    for (int it = 0; it < 2; ++it) {
-        const int rr = (tid >> 3) + it * 32;
+    for (int it = 0; it < kPasses; ++it) {
+        const int rr = tid / kSegs + it * kRowsPerPass;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             sx[(dd + c) * kLdsStride + rr] = s.x[it][c];
