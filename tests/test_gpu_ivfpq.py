@@ -9,7 +9,7 @@ import oracle as O
 from flashvector_amd import lib as L
 from flashvector_amd import search as F
 from flashvector_amd import synthetic as S
-from flashvector_amd.ivfpq_build import build_ivfpq
+from flashvector_amd.ivfpq_build import IvfPqTables, build_ivfpq
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -165,3 +165,50 @@ def test_uneven_batch_graph_replay(torch, case):
     e_ids, e_d = O.ivf_pq_search_uneven(h, qs, ks.astype(np.uint32), nps.astype(np.uint32),
                                         shortlist=60, out_stride=20)
     assert_same(L.split_results(eager), (e_ids, e_d))
+
+
+def with_empty_lists(t, before):
+    """The same tables with an empty posting list inserted before each list
+    index in `before` (its centroid a copy of that list's, so coarse
+    distances tie and the (distance, list id) order decides) and one more,
+    far away, at the end."""
+    cents, offs = [], [0]
+    for li in range(t.nlist):
+        if li in before:
+            cents.append(t.centroids[li])
+            offs.append(offs[-1])
+        cents.append(t.centroids[li])
+        offs.append(int(t.offsets[li + 1]))
+    cents.append(t.centroids[-1] + 1000.0)
+    offs.append(offs[-1])
+    return IvfPqTables(np.asarray(cents, np.float32), t.codebooks, t.codes,
+                       np.asarray(offs, np.uint32), t.vec_ids, t.vectors, t.m_pq)
+
+
+def test_clamps_and_empty_lists(torch, case):
+    """Parameter clamps of the legacy entry (ivf_pq_lookup.cu:351-362: nprobe
+    0 -> 1, nprobe above nlist -> nlist, shortlist above 10240 -> 10240) as
+    equalities between calls, and empty posting lists (tied with a populated
+    twin, in the middle and at the end) on both entry points."""
+    db, qs, t, idx, h = case
+    q40 = qs[:40]
+    q = dev(torch, q40)
+
+    def run(ix, k, npb, rr):
+        return L.split_results(ix.search_batch(q, k, npb, rr))
+
+    assert_same(run(idx, 10, 0, 100), run(idx, 10, 1, 100))
+    assert_same(run(idx, 10, 1000, 100), run(idx, 10, t.nlist, 100))
+    assert_same(run(idx, 10, 8, 20000), run(idx, 10, 8, 10240))
+    assert_same(run(idx, 10, 0, 100), O.ivf_pq_search_batch(h, q40, 10, 0, 100))
+    e = with_empty_lists(t, {0, 10, 30})
+    ei, eh = F.IvfPqIndex.from_host(e), host_tables(e)
+    for k, npb, rr in [(10, 8, 200), (10, e.nlist, 500), (5, 3, 1)]:
+        assert_same(run(ei, k, npb, rr), O.ivf_pq_search_batch(eh, q40, k, npb, rr))
+    assert_same(run(ei, 10, e.nlist, 500), run(idx, 10, t.nlist, 500))
+    rng = np.random.default_rng(93)
+    ks = rng.integers(1, 30, size=40).astype(np.uint32)
+    nps = rng.integers(1, 12, size=40).astype(np.uint32)
+    out = ei.search(q, 32, 12, 300, top_k=dev(torch, ks.view(np.int32)),
+                    nprobe=dev(torch, nps.view(np.int32)), use_list_order=True)
+    assert_same(L.split_results(out), O.ivf_pq_search_uneven(eh, q40, ks, nps, 300, out_stride=32))
