@@ -160,3 +160,34 @@ def test_deep_shortlist_workgroup_prunes(torch):
     db, qs = S.flat_workload(40000, 70, 64, 64, seeds=(81, 82, 83))
     got = run(torch, db, qs, 200, "l2", "f16", shortlist=2000)
     assert_same(got, O.flat_topk(db, qs, 200, "l2"))
+
+
+def test_fp16_tc_entry_clamps_and_no_ops(torch):
+    """cuda_fp16_tc_search_batch: top_k above 256 is clamped to 256 (the
+    output row width); zero rows, zero queries or k = 0 return 0 and leave the
+    output untouched (ivf_quant_lookup.cu:1047-1049); an all-zero mask
+    yields empty slots."""
+    import ctypes
+    db, qs = S.flat_workload(3000, 20, 64, 16, seeds=(91, 92, 93))
+    codes, norms = O.encode_fp16_padded(db)
+    d_codes, d_norms = dev(torch, codes.view(np.int16)), dev(torch, norms)
+    d_q, d_db = dev(torch, qs), dev(torch, db)
+    wide = F.fp16_tc_search_batch(d_codes, d_norms, d_q, 300, 64, "l2", None, d_db)
+    at256 = F.fp16_tc_search_batch(d_codes, d_norms, d_q, 256, 64, "l2", None, d_db)
+    torch.cuda.synchronize()
+    assert tuple(wide.shape) == (20, 256)
+    assert_same(L.split_results(wide), L.split_results(at256))
+    assert_same(L.split_results(wide), O.flat_topk(db, qs, 256, "l2"))
+    lib = L.load()
+    out = L.result_tensor(20, 8, d_q.device)
+    out.fill_(0x1234)
+    for n, nq, k in [(0, 20, 8), (3000, 0, 8), (3000, 20, 0)]:
+        t = L.Fp16GpuTables(L.ptr(d_codes), L.ptr(d_norms), L.ptr(d_db), n, 64, codes.shape[1])
+        assert lib.cuda_fp16_tc_search_batch(ctypes.byref(t), L.ptr(d_q), nq, k, 0, None,
+                                             L.ptr(out), L.stream_handle()) == 0
+    torch.cuda.synchronize()
+    assert (out.cpu().numpy() == 0x1234).all()
+    zero = np.zeros((3000 + 31) // 32, np.uint32)
+    ids, d = L.split_results(F.fp16_tc_search_batch(d_codes, d_norms, d_q, 8, 64, "l2",
+                                                    dev(torch, zero), d_db))
+    assert (ids == 0xFFFFFFFF).all() and (d == np.float32(1e30)).all()
