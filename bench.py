@@ -40,6 +40,28 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def best_of(fn, nq, reps=5):
+    """The reference harness's reading (scripts/run_benchmarks.py:60-70: one
+    warm-up, then the best of 5 calls), taken after the main timed loop with
+    HIP events around each call; also the median. `value` stays the mean over
+    the timed steps."""
+    import statistics
+
+    import torch
+    fn()
+    ts = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        torch.cuda.synchronize()
+        ts.append(a.elapsed_time(b))
+    best, med = min(ts), statistics.median(ts)
+    return {"reps": reps, "best_ms": round(best, 4), "median_ms": round(med, 4),
+            "best_qps": round(nq / (best / 1e3), 1), "median_qps": round(nq / (med / 1e3), 1)}
+
+
 def fp64_reference(db, qs, mask_bits, k):
     """Exact top-k in FP64 (ties by lower id) for a query sample."""
     import numpy as np
@@ -160,6 +182,7 @@ def main():
     step_ms = start.elapsed_time(stop) / args.steps
     qps = nq / (step_ms / 1e3)
     scan_avg_ms = scan_ms / max(launches, 1)
+    per_call = best_of(search, nq)
     # Masked rows need no work: the algorithmic contraction covers valid rows only.
     flops = 2.0 * nq * valid_count * dim
     achieved = flops / (scan_avg_ms / 1e3) / 1e12
@@ -267,6 +290,7 @@ def main():
                      "traffic": measured_traffic("flat_mfma_scan_kernel/flat_l2_mfma_rerank"),
                      "kernel": "flat_mfma_scan_kernel", "kernel_avg_ms": round(scan_avg_ms, 4),
                      "algorithmic_flop_per_launch": flops},
+        "reference_best_of_5": per_call,
         "cpu_baseline": cpu,
         "wall_s_timed": round(wall, 3),
     }
@@ -331,6 +355,7 @@ def bench_ivfpq(args):
     step_ms = e0.elapsed_time(e1) / steps
     graph_matches_eager = bool(torch.equal(out, eager))
     scan_avg = scan_ms / max(launches, 1)
+    per_call = best_of(g.replay, nq)
     nbytes = W.scanned_bytes(w, S)
     ids, dists = L.split_results(out)
     ks = w.top_k.cpu().numpy()
@@ -385,6 +410,7 @@ def bench_ivfpq(args):
                      "traffic": measured_traffic("ivf_scan_kernel/ivfpq_uneven_10m"),
                      "kernel": "ivf_scan_kernel", "kernel_avg_ms": round(scan_avg, 4),
                      "algorithmic_bytes_per_launch": nbytes},
+        "reference_best_of_5": per_call,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
@@ -435,6 +461,7 @@ def bench_exact(args):
         scan_ms, launches = kt.read()
     step_ms = start.elapsed_time(stop) / args.steps
     scan_avg = scan_ms / max(launches, 1)
+    per_call = best_of(search, nq)
     ops = 3.0 * nq * n * dim                # sub, mul, add per (query, row, dim), not fused
     achieved = ops / (scan_avg / 1e3) / 1e12
     ids, dists = L.split_results(out)
@@ -464,6 +491,7 @@ def bench_exact(args):
                      "traffic": measured_traffic("flat_exact_scan_kernel"),
                      "kernel": "flat_exact_scan_kernel", "kernel_avg_ms": round(scan_avg, 4),
                      "launches": launches, "algorithmic_ops_per_launch": ops},
+        "reference_best_of_5": per_call,
         "cpu_baseline": {"value": round(nq / cpu_s, 1), "unit": "queries/s", "cores": cores,
                          "kind": "port",
                          "sample": f"all {nq} queries against all {n} rows, exact FP32 "
@@ -612,6 +640,7 @@ def bench_fp8(args):
     wall = time.perf_counter() - wall0
     step_ms = e0.elapsed_time(e1) / steps
     graph_matches_eager = graph_matches_eager and bool(torch.equal(out, eager))
+    per_call = best_of(g.replay, nq)
     ids, dists = L.split_results(out)
 
     rs = min(args.recall_sample, nq)
@@ -675,6 +704,7 @@ def bench_fp8(args):
                      "traffic": measured_traffic("flat_mfma_scan_kernel/flat_ip_fp8_mixed_k"),
                      "kernel": "flat_mfma_scan_kernel", "kernel_avg_ms": round(scan_avg, 4),
                      "algorithmic_flop_per_launch": flops},
+        "reference_best_of_5": per_call,
         "cpu_baseline": cpu,
         "wall_s_timed": round(wall, 3),
     }
