@@ -21,6 +21,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cfloat>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -705,8 +706,17 @@ int fv_ivf_pq_build(const float* d_vectors, uint32_t n, uint32_t dim, uint32_t n
         errs[w] = err;
     };
     std::vector<std::thread> pool;
-    for (uint32_t w = 1; w < B.workers; ++w) pool.emplace_back(worker, w);
+    uint32_t started = 1;
+    for (uint32_t w = 1; w < B.workers; ++w) {
+        try {
+            pool.emplace_back(worker, w);
+        } catch (const std::system_error&) {
+            break;  // no more threads: the remaining slots run on this one
+        }
+        started = w + 1;
+    }
     worker(0);
+    for (uint32_t w = started; w < B.workers; ++w) worker(w);
     for (auto& t : pool) t.join();
     for (hipError_t we : errs)
         if (we != hipSuccess) return (int)we;

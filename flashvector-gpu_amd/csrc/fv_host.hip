@@ -1139,6 +1139,7 @@ int fvh_load_snapshot(FvEngine* e, const char* path, uint32_t flat_dtype) {
 
 #include <condition_variable>
 #include <deque>
+#include <system_error>
 #include <thread>
 
 namespace fvh {
@@ -1255,7 +1256,11 @@ int fvh_batcher_create_with_executor(FvBatchExecutor exec, void* user, uint32_t 
     b->dim = dim;
     b->max_batch = max_batch ? max_batch : 1u;  // batcher.rs:136
     b->max_wait = std::chrono::microseconds(max_wait_us);
-    b->worker = std::thread(fvh::batcher_loop, b.get());
+    try {
+        b->worker = std::thread(fvh::batcher_loop, b.get());
+    } catch (const std::system_error&) {
+        return (int)hipErrorOutOfMemory;  // no thread could be started
+    }
     *out = b.release();
     return 0;
 }
