@@ -660,11 +660,14 @@ constexpr uint32_t kTailMaxS = 256;
 constexpr uint32_t kTailWaves = 4;
 
 // Exact distances and error bounds of NR rows of one query (rows[] valid,
-// possibly repeated), summed as RerankSum does.
+// possibly repeated), summed as RerankSum does. Rows with live[r] false are
+// not read (their results are garbage and must be dropped): a load
+// instruction then touches only the lines of the lanes that need a row.
 template <int METRIC, int NR>
 __device__ __forceinline__ void rerank_rows(const float* __restrict__ X32, const float* __restrict__ qv,
                                             uint32_t dim, bool vec, const uint32_t (&rows)[NR],
-                                            float (&d32)[NR], float (&err)[NR]) {
+                                            const bool (&live)[NR], float (&d32)[NR],
+                                            float (&err)[NR]) {
     RerankSum<METRIC> sum[NR];
     if (vec) {
         const float* xr[NR];
@@ -677,7 +680,9 @@ __device__ __forceinline__ void rerank_rows(const float* __restrict__ X32, const
             for (int m = 0; m < 8; ++m) {
                 q4[m] = *reinterpret_cast<const float4*>(qv + d + 4 * m);
 #pragma unroll
-                for (int r = 0; r < NR; ++r) x4[r][m] = *reinterpret_cast<const float4*>(xr[r] + d + 4 * m);
+                for (int r = 0; r < NR; ++r)
+                    x4[r][m] = live[r] ? *reinterpret_cast<const float4*>(xr[r] + d + 4 * m)
+                                       : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             }
 #pragma unroll
             for (int m = 0; m < 8; ++m)
@@ -693,7 +698,8 @@ __device__ __forceinline__ void rerank_rows(const float* __restrict__ X32, const
             const float4 q4 = *reinterpret_cast<const float4*>(qv + d);
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
-                const float4 x4 = *reinterpret_cast<const float4*>(xr[r] + d);
+                const float4 x4 = live[r] ? *reinterpret_cast<const float4*>(xr[r] + d)
+                                          : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 sum[r].add(q4.x, x4.x);
                 sum[r].add(q4.y, x4.y);
                 sum[r].add(q4.z, x4.z);
@@ -704,7 +710,7 @@ __device__ __forceinline__ void rerank_rows(const float* __restrict__ X32, const
         for (uint32_t d = 0; d < dim; ++d) {
             const float a = qv[d];
 #pragma unroll
-            for (int r = 0; r < NR; ++r) sum[r].add(a, X32[(size_t)rows[r] * dim + d]);
+            for (int r = 0; r < NR; ++r) sum[r].add(a, live[r] ? X32[(size_t)rows[r] * dim + d] : 0.0f);
         }
     }
 #pragma unroll
@@ -840,11 +846,13 @@ __global__ __launch_bounds__(256) void flat_tail_kernel(
             float d[2], e[2];
             if (i0 + kWave < have) {
                 const uint32_t rows[2] = {ra, rb};
-                rerank_rows<METRIC, 2>(X32, qv, dim, vec, rows, d, e);
+                const bool live[2] = {ia < have, ib < have};
+                rerank_rows<METRIC, 2>(X32, qv, dim, vec, rows, live, d, e);
             } else {
                 const uint32_t rows[1] = {ra};
+                const bool live[1] = {ia < have};
                 float d1[1], e1[1];
-                rerank_rows<METRIC, 1>(X32, qv, dim, vec, rows, d1, e1);
+                rerank_rows<METRIC, 1>(X32, qv, dim, vec, rows, live, d1, e1);
                 d[0] = d1[0];
                 e[0] = e1[0];
                 d[1] = 0.0f;
