@@ -171,7 +171,10 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     constexpr int QW = Sh::kQW;
     constexpr int QB = Sh::kQB;
     constexpr int NJ = QW / 16;                  // 16-query column blocks per wave
-    constexpr int RT = R == 0 ? Sh::kRTDeep : Sh::kRT;  // rows per tile
+    // Rows per tile. The first pass (no dynamic LDS) and the workgroup-prune
+    // instantiations keep their double-buffered tiles within 50 KB, so that
+    // several workgroups share a CU.
+    constexpr int RT = (R == 0 || BOUND) ? Sh::kRTDeep : Sh::kRT;
     constexpr int LB = Sh::kLaneBytes;
     constexpr int kRowBytes = Sh::kRowBytes;
     constexpr int kLdsRow = kRowBytes + 16;      // padded LDS row
