@@ -1268,7 +1268,7 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
     // prunes use one shared scratch followed by four staging lists.
     static_assert(kRecBytes <= 1024u * 8u, "records must fit a wave's 1024-key prune scratch");
     const size_t stage_bytes = (size_t)kRecBytes;
-    p.lds_scan = p.block_prune ? (size_t)p.cap * 8 + 4 * (size_t)kRecBytes
+    p.lds_scan = p.block_prune ? (size_t)kScanSelBytes + 4 * (size_t)kRecBytes
                                : 4 * ((size_t)p.cap * 8 > stage_bytes ? (size_t)p.cap * 8 : stage_bytes);
     // Shallow shortlists: one fused wavefront per query after the scan.
     p.tail = smax <= kTailMaxS;

@@ -49,6 +49,9 @@ constexpr int kMfmaThreads = 256;
 // position (4 B); keys are built when the records are flushed.
 constexpr uint32_t kRec = 256;
 constexpr uint32_t kRecBytes = kRec * 20;
+// Workgroup-prune instantiations: the radix selection's LDS block, then the
+// four waves' record staging (16-byte aligned).
+constexpr uint32_t kScanSelBytes = (uint32_t)((sizeof(BlockSelectLds) + 15) / 16 * 16);
 
 template <int DT>
 struct MfmaOps;
@@ -222,7 +225,8 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     const uint32_t qb0 = qblock * QB;
     const uint32_t qw0 = qb0 + wave * QW;
     constexpr bool kBlockPrune = R == 0;
-    uint64_t* my_scratch = kBlockPrune ? s_scratch : s_scratch + (size_t)wave * cap;
+    uint64_t* my_scratch = s_scratch + (size_t)wave * cap;  // wave-prune instantiations only
+    BlockSelectLds& sel = *reinterpret_cast<BlockSelectLds*>(s_scratch);  // workgroup prunes
 
     // B fragments: lane (l16, grp) holds query qw0 + 16j + l16, bytes
     // LB*KS*grp + LB*s .. +LB of the encoded row for step s (the same
@@ -320,9 +324,9 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     // the per-score threshold tests and key building happen at the flush,
     // only for recorded lanes.
     // Wave mode: the records alias the wave's prune scratch (never live at the
-    // same time). Workgroup mode: they follow the shared scratch.
+    // same time). Workgroup mode: they follow the selection's LDS block.
     uint8_t* rec_base = kBlockPrune
-        ? reinterpret_cast<uint8_t*>(s_scratch + cap) + (size_t)wave * kRecBytes
+        ? reinterpret_cast<uint8_t*>(s_scratch) + kScanSelBytes + (size_t)wave * kRecBytes
         : reinterpret_cast<uint8_t*>(my_scratch);
     floatx4* rec_score = reinterpret_cast<floatx4*>(rec_base);
     uint32_t* rec_meta = reinterpret_cast<uint32_t*>(rec_base + (size_t)kRec * 16);
@@ -478,7 +482,9 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
                 wave_lds_sync();
             }
         } else {
-            // Workgroup prunes, one query at a time, through shared scratch.
+            // Workgroup prunes, one query at a time: radix selection of the
+            // `keep` smallest keys in place in the query's buffer (global
+            // memory, L2-resident), so no buffer-sized LDS scratch is needed.
             if (tid < QB / 32) s_need[tid] = 0;
             __syncthreads();
             if (my_live && s_cnt[my_ql] + pk * RT > cap) atomicOr(&s_need[my_ql >> 5], 1u << (my_ql & 31));
@@ -493,16 +499,13 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
                     const uint32_t c = s_cnt[ql];
                     const uint32_t keep = s_keep[ql];
                     uint64_t* bq = cand + ((size_t)chunk * nq + qg) * cap;
-                    const uint32_t np = next_pow2(c < 2u ? 2u : c);
-                    for (uint32_t i = tid; i < np; i += kMfmaThreads) my_scratch[i] = i < c ? bq[i] : kEmptyKey;
-                    __syncthreads();
-                    block_bitonic_sort(my_scratch, np);
-                    const uint32_t kept = c < keep ? c : keep;
-                    for (uint32_t i = tid; i < kept; i += kMfmaThreads) bq[i] = my_scratch[i];
-                    if (tid == 0) {
-                        s_cnt[ql] = kept;
-                        if (kept == keep) {
-                            const float th = key_dist(my_scratch[keep - 1]);
+                    // The trigger implies c > cap - pk * RT >= s_max + 64 > keep;
+                    // the keys of one buffer are distinct (one per row).
+                    if (c > keep) {
+                        const uint64_t kth = block_select(bq, c, keep, sel);
+                        if (tid == 0) {
+                            s_cnt[ql] = keep;
+                            const float th = key_dist(kth);
                             s_thr[ql] = fminf(s_thr[ql], th);
                             atomicMin(&gthr[qg], ord_f32(th));
                         }
