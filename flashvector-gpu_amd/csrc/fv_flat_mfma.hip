@@ -21,7 +21,10 @@
 
 namespace fv {
 
-constexpr uint32_t kSelectBatch = 256;
+// Threads of flat_select_kernel, one candidate each per round: 1024 for deep
+// shortlists (many candidates per query), 256 otherwise.
+constexpr uint32_t kSelectBatchSmall = 256;
+constexpr uint32_t kSelectBatchLarge = 1024;
 
 // Per query (one wavefront each, 4 per workgroup): a value with at least S of
 // the first pass's subset minima at or below it (the S-th smallest, or
@@ -97,7 +100,8 @@ __device__ __forceinline__ bool key_unadmitted(uint64_t k) { return (uint32_t)(k
 // within the final shared bound into an LDS top-S (sorted only when more
 // than S arrive) and writes the S best keys (approximate score, row) to
 // sel[q][0..have) and have to sel_n[q].
-__global__ __launch_bounds__(256) void flat_select_kernel(
+template <uint32_t kSelectBatch>
+__global__ __launch_bounds__(kSelectBatch) void flat_select_kernel(
     const uint64_t* __restrict__ cand, const uint32_t* __restrict__ counts, uint32_t n_chunks,
     uint32_t nq, uint32_t cap, const uint32_t* __restrict__ gthr, FlatSearchArgs a,
     uint32_t lcap, uint64_t* __restrict__ sel, uint32_t* __restrict__ sel_n) {
@@ -114,8 +118,8 @@ __global__ __launch_bounds__(256) void flat_select_kernel(
     }
     __syncthreads();
     // Candidate counts of every chunk, prefix-summed in LDS (chunks processed
-    // 256 at a time), then the concatenated candidates streamed in rounds of
-    // 256: one barrier per round instead of one per chunk.
+    // kSelectBatch at a time), then the concatenated candidates streamed in
+    // rounds of kSelectBatch: one barrier per round instead of one per chunk.
     __shared__ uint32_t s_pre[kSelectBatch + 1];
     for (uint32_t c0 = 0; c0 < n_chunks; c0 += kSelectBatch) {
         const uint32_t nc = min(kSelectBatch, n_chunks - c0);
@@ -1278,7 +1282,9 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
     if (a.flags & FV_SEARCH_F64_BOUNDARY) p.tail_wbytes += p.tail_hw * (8 + 4);
     p.tail_wbytes = (p.tail_wbytes + 15u) & ~15u;
     p.lds_tail = (size_t)kTailWaves * p.tail_wbytes;
-    p.lcap = next_pow2(smax + kSelectBatch);
+    p.sel_batch = (smax >= 2048u && next_pow2(smax + kSelectBatchLarge) <= 8192u) ? kSelectBatchLarge
+                                                                                  : kSelectBatchSmall;
+    p.lcap = next_pow2(smax + p.sel_batch);
     p.lds_select = (size_t)p.lcap * 8;
     p.hcap = next_pow2(smax < 2u ? 2u : smax);
     p.lds_final = (size_t)p.hcap * 8;
@@ -1435,8 +1441,12 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
     uint32_t* sel_n = reinterpret_cast<uint32_t*>(ws + p.off_seln);
     uint32_t* sel_off = reinterpret_cast<uint32_t*>(ws + p.off_seloff);
     float* err = (a.flags & FV_SEARCH_F64_BOUNDARY) ? reinterpret_cast<float*>(ws + p.off_err) : nullptr;
-    flat_select_kernel<<<nq, 256, p.lds_select, stream>>>(cand, counts, p.n_chunks, nq, p.cap, gthr,
-                                                          a, p.lcap, sel, sel_n);
+    if (p.sel_batch == kSelectBatchLarge)
+        flat_select_kernel<kSelectBatchLarge><<<nq, kSelectBatchLarge, p.lds_select, stream>>>(
+            cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.lcap, sel, sel_n);
+    else
+        flat_select_kernel<kSelectBatchSmall><<<nq, kSelectBatchSmall, p.lds_select, stream>>>(
+            cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.lcap, sel, sel_n);
     // Compact work list for the rerank: exclusive prefix of the per-query
     // counts (the last slot holds the total).
     exclusive_scan_kernel<<<1, 256, 0, stream>>>(sel_n, sel_off, nq, sel_off + nq);
