@@ -359,8 +359,10 @@ __global__ __launch_bounds__(256) void flat_rerank_kernel(
 // in FP64 and reordered by (FP64 distance, row). Outside a cluster the FP32
 // order is the exact order, so only clusters that start before position k are
 // touched.
-// 16 wavefronts: the kernel's LDS allows one workgroup per CU, and the FP64
-// re-scoring's row reads need many waves in flight.
+// Up to 16 wavefronts: with deep shortlists the kernel's LDS allows one
+// workgroup per CU, and the FP64 re-scoring's row reads need many waves in
+// flight; shallow shortlists run 4 wavefronts per workgroup (the kernel
+// strides by blockDim.x).
 constexpr int kFinalThreads = 1024;
 template <int METRIC>
 __global__ __launch_bounds__(kFinalThreads) void flat_finalize_kernel(
@@ -1285,6 +1287,7 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
     p.sel_batch = (smax >= 2048u && next_pow2(smax + kSelectBatchLarge) <= 8192u) ? kSelectBatchLarge
                                                                                   : kSelectBatchSmall;
     p.lcap = next_pow2(smax + p.sel_batch);
+    p.final_threads = smax >= 2048u ? (uint32_t)kFinalThreads : 256u;
     p.lds_select = (size_t)p.lcap * 8;
     p.hcap = next_pow2(smax < 2u ? 2u : smax);
     p.lds_final = (size_t)p.hcap * 8;
@@ -1453,13 +1456,13 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
     const dim3 rg((unsigned)(((size_t)nq * p.s_max + kRerankRows - 1) / kRerankRows));
     if (a.metric == METRIC_L2) {
         flat_rerank_kernel<METRIC_L2><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, nq, p.s_max, sel_n, sel_off, sel_off + nq, sel, err);
-        flat_finalize_kernel<METRIC_L2><<<nq, kFinalThreads, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
+        flat_finalize_kernel<METRIC_L2><<<nq, p.final_threads, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
     } else if (a.metric == METRIC_INNER_PRODUCT) {
         flat_rerank_kernel<METRIC_INNER_PRODUCT><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, nq, p.s_max, sel_n, sel_off, sel_off + nq, sel, err);
-        flat_finalize_kernel<METRIC_INNER_PRODUCT><<<nq, kFinalThreads, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
+        flat_finalize_kernel<METRIC_INNER_PRODUCT><<<nq, p.final_threads, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
     } else {
         flat_rerank_kernel<METRIC_COSINE><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, nq, p.s_max, sel_n, sel_off, sel_off + nq, sel, err);
-        flat_finalize_kernel<METRIC_COSINE><<<nq, kFinalThreads, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
+        flat_finalize_kernel<METRIC_COSINE><<<nq, p.final_threads, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
     }
     return hipGetLastError();
 }

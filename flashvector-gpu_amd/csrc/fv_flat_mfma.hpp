@@ -28,6 +28,7 @@ struct FlatMfmaPlan {
     size_t lds_scan;          // dynamic LDS of the scan kernel
     uint32_t lcap;            // LDS keys of the select/rerank kernel
     uint32_t sel_batch;       // threads of the select kernel (candidates per round)
+    uint32_t final_threads;   // threads of the finalize kernel
     size_t lds_select;        // dynamic LDS of the select kernel
     bool tail;                // fused wave-per-query select/rerank/finalize
     uint32_t tail_lw;         // its LDS keys per wavefront (next_pow2(s_max + 64))
