@@ -9,6 +9,7 @@
 #include <memory>
 #include <mutex>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/flashvector/cuda_bridge.h"
@@ -140,6 +141,14 @@ struct FvEngine {
     std::unordered_map<uint32_t, uint32_t> lid2slot;
     uint32_t main_dead = 0;
     uint32_t max_lid = 0;
+    // Deleted logical ids (index.rs:146 tombstones) and the next id that
+    // fvh_add_vectors hands out (index.rs:145 next_logical_id). Logical ids are
+    // unique over the main index, the delta buffer and the tombstones.
+    std::unordered_set<uint32_t> tombstones;
+    uint64_t next_lid = 0;
+    // Manifest fields this engine does not use but carries from a loaded
+    // snapshot to the next save (persistence.rs SnapshotManifest).
+    uint32_t snap_m = 32, snap_ef_construction = 128, snap_max_elements = 0;
     // delta buffer
     uint32_t n_delta = 0, delta_cap = 0, delta_dead = 0;
     fvh::DevArray<float> delta_rows;
@@ -221,7 +230,40 @@ static hipError_t size_scratch(FvEngine* e) {
 }
 
 static void track_lids(FvEngine* e, const uint32_t* lids, uint32_t n) {
-    for (uint32_t i = 0; i < n; ++i) e->max_lid = std::max(e->max_lid, lids ? lids[i] : i);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t l = lids ? lids[i] : i;
+        e->max_lid = std::max(e->max_lid, l);
+        e->next_lid = std::max<uint64_t>(e->next_lid, (uint64_t)l + 1);
+    }
+}
+
+static bool lids_unique(const uint32_t* lids, uint32_t n) {
+    if (!lids) return true;
+    std::unordered_set<uint32_t> seen;
+    seen.reserve(n);
+    for (uint32_t i = 0; i < n; ++i)
+        if (!seen.insert(lids[i]).second) return false;
+    return true;
+}
+
+// Slot of a logical id in the main index, or 0xFFFFFFFF.
+static uint32_t main_slot(const FvEngine* e, uint32_t lid) {
+    if (e->slot2lid.p) {
+        auto it = e->lid2slot.find(lid);
+        return it == e->lid2slot.end() ? 0xFFFFFFFFu : it->second;
+    }
+    return lid < e->n_main ? lid : 0xFFFFFFFFu;
+}
+
+// Delta alive bits from the tombstones (index.rs:696-714
+// upload_delta_valid_mask): row i is alive unless its logical id is deleted.
+static void rebuild_delta_alive(FvEngine* e) {
+    e->h_delta_alive.assign(words_for(e->delta_cap), 0u);
+    e->delta_dead = 0;
+    for (uint32_t i = 0; i < e->n_delta; ++i) {
+        if (e->tombstones.count(e->h_delta_ids[i])) ++e->delta_dead;
+        else e->h_delta_alive[i >> 5] |= 1u << (i & 31u);
+    }
 }
 
 // One device pass of at most max_batch queries (index.rs:2180-2237 for the
@@ -342,6 +384,15 @@ static int set_ids(FvEngine* e, const uint32_t* lids, uint32_t n) {
     e->lid2slot.clear();
     e->h_slot2lid.clear();
     e->slot2lid.release();
+    // A new main index starts with no tombstones and an empty delta buffer
+    // (index.rs:519-524); the delta's device buffers keep their capacity.
+    e->tombstones.clear();
+    e->n_delta = e->delta_dead = 0;
+    e->h_delta_ids.clear();
+    e->h_delta_alive.clear();
+    e->lid2delta.clear();
+    e->max_lid = 0;
+    e->next_lid = 0;
     if (lids) {
         e->h_slot2lid.assign(lids, lids + n);
         for (uint32_t s = 0; s < n; ++s) e->lid2slot[lids[s]] = s;
@@ -363,6 +414,7 @@ int fvh_load_flat(FvEngine* e, const float* h_vectors, uint32_t n, uint32_t dtyp
     if (!e || !h_vectors) return -1;
     if (n == 0 || (dtype != FV_DTYPE_F16 && dtype != FV_DTYPE_BF16 && dtype != FV_DTYPE_FP8_E4M3))
         return (int)hipErrorInvalidValue;
+    if (!fvh::lids_unique(h_logical_ids, n)) return (int)hipErrorInvalidValue;
     const uint32_t dim = e->cfg.dim;
     const uint32_t step = dtype == FV_DTYPE_FP8_E4M3 ? 128u : 32u;
     const uint32_t pd = (dim + step - 1) / step * step;
@@ -394,6 +446,7 @@ int fvh_load_ivfpq(FvEngine* e, const float* h_centroids, uint32_t nlist, const 
         return -1;
     const uint32_t dim = e->cfg.dim;
     if (n == 0 || nlist == 0 || m_pq == 0 || dim % m_pq) return (int)hipErrorInvalidValue;
+    if (!fvh::lids_unique(h_logical_ids, n)) return (int)hipErrorInvalidValue;
     hipError_t he;
     if ((he = e->centroids.alloc((size_t)nlist * dim)) != hipSuccess ||
         (he = e->codebooks.alloc((size_t)m_pq * 256 * (dim / m_pq))) != hipSuccess ||
@@ -421,23 +474,44 @@ int fvh_load_ivfpq(FvEngine* e, const float* h_centroids, uint32_t nlist, const 
 
 int fvh_add_delta(FvEngine* e, const float* h_vectors, const uint32_t* h_logical_ids, uint32_t n) {
     if (!e || !h_vectors || !h_logical_ids) return -1;
+    if (e->mode == 0) return -1;
     if (n == 0) return 0;
+    // Logical ids stay unique, as the reference's are (add_vectors hands out
+    // next_logical_id, index.rs:667-669): an id already used by the main
+    // index, the delta buffer or a deleted row is rejected, as is a repeat
+    // inside the batch. Nothing is changed when the call fails.
+    {
+        std::unordered_set<uint32_t> batch;
+        batch.reserve(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t lid = h_logical_ids[i];
+            if (!batch.insert(lid).second || fvh::main_slot(e, lid) != 0xFFFFFFFFu ||
+                e->lid2delta.count(lid) || e->tombstones.count(lid))
+                return (int)hipErrorInvalidValue;
+        }
+    }
     const uint32_t dim = e->cfg.dim;
+    if ((uint64_t)e->n_delta + n > UINT32_MAX / 2) return (int)hipErrorInvalidValue;
     const uint32_t need = e->n_delta + n;
     hipError_t he;
     if (need > e->delta_cap) {
         uint32_t cap = std::max(need, e->delta_cap * 2);
         fvh::DevArray<float> rows;
-        if ((he = rows.alloc((size_t)cap * dim)) != hipSuccess) return (int)he;
+        fvh::DevArray<uint32_t> ids, alive;
+        if ((he = rows.alloc((size_t)cap * dim)) != hipSuccess ||
+            (he = ids.alloc(cap)) != hipSuccess ||
+            (he = alive.alloc(fvh::words_for(cap))) != hipSuccess)
+            return (int)he;
         if (e->n_delta &&
             (he = hipMemcpy(rows.p, e->delta_rows.p, (size_t)e->n_delta * dim * 4,
                             hipMemcpyDeviceToDevice)) != hipSuccess)
             return (int)he;
         std::swap(rows.p, e->delta_rows.p);
         std::swap(rows.n, e->delta_rows.n);
-        if ((he = e->delta_ids.alloc(cap)) != hipSuccess ||
-            (he = e->delta_alive.alloc(fvh::words_for(cap))) != hipSuccess)
-            return (int)he;
+        std::swap(ids.p, e->delta_ids.p);
+        std::swap(ids.n, e->delta_ids.n);
+        std::swap(alive.p, e->delta_alive.p);
+        std::swap(alive.n, e->delta_alive.n);
         e->delta_cap = cap;
     }
     if ((he = fvh::upload(e->delta_rows.p + (size_t)e->n_delta * dim, h_vectors,
@@ -448,9 +522,8 @@ int fvh_add_delta(FvEngine* e, const float* h_vectors, const uint32_t* h_logical
         e->h_delta_ids.push_back(h_logical_ids[i]);
     }
     e->n_delta = need;
-    e->h_delta_alive.assign(fvh::words_for(e->delta_cap), 0u);
-    for (uint32_t i = 0; i < e->n_delta; ++i) e->h_delta_alive[i >> 5] |= 1u << (i & 31u);
-    e->delta_dead = 0;
+    // Earlier rows keep their state: alive bits follow the tombstones.
+    fvh::rebuild_delta_alive(e);
     fvh::track_lids(e, h_logical_ids, n);
     if ((he = fvh::upload(e->delta_ids.p, e->h_delta_ids.data(), (size_t)e->n_delta * 4)) != hipSuccess ||
         (he = fvh::upload(e->delta_alive.p, e->h_delta_alive.data(), e->h_delta_alive.size() * 4)) != hipSuccess)
@@ -458,34 +531,51 @@ int fvh_add_delta(FvEngine* e, const float* h_vectors, const uint32_t* h_logical
     return (int)fvh::size_scratch(e);
 }
 
+int fvh_add_vectors(FvEngine* e, const float* h_vectors, uint32_t n, uint32_t* h_assigned_ids) {
+    if (!e || !h_vectors || !h_assigned_ids) return -1;
+    if (e->mode == 0) return -1;
+    if (n == 0) return 0;
+    if (e->next_lid + n > UINT32_MAX) return (int)hipErrorInvalidValue;
+    for (uint32_t i = 0; i < n; ++i) h_assigned_ids[i] = (uint32_t)(e->next_lid + i);
+    return fvh_add_delta(e, h_vectors, h_assigned_ids, n);
+}
+
 int fvh_delete(FvEngine* e, const uint32_t* h_logical_ids, uint32_t n) {
     if (!e || !h_logical_ids) return -1;
+    // index.rs:583-640 delete_ids: an id already deleted is skipped; a delta
+    // row is tombstoned; a main row is tombstoned and its validity bit cleared.
+    bool delta_dirty = false, main_dirty = false;
     for (uint32_t i = 0; i < n; ++i) {
         const uint32_t lid = h_logical_ids[i];
-        uint32_t slot = 0xFFFFFFFFu;
-        if (e->slot2lid.p) {
-            auto it = e->lid2slot.find(lid);
-            if (it != e->lid2slot.end()) slot = it->second;
-        } else if (lid < e->n_main) {
-            slot = lid;
+        if (e->tombstones.count(lid)) continue;
+        if (e->lid2delta.count(lid)) {
+            e->tombstones.insert(lid);
+            delta_dirty = true;
+            continue;
         }
-        if (slot != 0xFFFFFFFFu && ((e->h_valid[slot >> 5] >> (slot & 31u)) & 1u)) {
-            e->h_valid[slot >> 5] &= ~(1u << (slot & 31u));
-            ++e->main_dead;
-        }
-        auto dt = e->lid2delta.find(lid);
-        if (dt != e->lid2delta.end()) {
-            const uint32_t j = dt->second;
-            if ((e->h_delta_alive[j >> 5] >> (j & 31u)) & 1u) {
-                e->h_delta_alive[j >> 5] &= ~(1u << (j & 31u));
-                ++e->delta_dead;
+        const uint32_t slot = fvh::main_slot(e, lid);
+        if (slot != 0xFFFFFFFFu) {
+            e->tombstones.insert(lid);
+            if ((e->h_valid[slot >> 5] >> (slot & 31u)) & 1u) {
+                e->h_valid[slot >> 5] &= ~(1u << (slot & 31u));
+                ++e->main_dead;
             }
+            main_dirty = true;
         }
     }
-    hipError_t he = fvh::upload(e->valid.p, e->h_valid.data(), e->h_valid.size() * 4);
-    if (he == hipSuccess && e->delta_alive.p)
+    hipError_t he = hipSuccess;
+    if (main_dirty) he = fvh::upload(e->valid.p, e->h_valid.data(), e->h_valid.size() * 4);
+    if (he == hipSuccess && delta_dirty) {
+        fvh::rebuild_delta_alive(e);
         he = fvh::upload(e->delta_alive.p, e->h_delta_alive.data(), e->h_delta_alive.size() * 4);
+    }
     return (int)he;
+}
+
+int fvh_next_logical_id(const FvEngine* e, uint32_t* out) {
+    if (!e || !out) return -1;
+    *out = (uint32_t)std::min<uint64_t>(e->next_lid, UINT32_MAX);
+    return 0;
 }
 
 int fvh_search(FvEngine* e, const float* h_queries, uint32_t nq, uint32_t top_k,
@@ -963,9 +1053,9 @@ int fvh_save_snapshot(FvEngine* e, const char* path) {
     FvSnapshotInfo I{};
     I.version = 1;
     I.dim = dim;
-    I.max_elements = n;
-    I.m = 32;
-    I.ef_construction = 128;
+    I.max_elements = std::max(n, e->snap_max_elements);
+    I.m = e->snap_m;
+    I.ef_construction = e->snap_ef_construction;
     I.metric = e->cfg.metric;
     I.num_vectors = n;
     uint32_t nlist = 0, m_pq = 0;
@@ -1003,14 +1093,13 @@ int fvh_save_snapshot(FvEngine* e, const char* path) {
     }
     I.nlist = nlist;
     I.m_pq = m_pq;
-    // Tombstones: logical ids of dead main slots and dead delta rows, ascending.
-    for (uint32_t s = 0; s < n; ++s)
-        if (!((e->h_valid[s >> 5] >> (s & 31u)) & 1u)) tomb.push_back(lids[s]);
-    for (uint32_t j = 0; j < e->n_delta; ++j)
-        if (!((e->h_delta_alive[j >> 5] >> (j & 31u)) & 1u)) tomb.push_back(e->h_delta_ids[j]);
+    // Tombstones ascending (RoaringBitset::to_vec, index.rs:984-991).
+    tomb.assign(e->tombstones.begin(), e->tombstones.end());
     std::sort(tomb.begin(), tomb.end());
-    tomb.erase(std::unique(tomb.begin(), tomb.end()), tomb.end());
-    uint64_t next = 0;
+    // next_logical_id: the engine's counter, which already covers every id
+    // it holds and the value of a loaded snapshot (ids the reference handed
+    // out and compacted away are never reused).
+    uint64_t next = e->next_lid;
     for (uint32_t l : lids) next = std::max<uint64_t>(next, (uint64_t)l + 1);
     for (uint32_t l : e->h_delta_ids) next = std::max<uint64_t>(next, (uint64_t)l + 1);
     I.next_logical_id = (uint32_t)std::min<uint64_t>(next, UINT32_MAX);
@@ -1118,8 +1207,10 @@ int fvh_load_snapshot(FvEngine* e, const char* path, uint32_t flat_dtype) {
     const int t = section_view(s.get(), "tombstones", &tomb, &c_tomb);
     if (t == 0) {
         if (c_tomb) rc = fvh_delete(e, tomb, (uint32_t)c_tomb);
+        // index.rs:1195-1198 keeps every listed id, also ids no row holds.
+        for (size_t i = 0; i < c_tomb; ++i) e->tombstones.insert(tomb[i]);
     } else if (t == 1) {
-        // index.rs:1195-1206: without tombstones, cleared validity bits.
+        // index.rs:1199-1206: without tombstones, cleared validity bits.
         std::vector<uint32_t> dead;
         for (uint32_t sl = 0; sl < n; ++sl)
             if (!((valid[sl >> 5] >> (sl & 31u)) & 1u)) dead.push_back(lids[sl]);
@@ -1127,7 +1218,13 @@ int fvh_load_snapshot(FvEngine* e, const char* path, uint32_t flat_dtype) {
     } else {
         return FVH_ERR_FORMAT;
     }
-    return rc;
+    if (rc) return rc;
+    // The manifest's id counter and the fields this engine only carries.
+    e->next_lid = std::max<uint64_t>(e->next_lid, I.next_logical_id);
+    e->snap_m = I.m;
+    e->snap_ef_construction = I.ef_construction;
+    e->snap_max_elements = (uint32_t)std::min<uint64_t>(I.max_elements, UINT32_MAX);
+    return 0;
 }
 
 }  // extern "C"

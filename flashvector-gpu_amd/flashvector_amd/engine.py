@@ -99,6 +99,21 @@ class Engine:
         L.check(self._lib.fvh_add_delta(self._h, v.ctypes.data, i.ctypes.data, v.shape[0]),
                 "fvh_add_delta")
 
+    def add_vectors(self, vectors):
+        """Appends rows under fresh logical ids (index.rs:647-694 add_vectors);
+        returns the assigned ids (uint32 [n])."""
+        v = _host_f32(vectors)
+        out = np.zeros(v.shape[0], np.uint32)
+        L.check(self._lib.fvh_add_vectors(self._h, v.ctypes.data, v.shape[0], out.ctypes.data),
+                "fvh_add_vectors")
+        return out
+
+    @property
+    def next_logical_id(self):
+        n = ctypes.c_uint32(0)
+        L.check(self._lib.fvh_next_logical_id(self._h, ctypes.byref(n)), "fvh_next_logical_id")
+        return n.value
+
     def delete(self, ids):
         i = _host_u32(ids)
         L.check(self._lib.fvh_delete(self._h, i.ctypes.data, i.shape[0]), "fvh_delete")

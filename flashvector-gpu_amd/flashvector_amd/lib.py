@@ -170,7 +170,9 @@ SIGNATURES = {
     "fvh_load_flat": (I, [P, P, U, U, P]),
     "fvh_load_ivfpq": (I, [P, P, U, P, U, P, P, P, P, U, P]),
     "fvh_add_delta": (I, [P, P, P, U]),
+    "fvh_add_vectors": (I, [P, P, U, P]),
     "fvh_delete": (I, [P, P, U]),
+    "fvh_next_logical_id": (I, [P, P]),
     "fvh_search": (I, [P, P, U, U, P, U, P]),
     "fvh_search_device": (I, [P, P, U, U, P, U, P, P]),
     "fvh_stats": (I, [P, P, P]),

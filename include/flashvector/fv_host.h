@@ -68,12 +68,27 @@ int fvh_load_ivfpq(FvEngine* e, const float* h_centroids, uint32_t nlist,
                    const uint32_t* h_offsets, const uint32_t* h_vec_ids, const float* h_vectors,
                    uint32_t n, const uint32_t* h_logical_ids);
 
-/* Appends rows to the delta buffer (exact FP32 scan, merged into every
- * search result). */
+/* Appends rows with caller-chosen logical ids to the delta buffer (exact
+ * FP32 scan, merged into every search result). Logical ids are unique over
+ * the main index, the delta buffer and the deleted ids: a used id, or one
+ * repeated in the batch, fails the whole call with hipErrorInvalidValue and
+ * changes nothing. Rows deleted earlier stay deleted. */
 int fvh_add_delta(FvEngine* e, const float* h_vectors, const uint32_t* h_logical_ids, uint32_t n);
 
-/* Tombstones logical ids in the main index and the delta buffer. */
+/* crates/engine/src/index.rs:647-694 add_vectors: appends n rows to the delta
+ * buffer under fresh logical ids next_logical_id, +1, ..., written to
+ * h_assigned_ids[n]. */
+int fvh_add_vectors(FvEngine* e, const float* h_vectors, uint32_t n, uint32_t* h_assigned_ids);
+
+/* crates/engine/src/index.rs:583-640 delete_ids: tombstones logical ids held
+ * by the main index or the delta buffer (ids already deleted, or held by no
+ * row, are skipped). A deleted id stays deleted until a new main index is
+ * loaded. */
 int fvh_delete(FvEngine* e, const uint32_t* h_logical_ids, uint32_t n);
+
+/* index.rs:2545-2547: the next logical id fvh_add_vectors hands out (one past
+ * every id the engine holds, or the loaded snapshot's counter if higher). */
+int fvh_next_logical_id(const FvEngine* e, uint32_t* out);
 
 /* Host queries [nq * dim] -> host results [nq * top_k]
  * (index.rs:2180-2237 batch_search_fp16_tc, :1979-2108 batch_search_ivf_mode). */

@@ -133,3 +133,51 @@ def test_concurrent_callers_match_sequential(torch):
     for a, b in zip(res, seq):
         np.testing.assert_array_equal(a[0], b[0])
         np.testing.assert_array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+
+
+def test_delete_survives_later_adds_and_snapshot(torch, tmp_path):
+    """index.rs:647-714: rows deleted from the delta buffer stay deleted when
+    more rows are added later (alive bits follow the tombstones), the deleted
+    id is written to the snapshot's tombstones, and ids are never reused."""
+    rng = np.random.default_rng(17)
+    n, dim, k = 5000, 32, 10
+    main = rng.integers(0, 50, size=(n, dim)).astype(np.float32)
+    qs = rng.integers(0, 50, size=(64, dim)).astype(np.float32)
+    eng = E.Engine(dim, "l2", max_batch=64, max_top_k=32)
+    eng.load_flat(main, "f16")
+    assert eng.next_logical_id == n
+    # First add: rows equal to the queries themselves, so each query's best
+    # match is its own delta row.
+    a_ids = eng.add_vectors(qs[:32])
+    np.testing.assert_array_equal(a_ids, np.arange(n, n + 32))
+    dead = np.array([a_ids[3], a_ids[7], 11], np.uint32)      # two delta rows, one main row
+    eng.delete(dead)
+    # Second add: more rows under fresh ids; the deleted rows must not return.
+    b_ids = eng.add_vectors(qs[32:])
+    np.testing.assert_array_equal(b_ids, np.arange(n + 32, n + 64))
+    delta = qs.copy()
+    delta_ids = np.concatenate([a_ids, b_ids]).astype(np.uint32)
+    main_ids = np.arange(n, dtype=np.uint32)
+    allow_all = lambda ids: np.ones(len(ids), bool)  # noqa: E731
+    exp = expected(main, main_ids, ~np.isin(main_ids, dead), delta, delta_ids,
+                   ~np.isin(delta_ids, dead), qs, k, "l2", allow_all)
+    got = eng.search(qs, k)
+    np.testing.assert_array_equal(got[0], exp[0])
+    np.testing.assert_array_equal(got[1].view(np.uint32), exp[1].view(np.uint32))
+    assert not np.isin(got[0], dead).any()
+    # Used and deleted ids are rejected, and a failed add changes nothing.
+    for bad in ([a_ids[3]], [b_ids[0]], [5], [n + 100, n + 100]):
+        with pytest.raises(L.FvError):
+            eng.add(qs[:len(bad)], np.array(bad, np.uint32))
+    np.testing.assert_array_equal(eng.search(qs, k)[0], exp[0])
+    # Snapshot round trip keeps the deletions and the id counter.
+    path = tmp_path / "snap.fv"
+    eng.save_snapshot(path)
+    eng2 = E.Engine(dim, "l2", max_batch=64, max_top_k=32)
+    eng2.load_snapshot(path, "f16")
+    got2 = eng2.search(qs, k)
+    np.testing.assert_array_equal(got2[0], exp[0])
+    np.testing.assert_array_equal(got2[1].view(np.uint32), exp[1].view(np.uint32))
+    assert eng2.next_logical_id == n + 64
+    c_ids = eng2.add_vectors(qs[:2])
+    np.testing.assert_array_equal(c_ids, [n + 64, n + 65])

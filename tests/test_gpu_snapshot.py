@@ -120,8 +120,8 @@ def test_loads_reference_style_file(torch, tmp_path):
             ("rabitq_centroids", empty_f32), ("rabitq_bits", empty_u32),
             ("rabitq_meta", empty_f32), ("vectors_fp16", f16.ravel()), ("vec_norms_sq", norms)]
     p = str(tmp_path / "ref.flshvec")
-    SN.write(p, secs, dim=dim, max_elements=n, m=32, ef_construction=128, nlist=256, m_pq=16,
-             metric=0, num_vectors=n, fp16_padded_dim=16, next_logical_id=n)
+    SN.write(p, secs, dim=dim, max_elements=n, m=24, ef_construction=200, nlist=256, m_pq=16,
+             metric=0, num_vectors=n, fp16_padded_dim=16, next_logical_id=n + 40)
     qs = np.concatenate([x[:4], rng.normal(size=(20, dim)).astype(np.float32)])
     b = E.Engine(dim, "l2", max_top_k=16, shortlist_factor=16)
     b.load_snapshot(p, "f16")
@@ -132,6 +132,16 @@ def test_loads_reference_style_file(torch, tmp_path):
     same(got, d.search(qs, 5))
     assert got[0][0][0] == 0 and got[0][2][0] == 2          # live rows find themselves
     assert 1 not in got[0][1] and 3 not in got[0][3]
+    # The manifest's id counter (ids handed out and compacted away by the
+    # reference) is kept: new rows get ids above it, and a save carries it
+    # and the graph fields the engine does not use.
+    assert b.next_logical_id == n + 40
+    np.testing.assert_array_equal(b.add_vectors(x[:2]), [n + 40, n + 41])
+    p2 = str(tmp_path / "resaved.flshvec")
+    b.save_snapshot(p2)
+    info, _ = SN.read(p2)
+    assert info["next_logical_id"] == n + 42
+    assert info["m"] == 24 and info["ef_construction"] == 200 and info["max_elements"] == n
 
 
 def test_fp16_row_codes_match_reference_encoder(torch):
