@@ -110,7 +110,7 @@ extern "C" int fv_flat_autotune(const FvFlatTables* t, const float* d_queries, u
     // (2) Work shape and first-pass fraction, by time.
     fv::FlatMfmaPlan ap;
     if (!fv::flat_plan_for(t, nq, &p, &ap)) return (int)hipErrorInvalidValue;
-    const uint32_t rows_plan = ap.compact ? base->valid_count : t->num_vectors;
+    const uint32_t rows_plan = ap.compact && !ap.dyn_rows ? base->valid_count : t->num_vectors;
     const uint32_t auto_div = ap.bound_rows ? std::max(1u, rows_plan / ap.bound_rows) : 1u;
     std::vector<uint32_t> chunks = {std::max(1u, ap.n_chunks / 2), ap.n_chunks, ap.n_chunks * 2,
                                     ap.n_chunks * 4, ap.n_chunks * 8};

@@ -12,6 +12,7 @@
 // wavefront (bisection over the distance, exact sort on heavy ties). A second
 // kernel merges the per-chunk lists of each query.
 #include "fv_flat_exact.hpp"
+#include "fv_workspace.hpp"
 
 namespace fv {
 
@@ -402,14 +403,13 @@ int cuda_flat_scan_batch(const float* d_vectors, const uint32_t* d_vec_ids, uint
     }
     if (dim == 0) return (int)hipErrorInvalidValue;
     fv::FlatExactPlan p = fv::plan_flat_exact(num_vectors, dim, num_queries, top_k, (int)metric);
-    void* ws = nullptr;
-    hipError_t e = hipMallocAsync(&ws, p.workspace_bytes, s);
+    fv::WorkspaceLease ws;
+    hipError_t e = fv::stream_workspace(s, p.workspace_bytes, ws);
     if (e != hipSuccess) return (int)e;
     e = fv::launch_flat_exact(d_vectors, d_vec_ids, num_vectors, dim, d_queries, num_queries,
-                              top_k, (int)metric, d_valid_mask, d_out_results, top_k, ws, p, s, false);
-    hipError_t e2 = hipFreeAsync(ws, s);
-    if (e != hipSuccess) return (int)e;
-    return e2 == hipSuccess ? 0 : (int)e2;
+                              top_k, (int)metric, d_valid_mask, d_out_results, top_k, ws.ptr, p, s,
+                              false);
+    return e == hipSuccess ? 0 : (int)e;
 }
 
 size_t fv_flat_exact_workspace_size(uint32_t num_vectors, uint32_t dim, uint32_t num_queries,

@@ -18,6 +18,7 @@
 //  * flat_tail_kernel: the three steps above fused into one wavefront per
 //    query, used when the shortlist has at most 256 rows.
 #include "fv_flat_mfma_scan.hpp"
+#include "fv_workspace.hpp"
 
 namespace fv {
 
@@ -1298,8 +1299,10 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
     p.qb = 4 * p.qw;
     p.qrow_bytes = (uint32_t)scan_row_bytes(dtype, p.ks);
     p.qblocks = (nq + p.qb - 1) / p.qb;
-    p.compact = has_mask && valid_count > 0 && (uint64_t)valid_count * 2 <= n;
-    const uint32_t rows_plan = p.compact ? valid_count : n;
+    p.dyn_rows = has_mask && valid_count == FV_VALID_COUNT_DEVICE;
+    p.compact = p.dyn_rows || (has_mask && valid_count > 0 && (uint64_t)valid_count * 2 <= n);
+    // Device-counted masks plan for every row (the upper bound of the count).
+    const uint32_t rows_plan = p.compact && !p.dyn_rows ? valid_count : n;
     // First pass: 16 subset minima per query per chunk over the first
     // bound_rows positions (a quarter of the rows, a sixteenth for rows wider
     // than 256 bytes, at least 64 MB of rows or all). With m >= 4*S minima (at least
@@ -1326,12 +1329,13 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
     // per XCD) when there are at least 8.
     const uint32_t min_rows = p.bound_chunks ? 2048u : 8192u;
     const uint32_t want = (2560u + p.qblocks - 1) / p.qblocks;
-    const uint32_t max_by_rows = (rows_plan + min_rows - 1) / min_rows;
-    uint32_t c = want < max_by_rows ? want : max_by_rows;
-    if (c > 256u) c = 256u;
-    if (c >= 8u) c &= ~7u;
-    if (c < 1u) c = 1u;
+    uint32_t c = chunks_for_rows(rows_plan, want, min_rows, 0xFFFFFFFFu);
+    p.chunk_want = p.chunk_min_rows = 0;
     if (a.n_chunks) c = a.n_chunks < 4096u ? a.n_chunks : 4096u;
+    else if (p.dyn_rows) {
+        p.chunk_want = want;
+        p.chunk_min_rows = min_rows;
+    }
     p.n_chunks = c;
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -1416,8 +1420,11 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
     }
     float* minima = p.bound_chunks ? reinterpret_cast<float*>(ws + p.off_minima) : nullptr;
     const uint32_t esz = t->dtype == FV_DTYPE_FP8_E4M3 ? 1u : 2u;
+    FlatSearchArgs sa = a;
+    sa.chunk_want = p.chunk_want;
+    sa.chunk_min_rows = p.chunk_min_rows;
     ScanLaunch L{&p, static_cast<const uint8_t*>(t->d_codes), t->padded_dim * esz, t->d_norms_sq,
-                 t->d_row_scales, Qh, qscale, nq, a, rows, nrows, t->num_vectors, gthr, counts,
+                 t->d_row_scales, Qh, qscale, nq, sa, rows, nrows, t->num_vectors, gthr, counts,
                  cand, minima, stream};
     hipError_t e = hipErrorInvalidValue;
     const bool large = p.ks > 8;
@@ -1569,20 +1576,22 @@ int cuda_fp16_tc_search_batch(const Fp16GpuTables* tables, const float* d_querie
     if (tc) return tc;
     FvFlatSearchParams sp{};
     sp.d_valid_mask = d_valid_mask;
+    // The mask's popcount is taken on the device: sparse masks (the 5%-valid
+    // filters of the benchmarks) are compacted and only their rows scored.
+    sp.valid_count = d_valid_mask ? FV_VALID_COUNT_DEVICE : 0u;
     sp.metric = (uint32_t)metric;
     sp.top_k = top_k;
     fv::FlatSearchArgs a;
     if (!fv::make_args(&t, &sp, a)) return (int)hipErrorInvalidValue;
     hipStream_t s = static_cast<hipStream_t>(stream);
     fv::FlatMfmaPlan p = fv::plan_flat_mfma(t.num_vectors, t.padded_dim, FV_DTYPE_F16, num_queries,
-                                            a, d_valid_mask != nullptr, 0);
-    void* ws = nullptr;
-    hipError_t e = hipMallocAsync(&ws, p.workspace_bytes, s);
+                                            a, d_valid_mask != nullptr, sp.valid_count);
+    if (p.lcap > 8192u || p.ks == 0) return (int)hipErrorInvalidValue;
+    fv::WorkspaceLease ws;
+    hipError_t e = fv::stream_workspace(s, p.workspace_bytes, ws);
     if (e != hipSuccess) return (int)e;
-    e = fv::launch_flat_mfma(&t, d_queries, num_queries, a, d_out_results, ws, p, s);
-    hipError_t e2 = hipFreeAsync(ws, s);
-    if (e != hipSuccess) return (int)e;
-    return e2 == hipSuccess ? 0 : (int)e2;
+    e = fv::launch_flat_mfma(&t, d_queries, num_queries, a, d_out_results, ws.ptr, p, s);
+    return e == hipSuccess ? 0 : (int)e;
 }
 
 }  // extern "C"

@@ -18,7 +18,25 @@ struct FlatSearchArgs {
     uint32_t flags;           // FV_SEARCH_* bits
     uint32_t n_chunks;        // plan override (0 = automatic)
     uint32_t bound_div;       // plan override (0 = automatic, ~0u = no first pass)
+    // Row count known only on the device (mask compacted without a host
+    // count): the main pass launches the plan's chunk count as an upper bound
+    // and uses min(chunk_want, ceil(rows / chunk_min_rows)) chunks (a
+    // multiple of 8 from 8 on); surplus workgroups only zero their counts.
+    // 0 = the launched chunk count is used as is.
+    uint32_t chunk_want;
+    uint32_t chunk_min_rows;
 };
+
+// Chunks of the main pass for `rows` rows (the planner's rule).
+__host__ __device__ inline uint32_t chunks_for_rows(uint32_t rows, uint32_t want,
+                                                    uint32_t min_rows, uint32_t cap) {
+    const uint32_t by_rows = (rows + min_rows - 1) / min_rows;
+    uint32_t c = want < by_rows ? want : by_rows;
+    if (c > 256u) c = 256u;
+    if (c >= 8u) c &= ~7u;
+    if (c < 1u) c = 1u;
+    return c < cap ? c : cap;
+}
 
 struct FlatMfmaPlan {
     uint32_t s_max;           // deepest per-query shortlist
@@ -44,6 +62,9 @@ struct FlatMfmaPlan {
     uint32_t qblocks;         // query blocks
     uint32_t n_chunks;
     bool compact;             // scan the compacted list of valid rows
+    bool dyn_rows;            // compacted without a host count: chunks chosen on the device
+    uint32_t chunk_want;      // device-chosen chunks: FlatSearchArgs::chunk_want / chunk_min_rows
+    uint32_t chunk_min_rows;
     uint32_t bound_chunks;    // first-pass chunks (0 = no first pass)
     uint32_t bound_rows;      // rows covered by the first pass
     uint32_t mask_blocks;

@@ -202,9 +202,24 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     const int l16 = lane & 15;
     const int grp = lane >> 4;
 
+    // Rows of the (possibly compacted) row space, and the chunks that cover
+    // them: the launched count, or fewer when it is chosen from the device
+    // row count (then the surplus workgroups zero their queries' counts).
+    uint32_t n_eff = nrows ? *nrows : n;
+    if (BOUND && n_eff > bound_rows) n_eff = bound_rows;
+    const uint32_t b = blockIdx.x;
+    if (!BOUND && a.chunk_want) {
+        const uint32_t nch = chunks_for_rows(n_eff, a.chunk_want, a.chunk_min_rows, n_chunks);
+        if (b >= nch * qblocks) {
+            const uint32_t e = b - nch * qblocks;
+            const uint32_t c = nch + e / qblocks, q = (e % qblocks) * QB + (uint32_t)threadIdx.x;
+            if (threadIdx.x < QB && q < nq) counts[(size_t)c * nq + q] = 0u;
+            return;
+        }
+        n_chunks = nch;
+    }
     // XCD-first work mapping: consecutive block ids land on different XCDs;
     // within an XCD the chunk varies slowest.
-    const uint32_t b = blockIdx.x;
     uint32_t chunk, qblock;
     if ((n_chunks & 7u) == 0u) {
         const uint32_t cpx = n_chunks >> 3;
@@ -215,9 +230,6 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
         chunk = b / qblocks;
         qblock = b % qblocks;
     }
-    // Chunk bounds in the (possibly compacted) row space.
-    uint32_t n_eff = nrows ? *nrows : n;
-    if (BOUND && n_eff > bound_rows) n_eff = bound_rows;
     uint32_t rpc = (n_eff + n_chunks - 1) / n_chunks;
     rpc = (rpc + RT - 1) / RT * RT;
     const uint32_t r_begin = min(n_eff, chunk * rpc);

@@ -176,9 +176,14 @@ static hipError_t upload(void* dst, const void* src, size_t bytes) {
     return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
 }
 
-static FvFlatSearchParams flat_params(const FvEngine* e, uint32_t k, const uint32_t* mask) {
+// A caller mask is compacted with its count taken on the device (the scan
+// then scores only the allowed rows); a mask of tombstones alone is tested
+// per row.
+static FvFlatSearchParams flat_params(const FvEngine* e, uint32_t k, const uint32_t* mask,
+                                      bool user_mask) {
     FvFlatSearchParams p{};
     p.d_valid_mask = mask;
+    p.valid_count = mask && user_mask ? FV_VALID_COUNT_DEVICE : 0u;
     p.metric = e->cfg.metric;
     p.top_k = k;
     p.shortlist_factor = e->cfg.shortlist_factor;
@@ -194,8 +199,10 @@ static hipError_t size_scratch(FvEngine* e) {
     size_t ws = 0;
     if (e->mode == 1) {
         for (uint32_t nq = B; nq >= 1; nq /= 2) {
-            FvFlatSearchParams p = flat_params(e, K, e->valid.p);
-            ws = std::max(ws, fv_flat_workspace_size(&e->flat, nq, &p));
+            for (int um = 0; um < 2; ++um) {
+                FvFlatSearchParams p = flat_params(e, K, e->valid.p, um == 1);
+                ws = std::max(ws, fv_flat_workspace_size(&e->flat, nq, &p));
+            }
             if (nq == 1) break;
         }
         if (ws == 0) return hipErrorInvalidValue;
@@ -305,7 +312,7 @@ static int search_pass(FvEngine* e, Scratch& s, hipStream_t st, const float* d_q
     }
     int rc;
     if (e->mode == 1) {
-        FvFlatSearchParams p = flat_params(e, k, mmask);
+        FvFlatSearchParams p = flat_params(e, k, mmask, d_user != nullptr);
         rc = fv_flat_search(&e->flat, d_q, nq, &p, main_out, s.ws.p, s.ws.n, st);
     } else {
         rc = fv_ivf_pq_search(&e->ivf, e->pq_codes_lo.p, d_q, nq, nullptr, k, nullptr,

@@ -27,6 +27,7 @@
 //    (distance, id) order.
 // No MFMA: the scan is bound by its LUT lookups in LDS and by list bytes.
 #include "fv_ivfpq.hpp"
+#include "fv_workspace.hpp"
 
 namespace fv {
 
@@ -593,14 +594,12 @@ int cuda_ivf_pq_search_batch(const IvfPqGpuTables* tables, const float* d_querie
     const int do_rerank = rerank_k != 1u && tables->d_vectors != nullptr;
     hipStream_t s = static_cast<hipStream_t>(stream);
     fv::IvfPqPlan p = fv::plan_ivfpq(tables, num_queries, np, S, top_k);
-    void* ws = nullptr;
-    hipError_t e = hipMallocAsync(&ws, p.workspace_bytes, s);
+    fv::WorkspaceLease ws;
+    hipError_t e = fv::stream_workspace(s, p.workspace_bytes, ws);
     if (e != hipSuccess) return (int)e;
     e = fv::launch_ivfpq(tables, nullptr, d_queries, num_queries, nullptr, top_k, nullptr, np,
-                         do_rerank, (int)metric, d_valid_mask, d_out_results, top_k, ws, p, s);
-    hipError_t e2 = hipFreeAsync(ws, s);
-    if (e != hipSuccess) return (int)e;
-    return e2 == hipSuccess ? 0 : (int)e2;
+                         do_rerank, (int)metric, d_valid_mask, d_out_results, top_k, ws.ptr, p, s);
+    return e == hipSuccess ? 0 : (int)e;
 }
 
 int fv_ivf_pq_list_order_codes(const IvfPqGpuTables* tables, uint8_t* d_codes_out, void* stream) {

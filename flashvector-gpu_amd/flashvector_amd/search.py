@@ -93,6 +93,8 @@ def device_count() -> int:
 
 
 FV_SEARCH_F64_BOUNDARY = 1
+# valid_count value: compact the mask with its popcount taken on the device.
+VALID_COUNT_DEVICE = 0xFFFFFFFF
 
 
 class FlatIndex:

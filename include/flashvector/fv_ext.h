@@ -73,12 +73,16 @@ int fv_encode_flat(const float* d_vectors, uint32_t num_vectors, uint32_t dim,
  * k_i; at most 7936 rows. valid_count is the number of set bits in the mask
  * when the caller knows it (0 = unknown): a mask keeping at most half of the
  * rows is then compacted on the device and only the valid rows are scored.
+ * FV_VALID_COUNT_DEVICE: the mask is always compacted and the count is read
+ * on the device, where the scan sizes its row chunks from it (no host
+ * count, no synchronization; the workspace is planned for every row).
  * FV_SEARCH_F64_BOUNDARY (needs d_vectors): reranked candidates whose FP32
  * distances are within twice their rounding-error bound of a neighbour are
  * re-scored in FP64 and ordered by (FP64 distance, row), so membership and
  * order of the returned top k follow the exact distances; those rows report
  * the FP64 distance rounded to FP32. */
 #define FV_SEARCH_F64_BOUNDARY 1u
+#define FV_VALID_COUNT_DEVICE 0xFFFFFFFFu
 typedef struct {
     const uint32_t* d_valid_mask; /* LSB-first u32 words, NULL = every row */
     const uint32_t* d_top_k;      /* per-query k (each <= top_k), or NULL */

@@ -128,10 +128,16 @@ def test_sparse_mask_compaction(torch):
     mask = S.random_valid_mask(80000, 0.05, 64)
     vc = int(S.unpack_bits(mask, 80000).sum())
     idx = F.FlatIndex(dev(torch, db), "f16")
-    for count in (vc, 0):                     # compacted, then dense masked scan
+    # Compacted with the host count, dense masked scan, compacted with the
+    # count taken on the device.
+    for count in (vc, 0, F.VALID_COUNT_DEVICE):
         out = idx.search(dev(torch, qs), 10, "l2", dev(torch, mask), valid_count=count)
         torch.cuda.synchronize()
         assert_same(L.split_results(out), O.flat_topk(db, qs, 10, "l2", valid_mask=mask))
+    dense = S.random_valid_mask(80000, 0.95, 66)          # device count, most rows valid
+    out = idx.search(dev(torch, qs), 10, "l2", dev(torch, dense), valid_count=F.VALID_COUNT_DEVICE)
+    torch.cuda.synchronize()
+    assert_same(L.split_results(out), O.flat_topk(db, qs, 10, "l2", valid_mask=dense))
     odd = S.random_valid_mask(80001, 0.2, 65)[: (80000 + 31) // 32]   # tail word bits past n
     out = idx.search(dev(torch, qs), 10, "l2", dev(torch, odd), valid_count=1)
     torch.cuda.synchronize()
