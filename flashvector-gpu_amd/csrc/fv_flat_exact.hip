@@ -11,6 +11,8 @@
 // current stage is computed. Buffers that could overflow are cut by one
 // wavefront (bisection over the distance, exact sort on heavy ties). A second
 // kernel merges the per-chunk lists of each query.
+#include <mutex>
+
 #include "fv_flat_exact.hpp"
 #include "fv_workspace.hpp"
 
@@ -301,7 +303,8 @@ __global__ __launch_bounds__(256) void merge_partials_kernel(
     }
 }
 
-FlatExactPlan plan_flat_exact(uint32_t n, uint32_t dim, uint32_t nq, uint32_t top_k, int metric) {
+FlatExactPlan plan_flat_exact(uint32_t n, uint32_t dim, uint32_t nq, uint32_t top_k, int metric,
+                              size_t max_bytes) {
     FlatExactPlan p{};
     p.keep = top_k;
     // Bisection prunes leave up to keep + 64 keys; one more tile adds 64.
@@ -325,6 +328,18 @@ FlatExactPlan plan_flat_exact(uint32_t n, uint32_t dim, uint32_t nq, uint32_t to
     if (chunks > max_by_rows) chunks = max_by_rows;
     if (chunks > max_by_merge) chunks = max_by_merge;
     if (chunks < 1) chunks = 1;
+    // Workspace cap: candidate buffers and partial lists are per (chunk,
+    // query), the cosine norms fixed.
+    const size_t fixed = metric == METRIC_COSINE ? ((size_t)n + nq) * sizeof(float) + 256 : 256;
+    const size_t per_chunk = (size_t)nq * (p.cap + p.keep) * sizeof(uint64_t);
+    if (max_bytes != ~(size_t)0) {
+        const size_t fit = max_bytes > fixed ? (max_bytes - fixed) / per_chunk : 0;
+        if (fit < 1) {
+            p.workspace_bytes = 0;
+            return p;
+        }
+        if (chunks > fit) chunks = (uint32_t)fit;
+    }
     uint32_t rows = (n + chunks - 1) / chunks;
     rows = (rows + kTN - 1) / kTN * kTN;
     p.rows_per_chunk = rows;
@@ -381,9 +396,73 @@ hipError_t launch_flat_exact(const float* X, const uint32_t* vec_ids, uint32_t n
     return hipGetLastError();
 }
 
+// ---- Static scratch of fv_flat_scan_batch_noalloc --------------------------
+// A file-scope device array, one copy per GPU, held until the process ends.
+// Its host-side name is only a stand-in, never a device address: the address
+// is taken with hipGetSymbolAddress once per device (with that device
+// current) and only that pointer reaches kernels. Nothing in it needs to
+// start at zero: the scan writes every candidate before reading it (counts
+// live in LDS) and writes every partial-list slot.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+constexpr int kNoAllocMaxDevices = 64;
+static std::once_flag g_scratch_once[kNoAllocMaxDevices];
+static void* g_scratch_ptr[kNoAllocMaxDevices];
+static hipError_t g_scratch_err[kNoAllocMaxDevices];
+
+static hipError_t noalloc_scratch(void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::call_once(g_scratch_once[dev], [dev] {
+        void* p = nullptr;
+        g_scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_ptr[dev] = g_scratch_err[dev] == hipSuccess ? p : nullptr;
+    });
+    *out = g_scratch_ptr[dev];
+    return g_scratch_err[dev];
+}
+
 }  // namespace fv
 
 extern "C" {
+
+// No counterpart in the reference: cuda_flat_scan_batch for callers that
+// cannot allocate device memory (HIP graph capture, serving loops that forbid
+// allocation). Same arguments, clamps and results; its scratch is the
+// library's static 64 MiB per-GPU array, so it calls no allocator. Calls that
+// overlap on one GPU share that array and race: serialize them (one stream,
+// or host-side ordering). A call whose scratch cannot fit even one row chunk
+// (about nq * (next_pow2(k + 128) + k) * 8 bytes, plus (n + nq) * 4 for
+// cosine) returns hipErrorOutOfMemory and launches nothing.
+int fv_flat_scan_batch_noalloc(const float* d_vectors, const uint32_t* d_vec_ids,
+                               uint32_t num_vectors, uint32_t dim, const float* d_queries,
+                               uint32_t num_queries, uint32_t top_k, MetricType metric,
+                               const uint32_t* d_valid_mask, GpuSearchResult* d_out_results,
+                               void* stream) {
+    if (!d_vectors || !d_queries || !d_out_results) return -1;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (num_queries == 0 || top_k == 0) return 0;
+    if (top_k > 256u) top_k = 256u;
+    if (num_vectors == 0) {
+        hipError_t e = fv::fill_results(d_out_results, (size_t)num_queries * top_k, 0xFFFFFFFFu,
+                                        0xFFFFFFFFu, s);
+        return e == hipSuccess ? 0 : (int)e;
+    }
+    if (dim == 0) return (int)hipErrorInvalidValue;
+    fv::FlatExactPlan p = fv::plan_flat_exact(num_vectors, dim, num_queries, top_k, (int)metric,
+                                              fv::kNoAllocScratchBytes);
+    if (p.workspace_bytes == 0 || p.workspace_bytes > fv::kNoAllocScratchBytes)
+        return (int)hipErrorOutOfMemory;
+    void* ws = nullptr;
+    hipError_t e = fv::noalloc_scratch(&ws);
+    if (e != hipSuccess) return (int)e;
+    e = fv::launch_flat_exact(d_vectors, d_vec_ids, num_vectors, dim, d_queries, num_queries,
+                              top_k, (int)metric, d_valid_mask, d_out_results, top_k, ws, p, s,
+                              false);
+    return e == hipSuccess ? 0 : (int)e;
+}
 
 // cuda_bridge.h:150-162 / kernels/src/sparse_hybrid.cu:504-524.
 int cuda_flat_scan_batch(const float* d_vectors, const uint32_t* d_vec_ids, uint32_t num_vectors,

@@ -17,7 +17,10 @@ struct FlatExactPlan {
     size_t workspace_bytes;
 };
 
-FlatExactPlan plan_flat_exact(uint32_t n, uint32_t dim, uint32_t nq, uint32_t top_k, int metric);
+// max_bytes: fewer row chunks when the plan's workspace would exceed it
+// (workspace_bytes = 0 when even one chunk does not fit).
+FlatExactPlan plan_flat_exact(uint32_t n, uint32_t dim, uint32_t nq, uint32_t top_k, int metric,
+                              size_t max_bytes = ~(size_t)0);
 
 hipError_t launch_flat_exact(const float* X, const uint32_t* vec_ids, uint32_t n, uint32_t dim,
                              const float* Qm, uint32_t nq, uint32_t top_k, int metric,

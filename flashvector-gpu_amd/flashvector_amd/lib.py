@@ -152,6 +152,7 @@ SIGNATURES = {
     "cuda_sync_stream": (I, [P]),
     "cuda_flat_scan_batch": (I, [P, P, U, U, P, U, U, I, P, P, P]),
     "cuda_merge_topk": (I, [P, U, U, U, U, P, P]),
+    "fv_flat_scan_batch_noalloc": (I, [P, P, U, U, P, U, U, I, P, P, P]),
     "cuda_compute_distances_warp": (I, [P, P, U, U, U, I, P, P]),
     "cuda_bitonic_sort_test": (I, [P, P, P, P, U, P]),
     "fv_flat_exact_workspace_size": (SZ, [U, U, U, U, I]),

@@ -21,6 +21,22 @@ extern "C" {
 #define FV_MAX_TOPK 1024u
 #define FV_MAX_SHORTLIST 16384u
 
+/* ---- exact FP32 flat scan that allocates nothing -----------------------
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): cuda_flat_scan_batch's arguments, clamps and results,
+ * with its scratch in a static 64 MiB array per GPU held by the library until
+ * the process ends. Calls that overlap on one GPU share that array and race;
+ * serialize them. Returns hipErrorOutOfMemory, launching nothing, when the
+ * scratch of one row chunk does not fit: about
+ * num_queries * (next_pow2(k + 128) + k) * 8 bytes (k = min(top_k, 256)),
+ * plus (num_vectors + num_queries) * 4 for cosine. */
+int fv_flat_scan_batch_noalloc(const float* d_vectors, const uint32_t* d_vec_ids,
+                               uint32_t num_vectors, uint32_t dim, const float* d_queries,
+                               uint32_t num_queries, uint32_t top_k, MetricType metric,
+                               const uint32_t* d_valid_mask, GpuSearchResult* d_out_results,
+                               void* stream);
+
 /* ---- exact FP32 flat scan without the 256 clamp ------------------------ */
 size_t fv_flat_exact_workspace_size(uint32_t num_vectors, uint32_t dim, uint32_t num_queries,
                                     uint32_t top_k, MetricType metric);
