@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/flashvector/cuda_bridge.h"
+#include "../../include/flashvector/fv_build.h"
 #include "../../include/flashvector/fv_host.h"
 #include "fv_flat_exact.hpp"
 
@@ -94,6 +95,8 @@ struct PinnedArray {
     ~PinnedArray() { release(); }
 };
 
+constexpr int kFlat = 1, kIvf = 2;
+
 // Per-stream scratch (streams.rs:22-69): everything one search touches.
 struct Scratch {
     hipStream_t stream = nullptr;
@@ -123,7 +126,7 @@ struct FvEngine {
     std::vector<std::unique_ptr<fvh::Scratch>> pool;
     std::atomic<uint64_t> rr{0};
     hipStream_t delta_stream = nullptr;
-    int mode = 0;  // 0 none, 1 flat, 2 IVF-PQ
+    int mode = 0;  // bits: kFlat (flat matrix-core tables), kIvf (IVF-PQ tables)
     uint32_t n_main = 0;
     // flat tables
     fvh::DevArray<float> rows, norms, scales;
@@ -149,6 +152,9 @@ struct FvEngine {
     // Manifest fields this engine does not use but carries from a loaded
     // snapshot to the next save (persistence.rs SnapshotManifest).
     uint32_t snap_m = 32, snap_ef_construction = 128, snap_max_elements = 0;
+    // Parameters of the last fvh_build, reused by fvh_compact.
+    uint32_t build_nlist = 0, build_coarse_iters = 25, build_pq_iters = 25;
+    uint64_t build_seed = 0;
     // delta buffer
     uint32_t n_delta = 0, delta_cap = 0, delta_dead = 0;
     fvh::DevArray<float> delta_rows;
@@ -197,7 +203,7 @@ static hipError_t size_scratch(FvEngine* e) {
     const FvEngineConfig& c = e->cfg;
     const uint32_t B = c.max_batch, K = c.max_top_k;
     size_t ws = 0;
-    if (e->mode == 1) {
+    if (e->mode & kFlat) {
         for (uint32_t nq = B; nq >= 1; nq /= 2) {
             for (int um = 0; um < 2; ++um) {
                 FvFlatSearchParams p = flat_params(e, K, e->valid.p, um == 1);
@@ -206,12 +212,15 @@ static hipError_t size_scratch(FvEngine* e) {
             if (nq == 1) break;
         }
         if (ws == 0) return hipErrorInvalidValue;
-    } else if (e->mode == 2) {
+    }
+    if (e->mode & kIvf) {
+        size_t wi = 0;
         for (uint32_t nq = B; nq >= 1; nq /= 2) {
-            ws = std::max(ws, fv_ivf_pq_workspace_size(&e->ivf, nq, c.ivf_nprobe, c.ivf_shortlist, K));
+            wi = std::max(wi, fv_ivf_pq_workspace_size(&e->ivf, nq, c.ivf_nprobe, c.ivf_shortlist, K));
             if (nq == 1) break;
         }
-        if (ws == 0) return hipErrorInvalidValue;
+        if (wi == 0) return hipErrorInvalidValue;
+        ws = std::max(ws, wi);
     }
     size_t dws = 0;
     if (e->delta_cap)
@@ -273,13 +282,70 @@ static void rebuild_delta_alive(FvEngine* e) {
     }
 }
 
+// Resolved per-call options of one search.
+struct CallPlan {
+    int algo;           // kFlat or kIvf
+    uint32_t nprobe;    // IVF-PQ
+    uint32_t shortlist; // IVF-PQ
+    int rerank;         // IVF-PQ
+    uint32_t factor;    // flat shortlist factor
+};
+
+static int resolve(const FvEngine* e, const FvSearchOptions* o, uint32_t k, CallPlan& cp) {
+    const FvEngineConfig& c = e->cfg;
+    const uint32_t algo = o ? o->algo : FVH_ALGO_DEFAULT;
+    if (algo == FVH_ALGO_DEFAULT) cp.algo = (e->mode & kFlat) ? kFlat : kIvf;
+    else if (algo == FVH_ALGO_FLAT) cp.algo = kFlat;
+    else if (algo == FVH_ALGO_IVFPQ) cp.algo = kIvf;
+    else return (int)hipErrorInvalidValue;
+    if (!(e->mode & cp.algo)) return -1;  // no such tables loaded
+    cp.factor = o && o->shortlist_factor ? o->shortlist_factor : c.shortlist_factor;
+    cp.nprobe = o && o->nprobe ? o->nprobe : c.ivf_nprobe;
+    if (cp.algo == kIvf && cp.nprobe > e->ivf.nlist) cp.nprobe = e->ivf.nlist;
+    if (o && o->rerank_k >= 0) {
+        // ivf_pq_lookup.cu:351-362: S = (rerank_k == 0 ? k : rerank_k), at
+        // least k, at most 10240; rerank unless rerank_k == 1.
+        uint32_t S = o->rerank_k == 0 ? k : (uint32_t)o->rerank_k;
+        if (S < k) S = k;
+        if (S > 10240u) S = 10240u;
+        cp.shortlist = S;
+        cp.rerank = o->rerank_k != 1;
+    } else {
+        cp.shortlist = std::max(c.ivf_shortlist, k);
+        cp.rerank = 1;
+    }
+    return 0;
+}
+
+// Grows the stream's workspace for a call shape beyond the engine's
+// configuration (after the buffer's previous work has finished).
+static hipError_t ensure_ws(Scratch& s, size_t need) {
+    if (need <= s.ws.n) return hipSuccess;
+    hipError_t he;
+    if (s.used && (he = hipEventSynchronize(s.free_ev)) != hipSuccess) return he;
+    return s.ws.alloc(need);
+}
+
 // One device pass of at most max_batch queries (index.rs:2180-2237 for the
 // flat mode, :1979-2108 for IVF-PQ, :1353-1471 for the delta scan and merge).
 static int search_pass(FvEngine* e, Scratch& s, hipStream_t st, const float* d_q, uint32_t nq,
                        uint32_t k, const uint32_t* d_user, uint32_t user_words,
-                       GpuSearchResult* d_out) {
+                       GpuSearchResult* d_out, const CallPlan& cp) {
     hipError_t he;
     const FvEngineConfig& c = e->cfg;
+    int rc;
+    FvFlatSearchParams fp{};
+    size_t need;
+    if (cp.algo == kFlat) {
+        fp = flat_params(e, k, nullptr, d_user != nullptr);
+        fp.shortlist_factor = cp.factor;
+        if (d_user || e->main_dead) fp.d_valid_mask = s.mask.p;
+        need = fv_flat_workspace_size(&e->flat, nq, &fp);
+    } else {
+        need = fv_ivf_pq_workspace_size(&e->ivf, nq, cp.nprobe, cp.shortlist, k);
+    }
+    if (need == 0) return (int)hipErrorInvalidValue;
+    if ((he = ensure_ws(s, need)) != hipSuccess) return (int)he;
     if (s.used && (he = hipStreamWaitEvent(st, s.free_ev, 0)) != hipSuccess) return (int)he;
     const bool has_delta = e->n_delta > e->delta_dead;
     GpuSearchResult* main_out = has_delta ? s.merge.p : d_out;
@@ -294,7 +360,7 @@ static int search_pass(FvEngine* e, Scratch& s, hipStream_t st, const float* d_q
                 e->delta_alive.p, e->delta_ids.p, e->n_delta, d_user, user_words, s.delta_mask.p);
             dmask = s.delta_mask.p;
         }
-        int rc = fv_flat_exact_search(e->delta_rows.p, e->n_delta, c.dim, d_q, nq, k,
+        rc = fv_flat_exact_search(e->delta_rows.p, e->n_delta, c.dim, d_q, nq, k,
                                       (MetricType)c.metric, dmask, delta_out, s.dws.p, s.dws.n,
                                       e->delta_stream);
         if (rc) return rc;
@@ -310,13 +376,11 @@ static int search_pass(FvEngine* e, Scratch& s, hipStream_t st, const float* d_q
             e->valid.p, e->slot2lid.p, e->n_main, d_user, user_words, s.mask.p);
         mmask = s.mask.p;
     }
-    int rc;
-    if (e->mode == 1) {
-        FvFlatSearchParams p = flat_params(e, k, mmask, d_user != nullptr);
-        rc = fv_flat_search(&e->flat, d_q, nq, &p, main_out, s.ws.p, s.ws.n, st);
+    if (cp.algo == kFlat) {
+        rc = fv_flat_search(&e->flat, d_q, nq, &fp, main_out, s.ws.p, s.ws.n, st);
     } else {
         rc = fv_ivf_pq_search(&e->ivf, e->pq_codes_lo.p, d_q, nq, nullptr, k, nullptr,
-                              c.ivf_nprobe, c.ivf_shortlist, 1, (MetricType)c.metric, mmask,
+                              cp.nprobe, cp.shortlist, cp.rerank, (MetricType)c.metric, mmask,
                               main_out, s.ws.p, s.ws.n, st);
     }
     if (rc) return rc;
@@ -416,32 +480,82 @@ static int set_ids(FvEngine* e, const uint32_t* lids, uint32_t n) {
     return (int)he;
 }
 
-int fvh_load_flat(FvEngine* e, const float* h_vectors, uint32_t n, uint32_t dtype,
-                  const uint32_t* h_logical_ids) {
-    if (!e || !h_vectors) return -1;
-    if (n == 0 || (dtype != FV_DTYPE_F16 && dtype != FV_DTYPE_BF16 && dtype != FV_DTYPE_FP8_E4M3))
+}  // extern "C"
+
+namespace fvh {
+
+// Flat matrix-core tables encoded from the FP32 rows already on the device.
+static int attach_flat(FvEngine* e, uint32_t dtype) {
+    const uint32_t n = e->n_main, dim = e->cfg.dim;
+    if (dtype != FV_DTYPE_F16 && dtype != FV_DTYPE_BF16 && dtype != FV_DTYPE_FP8_E4M3)
         return (int)hipErrorInvalidValue;
-    if (!fvh::lids_unique(h_logical_ids, n)) return (int)hipErrorInvalidValue;
-    const uint32_t dim = e->cfg.dim;
     const uint32_t step = dtype == FV_DTYPE_FP8_E4M3 ? 128u : 32u;
     const uint32_t pd = (dim + step - 1) / step * step;
     if (pd > 768) return (int)hipErrorInvalidValue;
     const size_t esz = dtype == FV_DTYPE_FP8_E4M3 ? 1 : 2;
     hipError_t he;
-    if ((he = e->rows.alloc((size_t)n * dim)) != hipSuccess ||
-        (he = e->codes.alloc((size_t)n * pd * esz)) != hipSuccess ||
+    if ((he = e->codes.alloc((size_t)n * pd * esz)) != hipSuccess ||
         (he = e->norms.alloc(n)) != hipSuccess || (he = e->norms64.alloc(n)) != hipSuccess ||
-        (he = e->scales.alloc(dtype == FV_DTYPE_FP8_E4M3 ? n : 0)) != hipSuccess ||
-        (he = fvh::upload(e->rows.p, h_vectors, (size_t)n * dim * 4)) != hipSuccess)
+        (he = e->scales.alloc(dtype == FV_DTYPE_FP8_E4M3 ? n : 0)) != hipSuccess)
         return (int)he;
     int rc = fv_encode_flat(e->rows.p, n, dim, pd, dtype, e->codes.p, e->norms.p, e->norms64.p,
                             e->scales.p, nullptr);
     if (rc) return rc;
     if ((he = hipDeviceSynchronize()) != hipSuccess) return (int)he;
     e->flat = FvFlatTables{e->codes.p, e->scales.p, e->norms.p, e->norms64.p, e->rows.p, n, dim, pd, dtype};
+    e->mode |= kFlat;
+    return 0;
+}
+
+static int alloc_ivf(FvEngine* e, uint32_t nlist, uint32_t m_pq) {
+    const uint32_t n = e->n_main, dim = e->cfg.dim;
+    hipError_t he;
+    if ((he = e->centroids.alloc((size_t)nlist * dim)) != hipSuccess ||
+        (he = e->codebooks.alloc((size_t)m_pq * 256 * (dim / m_pq))) != hipSuccess ||
+        (he = e->pq_codes.alloc((size_t)n * m_pq)) != hipSuccess ||
+        (he = e->pq_codes_lo.alloc((size_t)n * m_pq)) != hipSuccess ||
+        (he = e->offsets.alloc(nlist + 1)) != hipSuccess || (he = e->vec_ids.alloc(n)) != hipSuccess)
+        return (int)he;
+    return 0;
+}
+
+// IVF-PQ tables in place (centroids ... vec_ids filled): list-ordered codes.
+static int finish_ivf(FvEngine* e, uint32_t nlist, uint32_t m_pq) {
+    const uint32_t dim = e->cfg.dim;
+    e->ivf = IvfPqGpuTables{e->centroids.p, e->codebooks.p, e->pq_codes.p, e->offsets.p,
+                            e->vec_ids.p, e->rows.p, e->n_main, dim, nlist, m_pq, dim / m_pq};
+    int rc = fv_ivf_pq_list_order_codes(&e->ivf, e->pq_codes_lo.p, nullptr);
+    if (rc) return rc;
+    hipError_t he = hipDeviceSynchronize();
+    if (he != hipSuccess) return (int)he;
+    e->mode |= kIvf;
+    return 0;
+}
+
+static int upload_rows(FvEngine* e, const float* h_vectors, uint32_t n) {
+    hipError_t he;
+    if ((he = e->rows.alloc((size_t)n * e->cfg.dim)) != hipSuccess ||
+        (he = upload(e->rows.p, h_vectors, (size_t)n * e->cfg.dim * 4)) != hipSuccess)
+        return (int)he;
     e->n_main = n;
-    e->mode = 1;
-    if ((rc = set_ids(e, h_logical_ids, n))) return rc;
+    e->mode = 0;
+    return 0;
+}
+
+}  // namespace fvh
+
+extern "C" {
+
+int fvh_load_flat(FvEngine* e, const float* h_vectors, uint32_t n, uint32_t dtype,
+                  const uint32_t* h_logical_ids) {
+    if (!e || !h_vectors) return -1;
+    if (n == 0 || (dtype != FV_DTYPE_F16 && dtype != FV_DTYPE_BF16 && dtype != FV_DTYPE_FP8_E4M3))
+        return (int)hipErrorInvalidValue;
+    if (!fvh::lids_unique(h_logical_ids, n)) return (int)hipErrorInvalidValue;
+    int rc;
+    if ((rc = fvh::upload_rows(e, h_vectors, n)) || (rc = fvh::attach_flat(e, dtype)) ||
+        (rc = set_ids(e, h_logical_ids, n)))
+        return rc;
     return (int)fvh::size_scratch(e);
 }
 
@@ -454,29 +568,135 @@ int fvh_load_ivfpq(FvEngine* e, const float* h_centroids, uint32_t nlist, const 
     const uint32_t dim = e->cfg.dim;
     if (n == 0 || nlist == 0 || m_pq == 0 || dim % m_pq) return (int)hipErrorInvalidValue;
     if (!fvh::lids_unique(h_logical_ids, n)) return (int)hipErrorInvalidValue;
+    int rc;
+    if ((rc = fvh::upload_rows(e, h_vectors, n)) || (rc = fvh::alloc_ivf(e, nlist, m_pq)))
+        return rc;
     hipError_t he;
-    if ((he = e->centroids.alloc((size_t)nlist * dim)) != hipSuccess ||
-        (he = e->codebooks.alloc((size_t)m_pq * 256 * (dim / m_pq))) != hipSuccess ||
-        (he = e->pq_codes.alloc((size_t)n * m_pq)) != hipSuccess ||
-        (he = e->pq_codes_lo.alloc((size_t)n * m_pq)) != hipSuccess ||
-        (he = e->offsets.alloc(nlist + 1)) != hipSuccess || (he = e->vec_ids.alloc(n)) != hipSuccess ||
-        (he = e->rows.alloc((size_t)n * dim)) != hipSuccess ||
-        (he = fvh::upload(e->centroids.p, h_centroids, (size_t)nlist * dim * 4)) != hipSuccess ||
+    if ((he = fvh::upload(e->centroids.p, h_centroids, (size_t)nlist * dim * 4)) != hipSuccess ||
         (he = fvh::upload(e->codebooks.p, h_codebooks, (size_t)m_pq * 256 * (dim / m_pq) * 4)) != hipSuccess ||
         (he = fvh::upload(e->pq_codes.p, h_codes, (size_t)n * m_pq)) != hipSuccess ||
         (he = fvh::upload(e->offsets.p, h_offsets, (size_t)(nlist + 1) * 4)) != hipSuccess ||
-        (he = fvh::upload(e->vec_ids.p, h_vec_ids, (size_t)n * 4)) != hipSuccess ||
-        (he = fvh::upload(e->rows.p, h_vectors, (size_t)n * dim * 4)) != hipSuccess)
+        (he = fvh::upload(e->vec_ids.p, h_vec_ids, (size_t)n * 4)) != hipSuccess)
         return (int)he;
-    e->ivf = IvfPqGpuTables{e->centroids.p, e->codebooks.p, e->pq_codes.p, e->offsets.p,
-                            e->vec_ids.p, e->rows.p, n, dim, nlist, m_pq, dim / m_pq};
-    int rc = fv_ivf_pq_list_order_codes(&e->ivf, e->pq_codes_lo.p, nullptr);
-    if (rc) return rc;
-    if ((he = hipDeviceSynchronize()) != hipSuccess) return (int)he;
-    e->n_main = n;
-    e->mode = 2;
-    if ((rc = set_ids(e, h_logical_ids, n))) return rc;
+    if ((rc = fvh::finish_ivf(e, nlist, m_pq)) || (rc = set_ids(e, h_logical_ids, n))) return rc;
     return (int)fvh::size_scratch(e);
+}
+
+}  // extern "C"
+
+namespace fvh {
+
+static int build_tables(FvEngine* e, const float* h_vectors, uint32_t n, const uint32_t* lids,
+                        uint32_t flat_dtype, uint32_t nlist, uint32_t m_pq, uint32_t coarse_iters,
+                        uint32_t pq_iters, uint64_t seed);
+
+}  // namespace fvh
+
+extern "C" {
+
+int fvh_build(FvEngine* e, const float* h_vectors, uint32_t n, uint32_t flat_dtype,
+              uint32_t nlist, uint32_t m_pq, uint32_t coarse_iters, uint32_t pq_iters,
+              uint64_t seed) {
+    if (!e || !h_vectors) return -1;
+    return fvh::build_tables(e, h_vectors, n, nullptr, flat_dtype, nlist, m_pq, coarse_iters,
+                             pq_iters, seed);
+}
+
+int fvh_compact(FvEngine* e) {
+    using namespace fvh;
+    if (!e) return -1;
+    if (e->mode == 0) return -1;
+    if (e->tombstones.empty() && e->n_delta == 0) return 0;  // index.rs:719-721
+    const uint32_t dim = e->cfg.dim;
+    hipError_t he = hipDeviceSynchronize();
+    if (he != hipSuccess) return (int)he;
+    // Live main rows in slot order, then live delta rows (index.rs:736-750).
+    std::vector<float> main_rows((size_t)e->n_main * dim), delta_rows((size_t)e->n_delta * dim);
+    if ((he = hipMemcpy(main_rows.data(), e->rows.p, main_rows.size() * 4,
+                        hipMemcpyDeviceToHost)) != hipSuccess ||
+        (e->n_delta && (he = hipMemcpy(delta_rows.data(), e->delta_rows.p,
+                                       delta_rows.size() * 4, hipMemcpyDeviceToHost)) != hipSuccess))
+        return (int)he;
+    std::vector<float> rows;
+    std::vector<uint32_t> lids;
+    rows.reserve(main_rows.size() + delta_rows.size());
+    for (uint32_t sl = 0; sl < e->n_main; ++sl) {
+        const uint32_t lid = e->h_slot2lid.empty() ? sl : e->h_slot2lid[sl];
+        if (e->tombstones.count(lid)) continue;
+        lids.push_back(lid);
+        rows.insert(rows.end(), main_rows.begin() + (size_t)sl * dim,
+                    main_rows.begin() + (size_t)(sl + 1) * dim);
+    }
+    for (uint32_t j = 0; j < e->n_delta; ++j) {
+        const uint32_t lid = e->h_delta_ids[j];
+        if (e->tombstones.count(lid)) continue;
+        lids.push_back(lid);
+        rows.insert(rows.end(), delta_rows.begin() + (size_t)j * dim,
+                    delta_rows.begin() + (size_t)(j + 1) * dim);
+    }
+    if (lids.empty()) return (int)hipErrorInvalidValue;  // nothing left to index
+    const uint64_t watermark = e->next_lid;
+    const uint32_t fd = (e->mode & kFlat) ? e->flat.dtype : 0u;
+    const uint32_t nl = (e->mode & kIvf) ? std::max(e->build_nlist, e->ivf.nlist) : 0u;
+    const uint32_t mp = (e->mode & kIvf) ? e->ivf.m_pq : 0u;
+    const int rc = build_tables(e, rows.data(), (uint32_t)lids.size(), lids.data(), fd, nl, mp,
+                                e->build_coarse_iters, e->build_pq_iters, e->build_seed);
+    // index.rs:796-799: the id counter never goes back.
+    e->next_lid = std::max(e->next_lid, watermark);
+    return rc;
+}
+
+}  // extern "C"
+
+namespace fvh {
+
+static int build_tables(FvEngine* e, const float* h_vectors, uint32_t n, const uint32_t* lids,
+                        uint32_t flat_dtype, uint32_t nlist, uint32_t m_pq, uint32_t coarse_iters,
+                        uint32_t pq_iters, uint64_t seed) {
+    const uint32_t dim = e->cfg.dim;
+    if (n == 0 || (flat_dtype == 0 && (nlist == 0 || m_pq == 0))) return (int)hipErrorInvalidValue;
+    if (m_pq && dim % m_pq) return (int)hipErrorInvalidValue;
+    if (!lids_unique(lids, n)) return (int)hipErrorInvalidValue;
+    e->build_nlist = nlist;
+    e->build_coarse_iters = coarse_iters;
+    e->build_pq_iters = pq_iters;
+    e->build_seed = seed;
+    int rc;
+    if ((rc = fvh::upload_rows(e, h_vectors, n))) return rc;
+    if (flat_dtype && (rc = fvh::attach_flat(e, flat_dtype))) return rc;
+    if (nlist && m_pq) {
+        const uint32_t nl = std::min(nlist, n);
+        if ((rc = fvh::alloc_ivf(e, nl, m_pq))) return rc;
+        const size_t wsb = fv_ivf_pq_build_workspace_size(n, dim, nl, m_pq);
+        fvh::DevArray<uint8_t> ws;
+        hipError_t he;
+        if ((he = ws.alloc(std::max<size_t>(wsb, 1))) != hipSuccess) return (int)he;
+        if ((rc = fv_ivf_pq_build(e->rows.p, n, dim, nl, m_pq, coarse_iters, pq_iters, seed,
+                                  e->centroids.p, e->codebooks.p, e->pq_codes.p, e->offsets.p,
+                                  e->vec_ids.p, nullptr, ws.p, ws.n, nullptr)))
+            return rc;
+        if ((rc = fvh::finish_ivf(e, nl, m_pq))) return rc;
+    }
+    if ((rc = set_ids(e, lids, n))) return rc;
+    return (int)fvh::size_scratch(e);
+}
+
+}  // namespace fvh
+
+extern "C" {
+
+int fvh_info(const FvEngine* e, FvEngineInfo* out) {
+    if (!e || !out) return -1;
+    *out = FvEngineInfo{};
+    out->tables = (uint32_t)e->mode;
+    out->num_vectors = e->n_main;
+    out->delta_len = e->n_delta;
+    out->tombstones = (uint32_t)e->tombstones.size();
+    out->next_logical_id = (uint32_t)std::min<uint64_t>(e->next_lid, UINT32_MAX);
+    out->nlist = (e->mode & fvh::kIvf) ? e->ivf.nlist : 0u;
+    out->m_pq = (e->mode & fvh::kIvf) ? e->ivf.m_pq : 0u;
+    out->flat_dtype = (e->mode & fvh::kFlat) ? e->flat.dtype : 0u;
+    return 0;
 }
 
 int fvh_add_delta(FvEngine* e, const float* h_vectors, const uint32_t* h_logical_ids, uint32_t n) {
@@ -587,9 +807,17 @@ int fvh_next_logical_id(const FvEngine* e, uint32_t* out) {
 
 int fvh_search(FvEngine* e, const float* h_queries, uint32_t nq, uint32_t top_k,
                const uint32_t* h_mask_words, uint32_t mask_words, GpuSearchResult* h_out) {
+    return fvh_search_ex(e, h_queries, nq, top_k, nullptr, h_mask_words, mask_words, h_out);
+}
+
+int fvh_search_ex(FvEngine* e, const float* h_queries, uint32_t nq, uint32_t top_k,
+                  const FvSearchOptions* opts, const uint32_t* h_mask_words, uint32_t mask_words,
+                  GpuSearchResult* h_out) {
     if (!e || !h_queries || !h_out) return -1;
     int rc = fvh::check_search(e, top_k);
     if (rc) return rc;
+    fvh::CallPlan cp{};
+    if ((rc = fvh::resolve(e, opts, top_k, cp))) return rc;
     if (nq == 0) return 0;
     const auto t0 = std::chrono::steady_clock::now();
     fvh::Scratch& s = fvh::pick(e);
@@ -614,7 +842,8 @@ int fvh_search(FvEngine* e, const float* h_queries, uint32_t nq, uint32_t top_k,
         if ((he = hipMemcpyAsync(s.q.p, s.hq.p, (size_t)nb * dim * 4, hipMemcpyHostToDevice,
                                  s.stream)) != hipSuccess)
             return (int)he;
-        if ((rc = fvh::search_pass(e, s, s.stream, s.q.p, nb, top_k, d_user, uw, s.res.p))) return rc;
+        if ((rc = fvh::search_pass(e, s, s.stream, s.q.p, nb, top_k, d_user, uw, s.res.p, cp)))
+            return rc;
         if ((he = hipMemcpyAsync(s.hres.p, s.res.p, (size_t)nb * top_k * sizeof(GpuSearchResult),
                                  hipMemcpyDeviceToHost, s.stream)) != hipSuccess ||
             (he = hipStreamSynchronize(s.stream)) != hipSuccess)
@@ -631,9 +860,18 @@ int fvh_search(FvEngine* e, const float* h_queries, uint32_t nq, uint32_t top_k,
 int fvh_search_device(FvEngine* e, const float* d_queries, uint32_t nq, uint32_t top_k,
                       const uint32_t* d_mask_words, uint32_t mask_words, GpuSearchResult* d_out,
                       void* stream) {
+    return fvh_search_device_ex(e, d_queries, nq, top_k, nullptr, d_mask_words, mask_words, d_out,
+                                stream);
+}
+
+int fvh_search_device_ex(FvEngine* e, const float* d_queries, uint32_t nq, uint32_t top_k,
+                         const FvSearchOptions* opts, const uint32_t* d_mask_words,
+                         uint32_t mask_words, GpuSearchResult* d_out, void* stream) {
     if (!e || !d_queries || !d_out) return -1;
     int rc = fvh::check_search(e, top_k);
     if (rc) return rc;
+    fvh::CallPlan cp{};
+    if ((rc = fvh::resolve(e, opts, top_k, cp))) return rc;
     if (nq == 0) return 0;
     const auto t0 = std::chrono::steady_clock::now();
     fvh::Scratch& s = fvh::pick(e);
@@ -643,7 +881,7 @@ int fvh_search_device(FvEngine* e, const float* d_queries, uint32_t nq, uint32_t
     for (uint32_t q0 = 0; q0 < nq; q0 += B) {
         const uint32_t nb = std::min(B, nq - q0);
         if ((rc = fvh::search_pass(e, s, st, d_queries + (size_t)q0 * dim, nb, top_k, d_mask_words,
-                                   d_mask_words ? mask_words : 0u, d_out + (size_t)q0 * top_k)))
+                                   d_mask_words ? mask_words : 0u, d_out + (size_t)q0 * top_k, cp)))
             return rc;
     }
     e->queries.fetch_add(nq, std::memory_order_relaxed);
@@ -1067,7 +1305,7 @@ int fvh_save_snapshot(FvEngine* e, const char* path) {
     I.num_vectors = n;
     uint32_t nlist = 0, m_pq = 0;
     const char* extra_name = nullptr;
-    if (e->mode == 2) {
+    if (e->mode & kIvf) {
         nlist = e->ivf.nlist;
         m_pq = e->ivf.m_pq;
         if ((he = download(centroids, e->centroids.p, (size_t)nlist * dim)) != hipSuccess ||
@@ -1076,7 +1314,8 @@ int fvh_save_snapshot(FvEngine* e, const char* path) {
             (he = download(offsets, e->offsets.p, (size_t)nlist + 1)) != hipSuccess ||
             (he = download(vec_ids, e->vec_ids.p, (size_t)n)) != hipSuccess)
             return (int)he;
-    } else {
+    }
+    if (e->mode & kFlat) {
         const FvFlatTables& t = e->flat;
         const size_t esz = t.dtype == FV_DTYPE_FP8_E4M3 ? 1 : 2;
         if ((he = download(codes, static_cast<const uint8_t*>(t.d_codes), (size_t)n * t.padded_dim * esz)) != hipSuccess ||
@@ -1174,7 +1413,12 @@ int fvh_load_snapshot(FvEngine* e, const char* path, uint32_t flat_dtype) {
     bool remapped = false;
     for (uint32_t sl = 0; sl < n && !remapped; ++sl) remapped = lids[sl] != sl;
     const uint32_t* use_lids = remapped ? lids : nullptr;
-    if (flat_dtype == 0) {
+    // Low byte: FvDtype of flat tables encoded from the rows (0 = none);
+    // FVH_LOAD_IVFPQ: the file's IVF-PQ tables (also implied by 0).
+    const uint32_t dtype = flat_dtype & 0xFFu;
+    const bool want_ivf = dtype == 0 || (flat_dtype & FVH_LOAD_IVFPQ);
+    if ((flat_dtype & ~(0xFFu | FVH_LOAD_IVFPQ)) != 0) return (int)hipErrorInvalidValue;
+    if (want_ivf) {
         const float* cent; const float* books; const uint8_t* codes; const uint32_t* off;
         const uint32_t* vids;
         size_t c_cent, c_books, c_codes, c_off, c_vids;
@@ -1196,8 +1440,12 @@ int fvh_load_snapshot(FvEngine* e, const char* path, uint32_t flat_dtype) {
         for (uint32_t c = 0; c < nlist; ++c)
             if (off[c] > off[c + 1]) return FVH_ERR_FORMAT;
         rc = fvh_load_ivfpq(e, cent, nlist, books, m_pq, codes, off, vids, rows, n, use_lids);
+        if (rc == 0 && dtype) {
+            rc = attach_flat(e, dtype);
+            if (rc == 0) rc = (int)size_scratch(e);
+        }
     } else {
-        rc = fvh_load_flat(e, rows, n, flat_dtype, use_lids);
+        rc = fvh_load_flat(e, rows, n, dtype, use_lids);
     }
     if (rc) return rc;
     const float* drows; const uint32_t* dids;

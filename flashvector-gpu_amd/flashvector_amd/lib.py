@@ -93,6 +93,21 @@ class FvEngineConfig(ctypes.Structure):
         "ivf_nprobe", "ivf_shortlist", "reserved")]
 
 
+class FvSearchOptions(ctypes.Structure):
+    """include/flashvector/fv_host.h FvSearchOptions."""
+
+    _fields_ = [("algo", ctypes.c_uint32), ("nprobe", ctypes.c_uint32),
+                ("rerank_k", ctypes.c_int32), ("shortlist_factor", ctypes.c_uint32)]
+
+
+class FvEngineInfo(ctypes.Structure):
+    """include/flashvector/fv_host.h FvEngineInfo."""
+
+    _fields_ = [(n, ctypes.c_uint32) for n in (
+        "tables", "num_vectors", "delta_len", "tombstones", "next_logical_id", "nlist", "m_pq",
+        "flat_dtype")]
+
+
 class FvFlatTuning(ctypes.Structure):
     """include/flashvector/fv_ext.h FvFlatTuning."""
 
@@ -174,8 +189,13 @@ SIGNATURES = {
     "fvh_add_vectors": (I, [P, P, U, P]),
     "fvh_delete": (I, [P, P, U]),
     "fvh_next_logical_id": (I, [P, P]),
+    "fvh_build": (I, [P, P, U, U, U, U, U, U, ctypes.c_uint64]),
+    "fvh_compact": (I, [P]),
     "fvh_search": (I, [P, P, U, U, P, U, P]),
+    "fvh_search_ex": (I, [P, P, U, U, P, P, U, P]),
     "fvh_search_device": (I, [P, P, U, U, P, U, P, P]),
+    "fvh_search_device_ex": (I, [P, P, U, U, P, P, U, P, P]),
+    "fvh_info": (I, [P, P]),
     "fvh_stats": (I, [P, P, P]),
     "fvh_snapshot_write": (I, [ctypes.c_char_p, P, P, U]),
     "fvh_snapshot_open": (I, [ctypes.c_char_p, P, P]),

@@ -68,6 +68,25 @@ int fvh_load_ivfpq(FvEngine* e, const float* h_centroids, uint32_t nlist,
                    const uint32_t* h_offsets, const uint32_t* h_vec_ids, const float* h_vectors,
                    uint32_t n, const uint32_t* h_logical_ids);
 
+/* crates/engine/src/index.rs:405-458 build: uploads FP32 rows [n * dim] and
+ * builds, on the device, flat tables of `flat_dtype` (FvDtype 1..3, 0 = none)
+ * and IVF-PQ tables with min(nlist, n) lists and m_pq sub-quantizers
+ * (nlist = 0 = none) by fv_ivf_pq_build (coarse_iters / pq_iters Lloyd
+ * iterations, reference 25 / 25; deterministic in `seed`). Identity logical
+ * ids; the next fvh_add_vectors id is n. One index may hold both kinds of
+ * tables; each search picks one (FvSearchOptions.algo). */
+int fvh_build(FvEngine* e, const float* h_vectors, uint32_t n, uint32_t flat_dtype,
+              uint32_t nlist, uint32_t m_pq, uint32_t coarse_iters, uint32_t pq_iters,
+              uint64_t seed);
+
+/* crates/engine/src/index.rs:716-752 compact: rebuilds the index from its
+ * live rows (main rows in slot order, then delta rows) under their logical
+ * ids, with the tables it holds (same flat dtype; IVF-PQ with the last
+ * build's nlist, iterations and seed, or the loaded tables' nlist); the
+ * delta buffer and the tombstones are emptied and the id counter is kept.
+ * A no-op without tombstones or delta rows. Synchronizes the device. */
+int fvh_compact(FvEngine* e);
+
 /* Appends rows with caller-chosen logical ids to the delta buffer (exact
  * FP32 scan, merged into every search result). Logical ids are unique over
  * the main index, the delta buffer and the deleted ids: a used id, or one
@@ -91,9 +110,33 @@ int fvh_delete(FvEngine* e, const uint32_t* h_logical_ids, uint32_t n);
 int fvh_next_logical_id(const FvEngine* e, uint32_t* out);
 
 /* Host queries [nq * dim] -> host results [nq * top_k]
- * (index.rs:2180-2237 batch_search_fp16_tc, :1979-2108 batch_search_ivf_mode). */
+ * (index.rs:2180-2237 batch_search_fp16_tc, :1979-2108 batch_search_ivf_mode),
+ * with the engine's default options. */
 int fvh_search(FvEngine* e, const float* h_queries, uint32_t nq, uint32_t top_k,
                const uint32_t* h_mask_words, uint32_t mask_words, GpuSearchResult* h_out);
+
+/* Per-call options (crates/python/src/py_index.rs:361-436 search_ivf /
+ * search_fp16 arguments). algo: which tables serve the call (DEFAULT = flat
+ * if loaded, else IVF-PQ; -1 if the engine holds no such tables). nprobe:
+ * IVF-PQ probes, 0 = the engine's ivf_nprobe, clamped to nlist. rerank_k:
+ * IVF-PQ shortlist with the reference's meaning (ivf_pq_lookup.cu:351-362:
+ * 0 = k, 1 = ADC order without rerank, else max(k, rerank_k), at most 10240);
+ * negative = the engine's ivf_shortlist with rerank. shortlist_factor: flat
+ * shortlist = factor * k, 0 = the engine's. A call shape beyond the engine's
+ * configuration grows that stream's workspace first. */
+#define FVH_ALGO_DEFAULT 0u
+#define FVH_ALGO_FLAT 1u
+#define FVH_ALGO_IVFPQ 2u
+typedef struct {
+    uint32_t algo;
+    uint32_t nprobe;
+    int32_t rerank_k;
+    uint32_t shortlist_factor;
+} FvSearchOptions;
+
+int fvh_search_ex(FvEngine* e, const float* h_queries, uint32_t nq, uint32_t top_k,
+                  const FvSearchOptions* opts, const uint32_t* h_mask_words, uint32_t mask_words,
+                  GpuSearchResult* h_out);
 
 /* Device queries -> device results on the caller's stream (the DLPack path
  * of crates/python/src/dlpack.rs:15-121 and index.rs:2110-2178, without the
@@ -102,6 +145,23 @@ int fvh_search(FvEngine* e, const float* h_queries, uint32_t nq, uint32_t top_k,
 int fvh_search_device(FvEngine* e, const float* d_queries, uint32_t nq, uint32_t top_k,
                       const uint32_t* d_mask_words, uint32_t mask_words, GpuSearchResult* d_out,
                       void* stream);
+
+int fvh_search_device_ex(FvEngine* e, const float* d_queries, uint32_t nq, uint32_t top_k,
+                         const FvSearchOptions* opts, const uint32_t* d_mask_words,
+                         uint32_t mask_words, GpuSearchResult* d_out, void* stream);
+
+/* Engine state (index.rs:2520-2547 num_vectors / delta_len / tombstone_count
+ * / next_logical_id). tables: bit 1 flat, bit 2 IVF-PQ. */
+typedef struct {
+    uint32_t tables;
+    uint32_t num_vectors;     /* rows of the main index */
+    uint32_t delta_len;       /* rows of the delta buffer, deleted ones included */
+    uint32_t tombstones;      /* deleted logical ids */
+    uint32_t next_logical_id;
+    uint32_t nlist, m_pq;     /* IVF-PQ tables, or 0 */
+    uint32_t flat_dtype;      /* FvDtype of the flat tables, or 0 */
+} FvEngineInfo;
+int fvh_info(const FvEngine* e, FvEngineInfo* out);
 
 /* Queries served and summed wall time of fvh_search* calls (metrics.rs). */
 int fvh_stats(const FvEngine* e, uint64_t* queries, double* total_ms);
@@ -207,8 +267,10 @@ int fvh_save_snapshot(FvEngine* e, const char* path);
 /* Loads a snapshot into an engine created with the same dim and metric
  * (index.rs:1033-1258 for the sections above): flat_dtype 0 loads the IVF-PQ
  * tables (which must be present), FvDtype 1..3 a flat index encoded from the
- * FP32 rows; then the delta rows, then the tombstones (or, without a
- * tombstones section, the cleared bits of main_valid_words). */
+ * FP32 rows, FvDtype | FVH_LOAD_IVFPQ both; then the delta rows, then the
+ * tombstones (or, without a tombstones section, the cleared bits of
+ * main_valid_words). */
+#define FVH_LOAD_IVFPQ 0x100u
 int fvh_load_snapshot(FvEngine* e, const char* path, uint32_t flat_dtype);
 
 #ifdef __cplusplus
