@@ -66,6 +66,49 @@ __global__ __launch_bounds__(256) void ivf_coarse_kernel(
     }
 }
 
+// Coarse probe for lists too many to sort in LDS: every centroid's key
+// (distance as in ivf_coarse_kernel, cluster id; keys of unadmitted
+// distances have all-ones distance bits and stay distinct) goes to the
+// query's row of `keys`, and a radix selection keeps the np smallest there.
+// The probes come out in no particular order: the scan's result depends only
+// on the set of probed lists.
+__global__ __launch_bounds__(kSelThreads) void ivf_coarse_global_kernel(
+    const float* __restrict__ C, uint32_t nlist, uint32_t dim, const float* __restrict__ Qm,
+    const uint32_t* __restrict__ nprobe_q, uint32_t nprobe_u, uint32_t max_nprobe,
+    uint64_t* __restrict__ keys, uint32_t* __restrict__ probes) {
+    __shared__ BlockSelectLds L;
+    const uint32_t q = blockIdx.x;
+    const float* qv = Qm + (size_t)q * dim;
+    uint64_t* buf = keys + (size_t)q * nlist;
+    for (uint32_t c = threadIdx.x; c < nlist; c += blockDim.x) {
+        const float* cen = C + (size_t)c * dim;
+        float dist = 0.0f;
+        for (uint32_t d = 0; d < dim; ++d) {
+            float diff = qv[d] - cen[d];
+            dist = __builtin_fmaf(diff, diff, dist);
+        }
+        buf[c] = dist < kAdmit ? make_key(dist, c) : (0xFFFFFFFF00000000ull | c);
+    }
+    __syncthreads();
+    uint32_t np = nprobe_q ? nprobe_q[q] : nprobe_u;
+    if (np == 0) np = 1;
+    if (np > nlist) np = nlist;
+    if (np > max_nprobe) np = max_nprobe;  // the probe row holds max_nprobe slots
+    if (np < nlist) (void)block_select(buf, nlist, np, L);
+    // Admitted probes packed at the front (the scan stops at the first empty
+    // one), empty slots after them.
+    __shared__ uint32_t s_n;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    uint32_t* pq = probes + (size_t)q * max_nprobe;
+    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) {
+        const uint64_t k = buf[i];
+        if ((k >> 32) != 0xFFFFFFFFull) pq[atomicAdd(&s_n, 1u)] = key_slot(k);
+    }
+    __syncthreads();
+    for (uint32_t i = s_n + threadIdx.x; i < max_nprobe; i += blockDim.x) pq[i] = FV_EMPTY_ID;
+}
+
 // Workgroup control words and selection scratch of the scan kernel, in
 // dynamic LDS after the LUT and the candidate buffer.
 struct IvfCtl {
@@ -479,11 +522,15 @@ IvfPqPlan plan_ivfpq(const IvfPqGpuTables* t, uint32_t nq, uint32_t max_nprobe,
     p.off_order = take((size_t)nq * 4);
     p.off_counters = take(68 * 4);  // 33 counts, 33 fills, next query, pad
     p.off_gbuf = take(p.gbuf ? (size_t)p.grid * p.cap * 8 : 0);
-    p.tiled_coarse = max_nprobe <= 256u && t->nlist >= 64u;
-    if (p.tiled_coarse) {
+    if (max_nprobe <= kTiledCoarseMaxProbes && t->nlist >= 64u) p.coarse_mode = kCoarseTiled;
+    else if (p.npow <= kLdsCoarseMaxLists) p.coarse_mode = kCoarseLds;
+    else p.coarse_mode = kCoarseGlobal;
+    if (p.coarse_mode == kCoarseTiled) {
         p.coarse_plan = plan_flat_exact(t->nlist, t->dim, nq, max_nprobe, METRIC_L2);
         p.off_coarse_res = take((size_t)nq * max_nprobe * sizeof(GpuSearchResult));
         p.off_coarse_ws = take(p.coarse_plan.workspace_bytes);
+    } else if (p.coarse_mode == kCoarseGlobal) {
+        p.off_coarse_keys = take((size_t)nq * t->nlist * sizeof(uint64_t));
     }
     p.workspace_bytes = off;
     return p;
@@ -530,7 +577,7 @@ hipError_t launch_ivfpq(const IvfPqGpuTables* t, const uint8_t* codes_lo, const 
     uint32_t* bucket = reinterpret_cast<uint32_t*>(ws + p.off_bucket);
     uint32_t* order = reinterpret_cast<uint32_t*>(ws + p.off_order);
     uint32_t* ctr = reinterpret_cast<uint32_t*>(ws + p.off_counters);
-    if (p.tiled_coarse) {
+    if (p.coarse_mode == kCoarseTiled) {
         // Tiled exact scan with fmaf accumulation: same values and (distance,
         // cluster) order as the per-query coarse kernel, at GEMM-like reuse.
         GpuSearchResult* cres = reinterpret_cast<GpuSearchResult*>(ws + p.off_coarse_res);
@@ -541,9 +588,13 @@ hipError_t launch_ivfpq(const IvfPqGpuTables* t, const uint8_t* codes_lo, const 
         const size_t tot = (size_t)nq * p.max_nprobe;
         probes_from_results_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, stream>>>(
             cres, nq, p.max_nprobe, nprobe_q, nprobe_u, t->nlist, probes);
-    } else {
+    } else if (p.coarse_mode == kCoarseLds) {
         ivf_coarse_kernel<<<nq, 256, (size_t)p.npow * 8, stream>>>(
             t->d_centroids, t->nlist, t->dim, Qm, nprobe_q, nprobe_u, p.max_nprobe, p.npow, probes);
+    } else {
+        ivf_coarse_global_kernel<<<nq, kSelThreads, 0, stream>>>(
+            t->d_centroids, t->nlist, t->dim, Qm, nprobe_q, nprobe_u, p.max_nprobe,
+            reinterpret_cast<uint64_t*>(ws + p.off_coarse_keys), probes);
     }
     fill_u32(ctr, 68, 0u, stream);
     ivf_cost_kernel<<<(nq + 255) / 256, 256, 0, stream>>>(probes, p.max_nprobe, t->d_ivf_offsets,
@@ -570,7 +621,7 @@ hipError_t launch_ivfpq(const IvfPqGpuTables* t, const uint8_t* codes_lo, const 
 static bool tables_ok(const IvfPqGpuTables* t) {
     return t->d_centroids && t->d_pq_codebooks && t->d_pq_codes && t->d_ivf_offsets &&
            t->d_ivf_vec_ids && t->dim > 0 && t->m_pq > 0 && t->sub_dim > 0 &&
-           t->m_pq * t->sub_dim == t->dim && t->nlist <= kMaxNlist;
+           t->m_pq * t->sub_dim == t->dim;
 }
 
 }  // namespace fv

@@ -5,7 +5,12 @@
 
 namespace fv {
 
-constexpr uint32_t kMaxNlist = 8192;  // coarse selection sorts all centroids in 64 KB of LDS
+// Coarse probe: the tiled exact scan for up to 1024 probes (at least 64
+// lists), else every centroid's key sorted in LDS (up to 16384 lists, 128 KB),
+// else the keys in the workspace and a radix selection there.
+constexpr uint32_t kTiledCoarseMaxProbes = 1024;
+constexpr uint32_t kLdsCoarseMaxLists = 16384;
+enum CoarseMode : uint32_t { kCoarseTiled = 0, kCoarseLds = 1, kCoarseGlobal = 2 };
 constexpr uint32_t kMaxScanGrid = 2048; // persistent scan workgroups (8 per CU on 256 CUs)
 
 struct IvfPqPlan {
@@ -16,9 +21,9 @@ struct IvfPqPlan {
     bool gbuf;            // candidate buffer in global memory (deep shortlists)
     size_t lds_scan;      // dynamic LDS of the scan kernel
     uint32_t grid;        // persistent scan workgroups
-    bool tiled_coarse;    // coarse probe through the tiled exact scan (nprobe <= 256)
+    uint32_t coarse_mode; // CoarseMode
     FlatExactPlan coarse_plan;
-    size_t off_coarse_res, off_coarse_ws;
+    size_t off_coarse_res, off_coarse_ws, off_coarse_keys;
     size_t off_probes, off_bucket, off_order, off_counters, off_gbuf;
     size_t workspace_bytes;
 };
