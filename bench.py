@@ -103,8 +103,11 @@ def main():
                          "build: IVF-PQ table construction on the GPU")
     ap.add_argument("--cpu-rows", type=int, default=50_000,
                     help="build workload: rows of the CPU oracle's timed sample")
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="use the built-in plan instead of the autotuned one")
+    ap.add_argument("--autotune", action="store_true",
+                    help="measure the scan plan on this batch instead of using the committed "
+                         "plan (flashvector_amd/tuned_plans.json)")
+    ap.add_argument("--auto-plan", action="store_true",
+                    help="use the planner's automatic plan (no committed or tuned plan)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy first (reference point for traffic counters)")
     args = ap.parse_args()
@@ -153,11 +156,15 @@ def main():
     idx = F.FlatIndex(d_db, args.dtype)
     out = L.result_tensor(nq, k, d_q.device)
     torch.cuda.synchronize()
-    tuning = None
-    if not args.no_autotune:
-        from flashvector_amd import autotune as T
+    from flashvector_amd import autotune as T
+    tuning, plan_src = None, "automatic"
+    if args.autotune:
         log("autotuning the scan plan on this batch (shortlist fixed at 8k)")
-        tuning = T.tuned(idx, d_q, k, "l2", d_m, valid_count, shortlist_factor=8)
+        tuning = T.tuned(idx, d_q, k, "l2", d_m, valid_count, refresh=True, shortlist_factor=8)
+        plan_src = "autotuned in this run"
+    elif not args.auto_plan:
+        tuning, key = T.committed(idx, k, "l2", valid_count, nq, False, 8)
+        plan_src = f"committed ({key})" if tuning is not None else "automatic"
 
     def search(m=d_m, vc=valid_count, tn=tuning):
         idx.search(d_q, k, "l2", m, out=out, valid_count=vc, shortlist_factor=8, tuning=tn)
@@ -183,8 +190,12 @@ def main():
     qps = nq / (step_ms / 1e3)
     scan_avg_ms = scan_ms / max(launches, 1)
     per_call = best_of(search, nq)
-    # Masked rows need no work: the algorithmic contraction covers valid rows only.
+    # Two flop conventions for the dominant kernel: the contraction it performs
+    # (2*Q*valid*d: the mask is compacted, masked rows are never scored) and
+    # 2*Q*N*d over every row, which credits skipped rows as done and can pass
+    # the peak; only the first is a roofline fraction.
     flops = 2.0 * nq * valid_count * dim
+    flops_all = 2.0 * nq * n * dim
     achieved = flops / (scan_avg_ms / 1e3) / 1e12
     peak = MFMA_PEAK_TFLOPS[args.dtype]
 
@@ -198,23 +209,63 @@ def main():
 
     fp32 = None
     if args.fp32_steps > 0:
-        log("plain FP32 exact scan on the same inputs")
-        ws = torch.empty(L.load().fv_flat_exact_workspace_size(n, dim, nq, k, 0),
-                         dtype=torch.uint8, device="cuda")
-        F.flat_exact_search(d_db, d_q, k, "l2", d_m, workspace=ws)
+        # The plain exact FP32 GPU scan, twice: over the same valid rows only
+        # (compacted on the host, ids mapped back: like for like with the
+        # compacted matrix-core scan) and as a dense masked scan of all rows.
+        def time_exact(rows, m):
+            ws = torch.empty(L.load().fv_flat_exact_workspace_size(rows.shape[0], dim, nq, k, 0),
+                             dtype=torch.uint8, device="cuda")
+            F.flat_exact_search(rows, d_q, k, "l2", m, workspace=ws)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(args.fp32_steps):
+                o = F.flat_exact_search(rows, d_q, k, "l2", m, workspace=ws)
+            e1.record()
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) / args.fp32_steps, o
+
+        log("plain FP32 exact scan of the valid rows, then of all rows with the mask")
+        valid_idx = np.flatnonzero(mask_bits)
+        d_valid = d_db[torch.from_numpy(valid_idx).cuda()].contiguous()
+        c_ms, c_out = time_exact(d_valid, None)
+        c_ids, _ = L.split_results(c_out)
+        c_ids = np.where(c_ids == 0xFFFFFFFF, 0xFFFFFFFF, valid_idx[np.minimum(c_ids, len(valid_idx) - 1)])
+        del d_valid
+        m_ms, m_out = time_exact(d_db, d_m)
+        m_ids, _ = L.split_results(m_out)
+        fp32 = {"valid_rows": {"ms_per_step": round(c_ms, 3), "qps": round(nq / (c_ms / 1e3), 1),
+                               "speedup_of_value": round(c_ms / step_ms, 2),
+                               "ids_agree_fraction": round(float(np.mean(c_ids == ids)), 5)},
+                "all_rows_masked": {"ms_per_step": round(m_ms, 3),
+                                    "qps": round(nq / (m_ms / 1e3), 1),
+                                    "speedup_of_value": round(m_ms / step_ms, 2),
+                                    "ids_agree_fraction": round(float(np.mean(m_ids == ids)), 5)}}
+
+    legacy = None
+    if args.dtype == "f16":
+        # The reference's entry point (cuda_fp16_tc_search_batch) on the same
+        # tables: the mask is compacted with its count taken on the device and
+        # the scratch is the stream's leased buffer.
+        log("reference entry point cuda_fp16_tc_search_batch")
+
+        def legacy_call():
+            F.fp16_tc_search_batch(idx.codes, idx.norms, d_q, k, dim, "l2", d_m, d_db)
+
+        legacy_call()
         torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(args.fp32_steps):
-            fp32_out = F.flat_exact_search(d_db, d_q, k, "l2", d_m, workspace=ws)
-        e1.record()
+        l0, l1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        l0.record()
+        for _ in range(args.steps):
+            legacy_call()
+        l1.record()
         torch.cuda.synchronize()
-        fp32_ms = e0.elapsed_time(e1) / args.fp32_steps
-        fp32_ids, _ = L.split_results(fp32_out)
-        fp32 = {"ms_per_step": round(fp32_ms, 3), "qps": round(nq / (fp32_ms / 1e3), 1),
-                "speedup_of_value": round(fp32_ms / step_ms, 2),
-                "ids_agree_fraction": round(float(np.mean(fp32_ids == ids)), 5)}
-        del ws
+        l_ms = l0.elapsed_time(l1) / args.steps
+        l_ids, _ = L.split_results(F.fp16_tc_search_batch(idx.codes, idx.norms, d_q, k, dim, "l2",
+                                                          d_m, d_db))
+        legacy = {"entry_point": "cuda_fp16_tc_search_batch", "ms_per_step": round(l_ms, 4),
+                  "qps": round(nq / (l_ms / 1e3), 1), "ratio_to_value": round(step_ms / l_ms, 4),
+                  "ids_equal_value": bool(np.array_equal(l_ids, ids))}
 
     alt = None
     if args.alt_valid_frac > 0:
@@ -224,25 +275,33 @@ def main():
         d_am = torch.from_numpy(amask).cuda()
         avc = int(abits.sum())
         atun = None
-        if not args.no_autotune:
-            atun = T.tuned(idx, d_q, k, "l2", d_am, avc, shortlist_factor=8)
+        if args.autotune:
+            atun = T.tuned(idx, d_q, k, "l2", d_am, avc, refresh=True, shortlist_factor=8)
+        elif not args.auto_plan:
+            atun, _ = T.committed(idx, k, "l2", avc, nq, False, 8)
         search(d_am, avc, atun)
         torch.cuda.synchronize()
         a0, a1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         asteps = max(1, args.steps // 2)
-        a0.record()
-        for _ in range(asteps):
-            search(d_am, avc, atun)
-        a1.record()
-        torch.cuda.synchronize()
+        with L.KernelTimer() as akt:
+            a0.record()
+            for _ in range(asteps):
+                search(d_am, avc, atun)
+            a1.record()
+            torch.cuda.synchronize()
+            a_scan_ms, a_launches = akt.read()
         ams = a0.elapsed_time(a1) / asteps
+        a_scan_avg = a_scan_ms / max(a_launches, 1)
         a_ids, a_d = L.split_results(out)
         r_ids, r_d = fp64_reference(db, qs[:rs], abits, k)
         ahits = sum(len(set(a_ids[i].tolist()) & set(r_ids[i].tolist())) for i in range(rs))
         alt = {"valid_fraction": args.alt_valid_frac, "ms_per_step": round(ams, 4),
                "qps": round(nq / (ams / 1e3), 1), "recall_at_k": round(ahits / (rs * k), 5),
                "max_abs_distance_error": float(np.max(np.abs(a_d[:rs].astype(np.float64) - r_d))),
-               "end_to_end_tflops": round(2.0 * nq * avc * dim / (ams / 1e3) / 1e12, 2)}
+               "end_to_end_tflops": round(2.0 * nq * avc * dim / (ams / 1e3) / 1e12, 2),
+               "scan_kernel_avg_ms": round(a_scan_avg, 4),
+               "scan_frac_of_peak": round(2.0 * nq * avc * dim / (a_scan_avg / 1e3) / 1e12
+                                          / MFMA_PEAK_TFLOPS[args.dtype], 4)}
 
     cpu = None
     if not args.no_cpu:
@@ -276,20 +335,29 @@ def main():
         "config": {"workload": "flat_l2_mfma_rerank", "n": n, "dim": dim, "num_queries": nq,
                    "top_k": k, "shortlist": 8 * k, "valid_fraction": args.valid_frac,
                    "storage": args.dtype, "parallelism": "single_gpu",
-                   "plan": None if tuning is None else {
-                       "autotuned": True, "n_chunks": int(tuning.n_chunks),
-                       "bound_div": int(tuning.bound_div)}},
+                   "plan": {"source": plan_src,
+                            "n_chunks": 0 if tuning is None else int(tuning.n_chunks),
+                            "bound_div": 0 if tuning is None else int(tuning.bound_div)}},
         "recall_at_k": round(recall, 5),
         "valid_rows": valid_count,
         "recall_sample_queries": rs,
         "max_abs_distance_error": max_err,
         "plain_fp32_scan": fp32,
+        "reference_entry_point": legacy,
         "second_mask_reading": alt,
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak,
                      "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
                      "traffic": measured_traffic("flat_mfma_scan_kernel/flat_l2_mfma_rerank"),
                      "kernel": "flat_mfma_scan_kernel", "kernel_avg_ms": round(scan_avg_ms, 4),
-                     "algorithmic_flop_per_launch": flops},
+                     "algorithmic_flop_per_launch": flops,
+                     "convention": "2*Q*valid*d: the contraction the kernel performs over the "
+                                   "compacted valid rows",
+                     "all_rows_2QNd": {"flop_per_launch": flops_all,
+                                       "effective_tflops": round(flops_all / (scan_avg_ms / 1e3)
+                                                                 / 1e12, 1),
+                                       "note": "credits the 95% masked rows as done; not a "
+                                               "roofline fraction"},
+                     "frac_at_95pct_valid": None if alt is None else alt["scan_frac_of_peak"]},
         "reference_best_of_5": per_call,
         "cpu_baseline": cpu,
         "wall_s_timed": round(wall, 3),
@@ -420,7 +488,7 @@ def bench_exact(args):
     """Exact FP32 flat scan through the reference's entry point
     (cuda_flat_scan_batch): N=100,000 x d=128 integer mixture rows, Q=1,000
     queries, k=10, L2, no mask. One step is one batched call over all Q
-    queries (its scratch comes from hipMallocAsync, as the reference's does).
+    queries (its scratch is the stream's leased library buffer).
     Every query's ids and distances are checked bit-exact against the CPU
     oracle, which is also the CPU baseline (all 1,000 queries)."""
     import numpy as np
@@ -462,8 +530,12 @@ def bench_exact(args):
     step_ms = start.elapsed_time(stop) / args.steps
     scan_avg = scan_ms / max(launches, 1)
     per_call = best_of(search, nq)
-    ops = 3.0 * nq * n * dim                # sub, mul, add per (query, row, dim), not fused
-    achieved = ops / (scan_avg / 1e3) / 1e12
+    # The flat-scan convention 2*Q*N*d; the kernel issues 3 unfused FP32 ops
+    # per (query, row, dim) (sub, mul, add: the CPU path's rounding), whose
+    # rate is reported beside it.
+    flops = 2.0 * nq * n * dim
+    ops = 3.0 * nq * n * dim
+    achieved = flops / (scan_avg / 1e3) / 1e12
     ids, dists = L.split_results(out)
 
     import oracle as O
@@ -490,7 +562,10 @@ def bench_exact(args):
                      "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                      "traffic": measured_traffic("flat_exact_scan_kernel"),
                      "kernel": "flat_exact_scan_kernel", "kernel_avg_ms": round(scan_avg, 4),
-                     "launches": launches, "algorithmic_ops_per_launch": ops},
+                     "launches": launches, "algorithmic_flop_per_launch": flops,
+                     "convention": "2*Q*N*d",
+                     "issued_unfused_ops": {"per_launch": ops,
+                                            "tops": round(ops / (scan_avg / 1e3) / 1e12, 2)}},
         "reference_best_of_5": per_call,
         "cpu_baseline": {"value": round(nq / cpu_s, 1), "unit": "queries/s", "cores": cores,
                          "kind": "port",
@@ -531,7 +606,7 @@ def bench_build(args):
         assign_ms, launches = kt.read()
     sizes = np.diff(t.offsets.cpu().numpy().astype(np.int64))
     assign_avg = assign_ms / max(launches, 1)
-    flops = 3.0 * n * nlist * dim          # sub, mul, add per (row, centroid, dim)
+    flops = 2.0 * n * nlist * dim          # 2*N*nlist*d (the kernel issues 3 unfused ops each)
     achieved = flops / (assign_avg / 1e3) / 1e12
     cpu = None
     if not args.no_cpu:
@@ -560,7 +635,9 @@ def bench_build(args):
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
                      "kernel": "km_assign_kernel (coarse)", "kernel_avg_ms": round(assign_avg, 4),
-                     "launches": launches, "algorithmic_flops_per_launch": flops},
+                     "launches": launches, "algorithmic_flops_per_launch": flops,
+                     "convention": "2*N*nlist*d",
+                     "issued_unfused_tops": round(1.5 * flops / (assign_avg / 1e3) / 1e12, 2)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)

@@ -34,6 +34,8 @@
 //    sweep the same chunk together and share it through that XCD's L2.
 #pragma once
 
+#include <type_traits>
+
 #include "fv_flat_mfma.hpp"
 
 namespace fv {
@@ -105,6 +107,18 @@ struct ScanShape {
                                        : (2 * 32 * (kRowBytes + 16) <= 51200 ? 32 : 16);
 };
 
+// Rows of whole 256-byte multiples can be staged with direct-to-LDS loads
+// into an unpadded, XOR-swizzled tile (see flat_mfma_scan_kernel).
+template <int DT, int KS>
+constexpr bool scan_dma_ok() { return ScanShape<DT, KS>::kRowBytes % 256 == 0; }
+
+// One 16-byte direct-to-LDS load per lane: lane l's 16 bytes from `src` land
+// at lds_base + 16 * l (lds_base uniform across the wavefront).
+__device__ __forceinline__ void lds_dma16(const void* src, void* lds_base) {
+    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)lds_base, 16,
+                                     0, 0);
+}
+
 // Host-side mirror of ScanShape (queries per wave and rows per tile).
 inline int scan_qw(int dt, uint32_t ks) {
     const int lane = dt == FV_DTYPE_FP8_E4M3 ? 32 : 16;
@@ -158,7 +172,7 @@ __device__ __forceinline__ void query_k_s(const FlatSearchArgs& a, uint32_t q, u
 // keeps a running minimum per (lane, element) — 16 disjoint row subsets per
 // query per chunk — written to `minima`; rows are limited to the first
 // `bound_rows` positions. No thresholds, buffers or prunes.
-template <int DT, int METRIC, int KS, int R, bool BOUND>
+template <int DT, int METRIC, int KS, int R, bool BOUND, bool DMA>
 __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_mfma_scan_kernel(
     const uint8_t* __restrict__ X, uint32_t row_bytes, const float* __restrict__ norms,
     const float* __restrict__ row_scales, const uint8_t* __restrict__ Qh,
@@ -180,13 +194,27 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     constexpr int RT = (R == 0 || BOUND) ? Sh::kRTDeep : Sh::kRT;
     constexpr int LB = Sh::kLaneBytes;
     constexpr int kRowBytes = Sh::kRowBytes;
-    constexpr int kLdsRow = kRowBytes + 16;      // padded LDS row
+    // Register staging: rows padded by 16 bytes so fragment reads are
+    // conflict-free. Direct-to-LDS staging (DMA): rows unpadded, 16-byte chunk
+    // c of row r stored at chunk c ^ (r & 15) of its row (conflict-free for
+    // the same reads; rows are whole multiples of 16 chunks).
+    constexpr int kLdsRow = DMA ? kRowBytes : kRowBytes + 16;
     constexpr int kThrPerRow = kMfmaThreads / RT;
     constexpr int kSegBytes = kRowBytes / kThrPerRow;  // bytes one thread stages per row
-    constexpr int NPRE = kSegBytes / 16;
+    constexpr int NPRE = DMA ? 1 : kSegBytes / 16;
     static_assert(kSegBytes % 16 == 0, "staging segments are whole 16-byte loads");
+    static_assert(!DMA || kRowBytes % 256 == 0, "direct-to-LDS tiles need 256-byte rows");
+    // Direct-to-LDS loads per wave per tile (1 KB each, tile bytes split over
+    // the 4 waves).
+    constexpr int kDmaPerWave = DMA ? RT * kRowBytes / 4096 : 1;
+    static_assert(!DMA || (RT * kRowBytes) % 4096 == 0, "tiles split into 1 KB loads per wave");
 
-    __shared__ __attribute__((aligned(16))) uint8_t s_tile[2][RT * kLdsRow];
+    // Two tile buffers as distinct arrays, addressed with compile-time indices
+    // (the tile loop is unrolled by two): the compiler can then tell that the
+    // direct-to-LDS loads into one buffer do not alias the reads of the other
+    // and waits for them only at the barrier.
+    __shared__ __attribute__((aligned(16))) uint8_t s_tile0[RT * kLdsRow];
+    __shared__ __attribute__((aligned(16))) uint8_t s_tile1[RT * kLdsRow];
     __shared__ __attribute__((aligned(16))) float s_cinit[2][RT];
     __shared__ __attribute__((aligned(16))) float s_scale[2][RT];
     __shared__ uint32_t s_rowid[2][RT];
@@ -195,6 +223,12 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     __shared__ float s_thr[QB];
     __shared__ uint32_t s_need[QB / 32];
     extern __shared__ uint64_t s_scratch[];
+    auto tile_ptr = [&](auto bc) -> uint8_t* {
+        if constexpr (decltype(bc)::value == 0) return s_tile0;
+        else return s_tile1;
+    };
+    using B0 = std::integral_constant<int, 0>;
+    using B1 = std::integral_constant<int, 1>;
 
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
@@ -292,29 +326,60 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     // past the chunk end get C = +inf, so their data never matters): no
     // exec-masked loads, so no conservative waits around them.
     uint32_t rid_next = 0;
+    // Direct-to-LDS: load i of this lane covers LDS bytes
+    // (wave * kDmaPerWave + i) * 1024 + 16 * lane of the tile, i.e. row
+    // dma_row[i], global byte offset dma_off[i] within that row (the swizzle
+    // applied in reverse); its row id is fetched a tile ahead like rid_next.
+    uint32_t dma_row[kDmaPerWave], dma_off[kDmaPerWave], dma_rid[kDmaPerWave];
+#pragma unroll
+    for (int i = 0; i < kDmaPerWave; ++i) {
+        const uint32_t o = (uint32_t)((wave * kDmaPerWave + i) * 1024 + 16 * lane);
+        dma_row[i] = o / (uint32_t)kRowBytes;
+        const uint32_t unit = (o % (uint32_t)kRowBytes) / 16u;
+        dma_off[i] = (unit ^ (dma_row[i] & 15u)) * 16u;
+        dma_rid[i] = 0;
+    }
     auto fetch_rid = [&](uint32_t p0) {
         const uint32_t p = p0 + st_row;
         rid_next = rows ? rows[p < r_end ? p : r_end - 1] : p;
+        if constexpr (DMA) {
+#pragma unroll
+            for (int i = 0; i < kDmaPerWave; ++i) {
+                const uint32_t q = p0 + dma_row[i];
+                dma_rid[i] = rows ? rows[q < r_end ? q : r_end - 1] : (q < r_end ? q : r_end - 1);
+            }
+        }
     };
-    auto load_tile = [&](uint32_t p0) {
+    auto load_tile = [&](uint32_t p0, auto dbc) {
         const uint32_t p = p0 + st_row;
         pre_in = p < r_end;
         pre_row = pre_in ? rid_next : (rows ? rows[r_begin] : r_begin);
-        const uint8_t* src = X + (size_t)pre_row * row_bytes;
+        if constexpr (DMA) {
+            // Rows past the chunk end load a valid row (their C is +inf).
 #pragma unroll
-        for (int i = 0; i < NPRE; ++i) {
-            const uint32_t e = (uint32_t)(st_seg * kSegBytes + 16 * i);
-            if (e + 16 <= row_bytes) pre[i] = *reinterpret_cast<const uint4*>(src + e);
-            else pre[i] = make_uint4(0, 0, 0, 0);
+            for (int i = 0; i < kDmaPerWave; ++i)
+                lds_dma16(X + (size_t)dma_rid[i] * row_bytes + dma_off[i],
+                          tile_ptr(dbc) + (wave * kDmaPerWave + i) * 1024);
+        } else {
+            const uint8_t* src = X + (size_t)pre_row * row_bytes;
+#pragma unroll
+            for (int i = 0; i < NPRE; ++i) {
+                const uint32_t e = (uint32_t)(st_seg * kSegBytes + 16 * i);
+                if (e + 16 <= row_bytes) pre[i] = *reinterpret_cast<const uint4*>(src + e);
+                else pre[i] = make_uint4(0, 0, 0, 0);
+            }
         }
         if (METRIC != METRIC_INNER_PRODUCT) pre_norm = norms[pre_row];
         if (kFP8) pre_rs = row_scales[pre_row];
         pre_mword = (a.mask && !rows) ? a.mask[pre_row >> 5] : 0xFFFFFFFFu;
     };
-    auto store_tile = [&](int buf) {
-        uint8_t* dst = &s_tile[buf][st_row * kLdsRow + st_seg * kSegBytes];
+    auto store_tile = [&](auto bc) {
+        constexpr int buf = decltype(bc)::value;
+        if constexpr (!DMA) {
+            uint8_t* dst = tile_ptr(bc) + st_row * kLdsRow + st_seg * kSegBytes;
 #pragma unroll
-        for (int i = 0; i < NPRE; ++i) *reinterpret_cast<uint4*>(dst + 16 * i) = pre[i];
+            for (int i = 0; i < NPRE; ++i) *reinterpret_cast<uint4*>(dst + 16 * i) = pre[i];
+        }
         if (st_seg == 0) {
             const bool ok = pre_in && ((pre_mword >> (pre_row & 31u)) & 1u);
             float c = __builtin_inff(), sc = 1.0f;
@@ -352,12 +417,21 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
             const float th = s_thr[ql];
             const floatx4 sc = rec_score[e];
             uint64_t* bq = cand + ((size_t)chunk * nq + qb0 + ql) * cap;
+            // One LDS atomic per record (its passing scores together), then
+            // the key stores: no store is followed by an LDS atomic of the
+            // same record, so none has to complete before the next atomic.
+            bool ok[4];
+            uint32_t npass = 0;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                if (sc[i] < th) {
-                    const uint32_t pos = atomicAdd(&s_cnt[ql], 1u);
-                    bq[pos] = make_key(sc[i], s_rowid[buf][rbase + i]);
-                }
+                ok[i] = sc[i] < th;
+                npass += ok[i] ? 1u : 0u;
+            }
+            if (npass) {
+                uint32_t pos = atomicAdd(&s_cnt[ql], npass);
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (ok[i]) bq[pos++] = make_key(sc[i], s_rowid[buf][rbase + i]);
             }
         }
         nrec = 0;
@@ -366,9 +440,9 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
 
     if (ntiles > 0) {
         fetch_rid(r_begin);
-        load_tile(r_begin);
+        load_tile(r_begin, B0{});
         if (ntiles > 1) fetch_rid(r_begin + RT);
-        store_tile(0);
+        store_tile(B0{});
     }
     __syncthreads();
 
@@ -376,11 +450,14 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
 #pragma unroll
     for (int j = 0; j < NJ; ++j) runmin[j] = floatx4{kAdmit, kAdmit, kAdmit, kAdmit};
 
-    for (uint32_t t = 0; t < ntiles; ++t) {
-        const int buf = t & 1;
+    // One tile: compute on buffer `buf` while the next tile is staged into
+    // the other one.
+    auto step = [&](uint32_t t, auto bc) {
+        constexpr int buf = decltype(bc)::value;
+        using Next = std::integral_constant<int, buf ^ 1>;
         const uint32_t p0 = r_begin + t * RT;
         if (t + 1 < ntiles) {
-            load_tile(p0 + RT);
+            load_tile(p0 + RT, Next{});
             if (t + 2 < ntiles) fetch_rid(p0 + 2 * RT);
         }
         // Prefetch the shared bound; it is consumed after this tile's MMAs.
@@ -396,10 +473,31 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
         // the selection of tile rt, so that VALU work overlaps the matrix
         // pipe.
         auto mma_tile = [&](int rt, floatx4* acc) {
-            const uint8_t* arow = &s_tile[buf][(rt * 16 + l16) * kLdsRow + grp * (LB * KS)];
             vec afrag[KS];
+            if constexpr (DMA) {
+                // Row rt*16 + l16 (so its swizzle is l16); the lane's K bytes
+                // LB*KS*grp + LB*s are chunks (LB/16)*(KS*grp + s) (+1).
+                const uint8_t* rb = tile_ptr(bc) + (rt * 16 + l16) * kRowBytes;
+                constexpr int CPL = LB / 16;
 #pragma unroll
-            for (int s = 0; s < KS; ++s) afrag[s] = ld_vec<vec>(arow + LB * s);
+                for (int s = 0; s < KS; ++s) {
+                    const uint32_t u = (uint32_t)(CPL * (KS * grp + s));
+                    if constexpr (CPL == 1) {
+                        afrag[s] = ld_vec<vec>(rb + ((u ^ (uint32_t)l16) * 16u));
+                    } else {
+                        struct Two {
+                            uint4 a, b;
+                        } t2;
+                        t2.a = *reinterpret_cast<const uint4*>(rb + ((u ^ (uint32_t)l16) * 16u));
+                        t2.b = *reinterpret_cast<const uint4*>(rb + (((u + 1u) ^ (uint32_t)l16) * 16u));
+                        afrag[s] = __builtin_bit_cast(vec, t2);
+                    }
+                }
+            } else {
+                const uint8_t* arow = tile_ptr(bc) + (rt * 16 + l16) * kLdsRow + grp * (LB * KS);
+#pragma unroll
+                for (int s = 0; s < KS; ++s) afrag[s] = ld_vec<vec>(arow + LB * s);
+            }
             const floatx4 cin = *reinterpret_cast<const floatx4*>(&s_cinit[buf][rt * 16 + 4 * grp]);
 #pragma unroll
             for (int j = 0; j < NJ; ++j) acc[j] = kFP8 ? floatx4{0.0f, 0.0f, 0.0f, 0.0f} : cin;
@@ -455,16 +553,16 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
             select_tile(rt, cur);
         }
         if (!BOUND) flush_records(buf);
-        if (t + 1 < ntiles) store_tile(buf ^ 1);
+        if (t + 1 < ntiles) store_tile(Next{});
         __syncthreads();
-        if (BOUND) continue;
+        if (BOUND) return;
 
         // A buffer gains at most one key per row, so buffers are checked
         // every pk tiles and pruned when pk more tiles could overflow them.
         if ((t + 1) % pk != 0) {
             if (my_live) s_thr[my_ql] = fminf(s_thr[my_ql], bound_from_ord(gpre));
             wave_lds_sync();
-            continue;
+            return;
         }
         if (!kBlockPrune) {
             // Wave-private prunes of buffers that could overflow on the next tile.
@@ -530,6 +628,10 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
         // reads its own queries' thresholds, so a wave-level sync suffices.
         if (my_live) s_thr[my_ql] = fminf(s_thr[my_ql], bound_from_ord(gpre));
         wave_lds_sync();
+    };
+    for (uint32_t t = 0; t < ntiles; t += 2) {
+        step(t, B0{});
+        if (t + 1 < ntiles) step(t + 1, B1{});
     }
     if (BOUND) {
 #pragma unroll
@@ -548,35 +650,35 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
 
 // Static LDS of one main-pass instantiation (queried once); with the plan's
 // dynamic LDS it must fit the 160 KB of a CU.
-template <int DT, int METRIC, int KS, int R>
+template <int DT, int METRIC, int KS, int R, bool DMA>
 static size_t scan_static_lds() {
     static const size_t v = [] {
         hipFuncAttributes attr{};
-        const void* f = reinterpret_cast<const void*>(flat_mfma_scan_kernel<DT, METRIC, KS, R, false>);
+        const void* f = reinterpret_cast<const void*>(flat_mfma_scan_kernel<DT, METRIC, KS, R, false, DMA>);
         return hipFuncGetAttributes(&attr, f) == hipSuccess ? attr.sharedSizeBytes : (size_t)1 << 30;
     }();
     return v;
 }
 
-template <int DT, int METRIC, int KS, int R>
+template <int DT, int METRIC, int KS, int R, bool DMA>
 static hipError_t launch_scan_r(const ScanLaunch& L) {
     const FlatMfmaPlan& p = *L.p;
     const dim3 g(p.qblocks * p.n_chunks);
-    flat_mfma_scan_kernel<DT, METRIC, KS, R, false><<<g, kMfmaThreads, p.lds_scan, L.stream>>>(
+    flat_mfma_scan_kernel<DT, METRIC, KS, R, false, DMA><<<g, kMfmaThreads, p.lds_scan, L.stream>>>(
         L.X, L.row_bytes, L.norms, L.row_scales, L.Qh, L.qscale, L.nq, L.a, p.n_chunks, p.qblocks,
         p.cap, L.rows, L.nrows, L.n, L.gthr, L.counts, L.cand, nullptr, 0u);
     return hipSuccess;
 }
 
-template <int DT, int METRIC, int KS>
-static hipError_t launch_scan(const ScanLaunch& L) {
+template <int DT, int METRIC, int KS, bool DMA>
+static hipError_t launch_scan_d(const ScanLaunch& L) {
     const FlatMfmaPlan& p = *L.p;
-    const size_t st = p.prune_regs == 16 ? scan_static_lds<DT, METRIC, KS, 16>()
-                                         : scan_static_lds<DT, METRIC, KS, 0>();
+    const size_t st = p.prune_regs == 16 ? scan_static_lds<DT, METRIC, KS, 16, DMA>()
+                                         : scan_static_lds<DT, METRIC, KS, 0, DMA>();
     if (st + p.lds_scan > 160u * 1024u) return hipErrorInvalidValue;
     if (p.bound_chunks) {
         const dim3 g(p.qblocks * p.bound_chunks);
-        flat_mfma_scan_kernel<DT, METRIC, KS, 16, true><<<g, kMfmaThreads, 0, L.stream>>>(
+        flat_mfma_scan_kernel<DT, METRIC, KS, 16, true, DMA><<<g, kMfmaThreads, 0, L.stream>>>(
             L.X, L.row_bytes, L.norms, L.row_scales, L.Qh, L.qscale, L.nq, L.a, p.bound_chunks,
             p.qblocks, p.cap, L.rows, L.nrows, L.n, L.gthr, L.counts, L.cand, L.minima,
             p.bound_rows);
@@ -584,10 +686,21 @@ static hipError_t launch_scan(const ScanLaunch& L) {
     }
     // The kernel timer brackets the main pass only.
     void* timer = timer_begin(L.stream);
-    const hipError_t e = p.prune_regs == 16 ? launch_scan_r<DT, METRIC, KS, 16>(L)
-                                            : launch_scan_r<DT, METRIC, KS, 0>(L);
+    const hipError_t e = p.prune_regs == 16 ? launch_scan_r<DT, METRIC, KS, 16, DMA>(L)
+                                            : launch_scan_r<DT, METRIC, KS, 0, DMA>(L);
     timer_end(timer, L.stream);
     return e;
+}
+
+// Direct-to-LDS staging when the rows are whole 256-byte multiples with no
+// K padding beyond the table's own (row stride == staged row bytes).
+template <int DT, int METRIC, int KS>
+static hipError_t launch_scan(const ScanLaunch& L) {
+    if constexpr (scan_dma_ok<DT, KS>()) {
+        if (L.row_bytes == (uint32_t)ScanShape<DT, KS>::kRowBytes)
+            return launch_scan_d<DT, METRIC, KS, true>(L);
+    }
+    return launch_scan_d<DT, METRIC, KS, false>(L);
 }
 
 template <int DT, int KS>

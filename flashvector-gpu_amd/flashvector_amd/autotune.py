@@ -47,6 +47,22 @@ def _store(table: dict) -> None:
     os.replace(tmp, path)
 
 
+# Plans measured on MI355X for the benchmark shapes and committed with the
+# package, so that a benchmark run times a fixed, reproducible plan.
+COMMITTED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_plans.json")
+
+
+def committed(index, top_k, metric, rows, nq, f64_boundary=False, shortlist_factor=0):
+    """(the committed FvFlatTuning for this shape or None, its key)."""
+    key = tuning_key(index, top_k, metric, rows, nq, f64_boundary, shortlist_factor)
+    try:
+        with open(COMMITTED) as f:
+            entry = json.load(f).get(key)
+    except (OSError, ValueError):
+        return None, key
+    return (to_tuning(entry) if entry else None), key
+
+
 def to_tuning(entry: dict):
     t = L.FvFlatTuning()
     t.shortlist_factor = entry["shortlist_factor"]
