@@ -108,6 +108,8 @@ def main():
                          "plan (flashvector_amd/tuned_plans.json)")
     ap.add_argument("--auto-plan", action="store_true",
                     help="use the planner's automatic plan (no committed or tuned plan)")
+    ap.add_argument("--ivf-plan", default="committed",
+                    help="IVF-PQ work-item plan: committed | default | tune | TARGET_ITEMS:CHUNK_MIN")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy first (reference point for traffic counters)")
     args = ap.parse_args()
@@ -391,8 +393,29 @@ def bench_ivfpq(args):
     nq, S = w.queries.shape[0], 1000
     out = L.result_tensor(nq, 100, w.queries.device)
 
+    # Work-item plan: the committed measured plan, the library default, an
+    # explicit "target_items:chunk_min", or one autotuned in this run.
+    from flashvector_amd import autotune as AT
+    plan = {"source": args.ivf_plan}
+    tuning = None
+    if args.ivf_plan == "tune":
+        tuning = idx.autotune(w.queries, 100, 64, S, top_k=w.top_k, nprobe=w.nprobe, trials=3)
+        plan["source"] = "fv_ivf_pq_autotune in this run"
+    elif args.ivf_plan == "committed":
+        entry = AT.committed_entry("ivfpq_uneven_10m_q8192_s1000")
+        if entry is not None:
+            tuning = L.FvIvfPqTuning(entry["target_items"], entry["chunk_min"], 0.0, 0)
+            plan["source"] = "committed (ivfpq_uneven_10m_q8192_s1000)"
+        else:
+            plan["source"] = "library default (no committed plan)"
+    elif args.ivf_plan != "default":
+        a_, b_ = (int(v) for v in args.ivf_plan.split(":"))
+        tuning = L.FvIvfPqTuning(a_, b_, 0.0, 0)
+    if tuning is not None:
+        plan.update(target_items=int(tuning.target_items), chunk_min=int(tuning.chunk_min))
+
     def step():
-        idx.search(w.queries, 100, 64, S, top_k=w.top_k, nprobe=w.nprobe, out=out)
+        idx.search(w.queries, 100, 64, S, top_k=w.top_k, nprobe=w.nprobe, out=out, tuning=tuning)
 
     for _ in range(max(1, args.warmup)):
         step()
@@ -469,7 +492,8 @@ def bench_ivfpq(args):
         "data": "synthetic (seeded Gaussian mixture, Zipf list sizes)",
         "config": {"workload": "ivfpq_uneven_10m", "n": int(w.vectors.shape[0]), "dim": 128,
                    "nlist": 4096, "m_pq": 16, "num_queries": nq, "shortlist": S,
-                   "nprobe": "per query in {8,16,32,64}", "top_k": "per query in {10,100}"},
+                   "nprobe": "per query in {8,16,32,64}", "top_k": "per query in {10,100}",
+                   "plan": plan},
         "recall_at_k_vs_exact": round(hits / tot, 5), "recall_sample_queries": rs,
         "ids_bit_exact_vs_cpu_adc": ids_match,
         "hip_graph": True, "graph_replay_matches_eager": graph_matches_eager,

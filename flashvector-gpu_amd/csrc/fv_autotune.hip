@@ -12,6 +12,7 @@
 
 #include "../../include/flashvector/fv_ext.h"
 #include "fv_flat_mfma.hpp"
+#include "fv_ivfpq.hpp"
 
 namespace {
 
@@ -169,6 +170,81 @@ extern "C" int fv_flat_autotune(const FvFlatTables* t, const float* d_queries, u
     if (rc) return rc;
     best.shortlist_factor = factor;
     best.recall = recall;
+    *out_best = best;
+    return 0;
+}
+
+// Autotuner of the IVF-PQ work-item plan (fv_ivf_pq_autotune). Every plan
+// returns the same results, so only time decides.
+extern "C" int fv_ivf_pq_autotune(const IvfPqGpuTables* t, const uint8_t* d_codes_list_order,
+                                  const float* d_queries, uint32_t nq, const uint32_t* d_top_k,
+                                  uint32_t max_top_k, const uint32_t* d_nprobe,
+                                  uint32_t max_nprobe, uint32_t shortlist, int rerank,
+                                  MetricType metric, const uint32_t* d_valid_mask,
+                                  uint32_t trials, FvIvfPqTuning* out_best, void* stream) {
+    if (!t || !d_queries || !out_best) return -1;
+    if (nq == 0) return (int)hipErrorInvalidValue;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t need = fv_ivf_pq_workspace_size(t, nq, max_nprobe, shortlist, max_top_k);
+    if (need == 0) return (int)hipErrorInvalidValue;
+    DevBuf ws, res;
+    hipError_t e = ws.ensure(need);
+    if (e == hipSuccess) e = res.ensure((size_t)nq * max_top_k * sizeof(GpuSearchResult));
+    if (e != hipSuccess) return (int)e;
+    hipEvent_t e0, e1;
+    if ((e = hipEventCreate(&e0)) != hipSuccess) return (int)e;
+    if ((e = hipEventCreate(&e1)) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return (int)e;
+    }
+    auto run = [&](const FvIvfPqTuning* tu) {
+        return fv_ivf_pq_search_tuned(t, d_codes_list_order, d_queries, nq, d_top_k, max_top_k,
+                                      d_nprobe, max_nprobe, shortlist, rerank, metric,
+                                      d_valid_mask, static_cast<GpuSearchResult*>(res.p), ws.p,
+                                      ws.bytes, tu, stream);
+    };
+    auto time_plan = [&](const FvIvfPqTuning* tu, float* out_ms) -> int {
+        int r = run(tu);  // warm-up
+        std::vector<float> ts;
+        for (uint32_t i = 0; i < std::max(1u, trials) && r == 0; ++i) {
+            (void)hipEventRecord(e0, s);
+            r = run(tu);
+            (void)hipEventRecord(e1, s);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            ts.push_back(ms);
+        }
+        if (r == 0) {
+            std::sort(ts.begin(), ts.end());
+            *out_ms = ts[ts.size() / 2];
+        }
+        return r;
+    };
+    float auto_ms = 0.0f;
+    int rc = time_plan(nullptr, &auto_ms);
+    FvIvfPqTuning best{};
+    best.target_items = fv::kDefaultTargetItems;
+    best.chunk_min = fv::kDefaultChunkMin;
+    best.ms = auto_ms;
+    for (uint32_t items : {0u, 256u, 1024u, 4096u, 16384u}) {
+        for (uint32_t cmin : {4096u, 16384u, 65536u}) {
+            if (rc) break;
+            if (items == 0u && cmin != 16384u) continue;  // chunk_min is unused without splitting
+            FvIvfPqTuning c{};
+            c.target_items = items;
+            c.chunk_min = cmin;
+            float ms = 0.0f;
+            rc = time_plan(&c, &ms);
+            if (rc == 0 && ms < best.ms && ms < 0.97f * auto_ms) {
+                best = c;
+                best.ms = ms;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
     *out_best = best;
     return 0;
 }

@@ -17,7 +17,10 @@
 //      streamed 256 entries per pass. With a list-ordered code copy every
 //      lane reads its entry's 16 code bytes with one coalesced 128-bit load;
 //      with the reference's id-indexed codes it gathers them by vector id.
-//      ADC = sequential FP32 adds over m = 0..M-1. An entry is kept if its
+//      ADC = sequential FP32 adds over m = 0..M-1, abandoned after any
+//      fourth term once the sum is past the threshold's distance (LUT
+//      entries are >= 0, so such an entry could not be kept; kept entries
+//      add every term). An entry is kept if its
 //      (adc, id) key is below the running threshold key; keys go to a
 //      candidate buffer (LDS, or global memory for very deep shortlists)
 //      that is cut to the S best by radix selection whenever the next keys
@@ -182,14 +185,67 @@ __device__ __forceinline__ void build_lut(float* lut, const float* __restrict__ 
     }
 }
 
-// Persistent: each workgroup repeatedly takes the next query from `order`
-// (heaviest first) through a global atomic counter, so queries of very uneven
-// cost (Zipf lists, per-query nprobe) are balanced dynamically.
+// Arguments of the scan kernel.
+struct IvfScanArgs {
+    IvfPqGpuTables t;
+    const uint8_t* codes_lo;      // list-ordered codes, or nullptr (gather by vector id)
+    const float* Qm;
+    const uint32_t* probes;       // [nq][max_nprobe], packed, FV_EMPTY_ID after the last
+    uint32_t max_nprobe;
+    const uint32_t* k_q;
+    uint32_t k_u;
+    uint32_t shortlist;           // S of the plan (>= every query's S); partial slot size
+    uint32_t cap;
+    int do_rerank;
+    const uint32_t* mask;
+    uint64_t* gbuf;
+    GpuSearchResult* out;
+    uint32_t out_stride;
+    // Work items: query order[i] owns items item_start[i] .. item_start[i+1]-1,
+    // each a contiguous run of its probed entries (lists in probe order).
+    uint32_t nq;
+    const uint32_t* order;        // queries, heaviest first
+    const uint32_t* item_start;   // [nq + 1]
+    const uint32_t* cost;         // [nq] probed entries per query
+    const uint32_t* slot_start;   // [nq] first partial slot of a split query
+    uint32_t* next_item;
+    uint32_t* done;               // [nq] finished items of a split query
+    uint64_t* qbound;             // [nq] shared admission bound of a split query
+    uint64_t* part_keys;          // [slots][shortlist] item shortlists of split queries
+    uint32_t* part_cnt;           // [slots]
+};
+
+// Buffer resource over `bytes` bytes at p (raw, 32-bit elements): loads past
+// the end return zero.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
+                                             0x00020000);
+}
+
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Persistent: each workgroup repeatedly takes the next work item through a
+// global atomic counter. A query whose probed lists hold more entries than
+// the batch's chunk length is split into items of at most that many entries
+// (its lists concatenated in probe order), so one query over a huge (Zipf
+// head) list is scanned by several workgroups at once; queries are ordered
+// heaviest first. Each item of a split query keeps its S best keys in a
+// partial slot and publishes its S-th best to the query's shared bound (any
+// item's S-th best bounds the query's S-th best, so the other items admit
+// only keys at or below it); the item that finishes last merges the slots
+// and finishes the query. The result depends only on the set of probed
+// entries, so it is the same however the query is split.
 //
 // M16 (16 sub-quantizers): the LUT is a static LDS array, the kernel's only
 // static LDS object and so at LDS address 0, which lets each lookup's byte
 // extract and scale fold into one SDWA shift (the address is code*4 plus an
-// immediate). Other M keep the LUT at the start of dynamic LDS.
+// immediate). Other M keep the LUT at the start of dynamic LDS. With
+// list-ordered codes the ids and codes of the next check are loaded (buffer
+// loads, zero past the run's end) while the current one is scored.
 //
 // Candidates: each check of kEntriesPerCheck entries computes its keys into
 // registers and every wave publishes how many are below the threshold. Only
@@ -198,101 +254,216 @@ __device__ __forceinline__ void build_lut(float* lut, const float* __restrict__ 
 // then appended at positions from the per-wave counts, so the buffer fill and
 // the threshold are uniform registers, identical in every thread.
 template <bool LISTORDER, bool GBUF, int METRIC, bool M16>
-__global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(
-    IvfPqGpuTables t, const uint8_t* __restrict__ codes_lo, const float* __restrict__ Qm,
-    uint32_t nq, const uint32_t* __restrict__ order, uint32_t* __restrict__ next_q,
-    const uint32_t* __restrict__ probes, uint32_t max_nprobe, const uint32_t* __restrict__ k_q,
-    uint32_t k_u, uint32_t shortlist_u, uint32_t cap, int do_rerank,
-    const uint32_t* __restrict__ mask, uint64_t* __restrict__ gbuf,
-    GpuSearchResult* __restrict__ out, uint32_t out_stride) {
+__global__ __launch_bounds__(kIvfThreads, 4) void ivf_scan_kernel(IvfScanArgs a) {
     // Dynamic LDS: LUT [M][256] f32 (unless static), candidate buffer (unless
     // in global memory), control words, the probed list's residual [dim].
     __shared__ float s_lut16[M16 ? 16 * 256 : 1];
     extern __shared__ uint8_t s_dyn[];
+    const IvfPqGpuTables& t = a.t;
+    const uint32_t cap = a.cap;
     float* lut = M16 ? s_lut16 : reinterpret_cast<float*>(s_dyn);
     const size_t lut_bytes = M16 ? 0 : (size_t)t.m_pq * 256 * sizeof(float);
-    uint64_t* buf = GBUF ? gbuf + (size_t)blockIdx.x * cap : reinterpret_cast<uint64_t*>(s_dyn + lut_bytes);
+    uint64_t* buf = GBUF ? a.gbuf + (size_t)blockIdx.x * cap : reinterpret_cast<uint64_t*>(s_dyn + lut_bytes);
     uint8_t* ctl_base = s_dyn + lut_bytes + (GBUF ? (size_t)0 : (size_t)cap * 8);
     IvfCtl& ctl = *reinterpret_cast<IvfCtl*>(ctl_base);
     float* res = reinterpret_cast<float*>(ctl_base + kIvfCtlBytes);
     constexpr uint32_t kW = kIvfThreads / kWave;
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const uint32_t n_items = a.item_start[a.nq];
 
   for (;;) {
     __syncthreads();
-    if (threadIdx.x == 0) ctl.item = atomicAdd(next_q, 1u);
+    if (threadIdx.x == 0) ctl.item = atomicAdd(a.next_item, 1u);
     __syncthreads();
-    if (ctl.item >= nq) return;
-    const uint32_t q = order[ctl.item];
+    const uint32_t item = __builtin_amdgcn_readfirstlane(ctl.item);
+    if (item >= n_items) return;
+    // The item's query: the last position i with item_start[i] <= item.
+    uint32_t lo = 0, hi = a.nq;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.item_start[mid] <= item) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t q = a.order[lo];
+    const uint32_t nit = a.item_start[lo + 1] - a.item_start[lo], part = item - a.item_start[lo];
+    const uint32_t T = a.cost[q];
+    const uint64_t clen = ((uint64_t)T + nit - 1u) / nit;
+    const uint32_t e0 = (uint32_t)min((uint64_t)T, part * clen);
+    const uint32_t e1 = (uint32_t)min((uint64_t)T, e0 + clen);
+    const bool split = nit > 1u;
+
     const uint32_t M = t.m_pq, sub = t.sub_dim, dim = t.dim;
-    const float* qv = Qm + (size_t)q * dim;
-    uint32_t k = k_q ? k_q[q] : k_u;
-    if (k > out_stride) k = out_stride;
-    uint32_t S = shortlist_u < k ? k : shortlist_u;
+    const float* qv = a.Qm + (size_t)q * dim;
+    uint32_t k = a.k_q ? a.k_q[q] : a.k_u;
+    if (k > a.out_stride) k = a.out_stride;
+    uint32_t S = a.shortlist < k ? k : a.shortlist;
     if (S > cap - kEntriesPerCheck) S = cap - kEntriesPerCheck;
     if (S == 0) S = 1;
-    uint32_t cnt = 0;         // keys in the candidate buffer
-    uint64_t thr = kEmptyKey; // admission threshold: the S-th best key so far
-    uint32_t ph = 0;          // which wn[] row the current check uses
+    uint32_t cnt = 0;          // keys in the candidate buffer
+    uint64_t thr = kEmptyKey;  // admission threshold: the S-th best key so far
+    uint32_t ph = 0;           // which wn[] row the current check uses
+    bool publish = split;      // publish cuts to the query's shared bound
 
-    for (uint32_t p = 0; p < max_nprobe; ++p) {
-        const uint32_t c = probes[(size_t)q * max_nprobe + p];
-        if (c >= t.nlist) break;  // probes are packed; the rest are empty
-        const float* cen = t.d_centroids + (size_t)c * dim;
-        // The previous list's LUT build is done with res (barrier after it).
-        for (uint32_t d = threadIdx.x; d < dim; d += kIvfThreads) res[d] = qv[d] - cen[d];
-        __syncthreads();  // res visible; the previous list's lookups are done with the LUT
-        build_lut(lut, t.d_pq_codebooks, res, M, sub);
-        const uint32_t beg = t.d_ivf_offsets[c], end = t.d_ivf_offsets[c + 1];
-        // List-ordered codes, 16 sub-quantizers: the ids and codes of the next
-        // check are loaded while the current one is scored, so each wave
-        // keeps up to 2 * kPassesPerCheck loads in flight across barriers.
-        constexpr bool kPrefetch = M16 && LISTORDER;
-        uint32_t nids[kPassesPerCheck];
-        uint4 ncw[kPassesPerCheck];
-        if (kPrefetch && beg < end) {
+    // Appends the keys of one check that are below `lim` (<= thr), cutting
+    // the buffer to the S best first if they would not fit.
+    auto append = [&](const uint64_t (&kk)[kPassesPerCheck], uint64_t lim) {
+        // wn[ph] row: written before this check's barrier, read after it
+        // and before the next one; the next write to it is two checks on.
+        uint32_t wn = 0;
+#pragma unroll
+        for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass)
+            wn += (uint32_t)__popcll(__ballot(kk[pass] < lim));
+        if (lane == 0) ctl.wn[ph][wave] = wn;
+        __syncthreads();
+        uint32_t row = ph, total = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < kW; ++w) total += ctl.wn[ph][w];
+        if (cnt + total > cap) {
+            thr = uniform_u64(block_select(buf, cnt, S, ctl.bs));
+            cnt = S;
+            if (publish && threadIdx.x == 0) atomicMin((unsigned long long*)&a.qbound[q], thr);
+            lim = lim < thr ? lim : thr;
+            wn = 0;
+#pragma unroll
+            for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass)
+                wn += (uint32_t)__popcll(__ballot(kk[pass] < lim));
+            if (lane == 0) ctl.wn[2][wave] = wn;
+            __syncthreads();
+            row = 2;
+            total = 0;
+#pragma unroll
+            for (uint32_t w = 0; w < kW; ++w) total += ctl.wn[2][w];
+        }
+        if (total != 0u) {
+            uint32_t pos = cnt;
+#pragma unroll
+            for (uint32_t w = 0; w < kW; ++w) pos += w < wave ? ctl.wn[row][w] : 0u;
 #pragma unroll
             for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
-                const uint32_t e = beg + pass * kIvfThreads + threadIdx.x;
-                const uint32_t ec = e < end ? e : end - 1;
-                nids[pass] = t.d_ivf_vec_ids[ec];
-                ncw[pass] = *reinterpret_cast<const uint4*>(codes_lo + (size_t)ec * 16u);
+                const bool in = kk[pass] < lim;
+                const uint64_t bal = __ballot(in);
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                           __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                if (in) buf[pos + below] = kk[pass];
+                pos += (uint32_t)__popcll(bal);
             }
         }
-        __syncthreads();
-        for (uint32_t base = beg; base < end; base += kEntriesPerCheck) {
-            uint64_t kk[kPassesPerCheck];
-            if (M16) {
-                // Issue the id and code loads of all passes first so that each
-                // wave keeps kPassesPerCheck 128-bit loads in flight.
-                uint32_t ids[kPassesPerCheck];
-                uint4 cw[kPassesPerCheck];
-                if (kPrefetch) {
+        cnt += total;
+        ph ^= 1u;
+    };
+    // Admission bound of the next check: the own threshold, and for a split
+    // query also its shared bound (keys are distinct, so key <= bound is
+    // key < bound + 1). The shared bound is read one check ahead, always (an
+    // unconditional load keeps the wait counts of the prefetched loads exact).
+    uint64_t gnext = kEmptyKey;
+    auto bound = [&]() -> uint64_t {
+        const uint64_t g = uniform_u64(gnext);
+        gnext = __hip_atomic_load(&a.qbound[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!publish) return thr;
+        const uint64_t gb = g == kEmptyKey ? kEmptyKey : g + 1u;
+        return gb < thr ? gb : thr;
+    };
+
+    // Keys of 16-sub-quantizer entries: four lookups per segment; an entry
+    // whose running sum is past the bound's distance skips the rest (LUT
+    // entries are sums of squares, >= 0 or NaN, so the FP32 running sum never
+    // decreases; admitted entries add all 16 terms in order).
+    auto keys_m16 = [&](uint32_t n, const uint32_t (&ids)[kPassesPerCheck],
+                        const uint4 (&cw)[kPassesPerCheck], float thr_d, bool masked,
+                        uint64_t (&kk)[kPassesPerCheck]) {
+        auto seg4 = [&](int seg, uint32_t w, float s) {
+            float l[4];
 #pragma unroll
-                    for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
-                        ids[pass] = nids[pass];
-                        cw[pass] = ncw[pass];
-                    }
-                    const uint32_t nb = base + kEntriesPerCheck;
-                    if (nb < end) {
+            for (int b = 0; b < 4; ++b) l[b] = s_lut16[(seg * 4 + b) * 256u + ((w >> (8 * b)) & 255u)];
 #pragma unroll
-                        for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
-                            const uint32_t e = nb + pass * kIvfThreads + threadIdx.x;
-                            const uint32_t ec = e < end ? e : end - 1;
-                            nids[pass] = t.d_ivf_vec_ids[ec];
-                            ncw[pass] = *reinterpret_cast<const uint4*>(codes_lo + (size_t)ec * 16u);
+            for (int b = 0; b < 4; ++b) s = s + l[b];
+            // Issue the four lookups together (addresses, LDS reads, adds),
+            // not one round trip at a time.
+            __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);
+            return s;
+        };
+#pragma unroll
+        for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
+            kk[pass] = kEmptyKey;
+            if (pass * kIvfThreads + threadIdx.x < n && (!masked || mask_allows(a.mask, ids[pass]))) {
+                float s = seg4(0, cw[pass].x, 0.0f);
+                if (s <= thr_d) {
+                    s = seg4(1, cw[pass].y, s);
+                    if (s <= thr_d) {
+                        s = seg4(2, cw[pass].z, s);
+                        if (s <= thr_d) {
+                            s = seg4(3, cw[pass].w, s);
+                            if (s <= thr_d && s < kAdmit) kk[pass] = make_key(s, ids[pass]);
                         }
                     }
+                }
+            }
+        }
+    };
+    auto dist_of = [](uint64_t lim) {
+        return lim == kEmptyKey ? __builtin_inff() : key_dist(lim);
+    };
+    // List-ordered ids and codes of the check at list position `base` (none
+    // at or past `end`: the loads then return zeros without reading memory).
+    auto load_lo = [&](uint32_t base, uint32_t end, uint32_t (&ids)[kPassesPerCheck],
+                       uint4 (&cw)[kPassesPerCheck]) {
+        // base and end are uniform; readfirstlane keeps the descriptors in
+        // scalar registers.
+        base = __builtin_amdgcn_readfirstlane(base);
+        const uint32_t left = base < end ? end - base : 0u;
+        const uint32_t n = __builtin_amdgcn_readfirstlane(left < kEntriesPerCheck ? left : kEntriesPerCheck);
+        const __amdgpu_buffer_rsrc_t rc = raw_rsrc(a.codes_lo + (size_t)base * 16u, n * 16u);
+        const __amdgpu_buffer_rsrc_t ri = raw_rsrc(t.d_ivf_vec_ids + base, n * 4u);
+#pragma unroll
+        for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
+            const uint32_t e = pass * kIvfThreads + threadIdx.x;
+            cw[pass] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rc, e * 16u, 0, 0));
+            ids[pass] = __builtin_amdgcn_raw_buffer_load_b32(ri, e * 4u, 0, 0);
+        }
+    };
+    // Scores list positions [beg, end) of the list whose LUT is built.
+    auto scan_run = [&](uint32_t beg, uint32_t end) {
+        if constexpr (M16 && LISTORDER) {
+            if (a.mask == nullptr) {
+                // Two register sets, alternating: the next check's loads are
+                // issued (unconditionally) before the current one is scored.
+                uint32_t idA[kPassesPerCheck], idB[kPassesPerCheck];
+                uint4 cwA[kPassesPerCheck], cwB[kPassesPerCheck];
+                uint64_t kk[kPassesPerCheck];
+                load_lo(beg, end, idA, cwA);
+                for (uint32_t base = beg; base < end; base += 2u * kEntriesPerCheck) {
+                    const uint32_t b1 = base + kEntriesPerCheck;
+                    load_lo(b1, end, idB, cwB);
+                    uint64_t lim = bound();
+                    keys_m16(end - base, idA, cwA, dist_of(lim), false, kk);
+                    append(kk, lim);
+                    if (b1 >= end) break;
+                    load_lo(b1 + kEntriesPerCheck, end, idA, cwA);
+                    lim = bound();
+                    keys_m16(end - b1, idB, cwB, dist_of(lim), false, kk);
+                    append(kk, lim);
+                }
+                return;
+            }
+        }
+        for (uint32_t base = beg; base < end; base += kEntriesPerCheck) {
+            const uint64_t lim = bound();
+            const float thr_d = dist_of(lim);
+            uint64_t kk[kPassesPerCheck];
+            if constexpr (M16) {
+                uint32_t ids[kPassesPerCheck];
+                uint4 cw[kPassesPerCheck];
+                if constexpr (LISTORDER) {
+                    load_lo(base, end, ids, cw);
                 } else {
+                    // Codes gathered by vector id.
 #pragma unroll
                     for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
                         const uint32_t e = base + pass * kIvfThreads + threadIdx.x;
-                        const uint32_t ec = e < end ? e : end - 1;
-                        ids[pass] = t.d_ivf_vec_ids[ec];
-                        if (LISTORDER) cw[pass] = *reinterpret_cast<const uint4*>(codes_lo + (size_t)ec * 16u);
+                        ids[pass] = t.d_ivf_vec_ids[e < end ? e : end - 1];
                     }
-                }
-                if (!LISTORDER) {
 #pragma unroll
                     for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
                         const uint8_t* cp = t.d_pq_codes + (size_t)ids[pass] * 16u;
@@ -306,21 +477,7 @@ __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(
                         }
                     }
                 }
-#pragma unroll
-                for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
-                    const uint32_t e = base + pass * kIvfThreads + threadIdx.x;
-                    const uint32_t id = ids[pass];
-                    kk[pass] = kEmptyKey;
-                    if (e >= end || !mask_allows(mask, id)) continue;
-                    const uint32_t ww[4] = {cw[pass].x, cw[pass].y, cw[pass].z, cw[pass].w};
-                    float adc = 0.0f;
-#pragma unroll
-                    for (int b = 0; b < 16; ++b) {
-                        const uint32_t code = (ww[b >> 2] >> (8 * (b & 3))) & 255u;
-                        adc = adc + s_lut16[b * 256u + code];
-                    }
-                    if (adc < kAdmit) kk[pass] = make_key(adc, id);
-                }
+                keys_m16(end - base, ids, cw, thr_d, true, kk);
             } else {
 #pragma unroll
                 for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
@@ -328,53 +485,43 @@ __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(
                     kk[pass] = kEmptyKey;
                     if (e >= end) continue;
                     const uint32_t id = t.d_ivf_vec_ids[e];
-                    if (!mask_allows(mask, id)) continue;
-                    const uint8_t* cp = LISTORDER ? codes_lo + (size_t)e * M : t.d_pq_codes + (size_t)id * M;
+                    if (!mask_allows(a.mask, id)) continue;
+                    const uint8_t* cp = LISTORDER ? a.codes_lo + (size_t)e * M : t.d_pq_codes + (size_t)id * M;
                     float adc = 0.0f;
-                    for (uint32_t m = 0; m < M; ++m) adc = adc + lut[m * 256u + cp[m]];
-                    if (adc < kAdmit) kk[pass] = make_key(adc, id);
+                    bool full = true;
+                    for (uint32_t m = 0; m < M; ++m) {
+                        adc = adc + lut[m * 256u + cp[m]];
+                        if ((m & 3u) == 3u && m + 1u < M && !(adc <= thr_d)) {
+                            full = false;
+                            break;
+                        }
+                    }
+                    if (full && adc < kAdmit) kk[pass] = make_key(adc, id);
                 }
             }
-            // wn[ph] row: written before this check's barrier, read after it
-            // and before the next one; the next write to it is two checks on.
-            uint32_t wn = 0;
-#pragma unroll
-            for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass)
-                wn += (uint32_t)__popcll(__ballot(kk[pass] < thr));
-            if (lane == 0) ctl.wn[ph][wave] = wn;
-            __syncthreads();
-            uint32_t row = ph, total = 0;
-#pragma unroll
-            for (uint32_t w = 0; w < kW; ++w) total += ctl.wn[ph][w];
-            if (cnt + total > cap) {
-                thr = block_select(buf, cnt, S, ctl.bs);
-                cnt = S;
-                wn = 0;
-#pragma unroll
-                for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass)
-                    wn += (uint32_t)__popcll(__ballot(kk[pass] < thr));
-                if (lane == 0) ctl.wn[2][wave] = wn;
-                __syncthreads();
-                row = 2;
-                total = 0;
-#pragma unroll
-                for (uint32_t w = 0; w < kW; ++w) total += ctl.wn[2][w];
-            }
-            uint32_t pos = cnt;
-#pragma unroll
-            for (uint32_t w = 0; w < kW; ++w) pos += w < wave ? ctl.wn[row][w] : 0u;
-#pragma unroll
-            for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
-                const bool in = kk[pass] < thr;
-                const uint64_t bal = __ballot(in);
-                const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
-                                           __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                if (in) buf[pos + below] = kk[pass];
-                pos += (uint32_t)__popcll(bal);
-            }
-            cnt += total;
-            ph ^= 1u;
+            append(kk, lim);
         }
+    };
+
+    // This item's entries: positions [e0, e1) of the concatenation of the
+    // probed lists.
+    uint64_t cum = 0;
+    for (uint32_t p = 0; p < a.max_nprobe && cum < e1; ++p) {
+        const uint32_t c = a.probes[(size_t)q * a.max_nprobe + p];
+        if (c >= t.nlist) break;  // probes are packed; the rest are empty
+        const uint32_t lb = t.d_ivf_offsets[c], le = t.d_ivf_offsets[c + 1];
+        const uint64_t len = le - lb;
+        const uint64_t s0 = cum > e0 ? cum : e0, s1 = cum + len < e1 ? cum + len : e1;
+        if (s0 < s1) {
+            const float* cen = t.d_centroids + (size_t)c * dim;
+            // The previous list's LUT build is done with res (barrier after it).
+            for (uint32_t d = threadIdx.x; d < dim; d += kIvfThreads) res[d] = qv[d] - cen[d];
+            __syncthreads();  // res visible; the previous list's lookups are done with the LUT
+            build_lut(lut, t.d_pq_codebooks, res, M, sub);
+            __syncthreads();
+            scan_run(lb + (uint32_t)(s0 - cum), lb + (uint32_t)(s1 - cum));
+        }
+        cum += len;
     }
     __syncthreads();
     uint32_t kept = cnt;
@@ -383,9 +530,48 @@ __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(
         kept = S;
         __syncthreads();
     }
+    if (split) {
+        // Keep this item's shortlist; the last item of the query merges.
+        const uint32_t slot = a.slot_start[q] + part;
+        uint64_t* dst = a.part_keys + (size_t)slot * a.shortlist;
+        for (uint32_t i = threadIdx.x; i < kept; i += kIvfThreads) dst[i] = buf[i];
+        if (threadIdx.x == 0) a.part_cnt[slot] = kept;
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) ctl.item = atomicAdd(&a.done[q], 1u) + 1u == nit ? 1u : 0u;
+        __syncthreads();
+        if (__builtin_amdgcn_readfirstlane(ctl.item) == 0u) continue;
+        __threadfence();
+        cnt = 0;
+        thr = kEmptyKey;
+        publish = false;
+        const uint32_t s0 = a.slot_start[q];
+        for (uint32_t sl = s0; sl < s0 + nit; ++sl) {
+            const uint32_t n = __builtin_amdgcn_readfirstlane(
+                __hip_atomic_load(&a.part_cnt[sl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            uint64_t* src = a.part_keys + (size_t)sl * a.shortlist;
+            for (uint32_t base = 0; base < n; base += kEntriesPerCheck) {
+                uint64_t kk[kPassesPerCheck];
+#pragma unroll
+                for (uint32_t pass = 0; pass < kPassesPerCheck; ++pass) {
+                    const uint32_t e = base + pass * kIvfThreads + threadIdx.x;
+                    kk[pass] = e < n ? __hip_atomic_load(&src[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                     : kEmptyKey;
+                }
+                append(kk, thr);
+            }
+        }
+        __syncthreads();
+        kept = cnt;
+        if (kept > S) {
+            block_select(buf, kept, S, ctl.bs);
+            kept = S;
+            __syncthreads();
+        }
+    }
     // Exact rerank of the shortlist, in place. Rows that are not admitted get
     // distinct keys above every admitted one (distance bits all ones).
-    if (do_rerank && t.d_vectors) {
+    if (a.do_rerank && t.d_vectors) {
         for (uint32_t i = threadIdx.x; i < kept; i += blockDim.x) {
             const uint32_t id = key_slot(buf[i]);
             const float d = id < t.num_vectors
@@ -416,7 +602,7 @@ __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(
         __syncthreads();
         block_bitonic_sort(buf, n);
     }
-    for (uint32_t i = threadIdx.x; i < out_stride; i += blockDim.x) {
+    for (uint32_t i = threadIdx.x; i < a.out_stride; i += blockDim.x) {
         GpuSearchResult r;
         const uint64_t key = i < kout ? buf[i] : kEmptyKey;
         if ((key >> 32) == 0xFFFFFFFFull) {
@@ -426,17 +612,18 @@ __global__ __launch_bounds__(kIvfThreads) void ivf_scan_kernel(
             r.id = key_slot(key);
             r.distance = key_dist(key);
         }
-        out[(size_t)q * out_stride + i] = r;
+        a.out[(size_t)q * a.out_stride + i] = r;
     }
   }
 }
 
 // Scan cost of each query (entries over its probed lists), bucketed by
-// log2 so that the heaviest queries are dispatched first.
+// log2 so that the heaviest queries are dispatched first. Costs saturate at
+// 2^32 - 1 entries.
 __global__ void ivf_cost_kernel(const uint32_t* __restrict__ probes, uint32_t max_nprobe,
                                 const uint32_t* __restrict__ offsets, uint32_t nlist,
-                                uint32_t nq, uint32_t* __restrict__ bucket,
-                                uint32_t* __restrict__ bucket_count) {
+                                uint32_t nq, uint32_t* __restrict__ cost_out,
+                                uint32_t* __restrict__ bucket, uint32_t* __restrict__ bucket_count) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nq) return;
     uint64_t cost = 0;
@@ -446,6 +633,7 @@ __global__ void ivf_cost_kernel(const uint32_t* __restrict__ probes, uint32_t ma
         cost += offsets[c + 1] - offsets[c];
     }
     const uint32_t c32 = cost > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)cost;
+    cost_out[q] = c32;
     const uint32_t b = (uint32_t)__clz(c32 | 1u);  // 0 = heaviest
     bucket[q] = b;
     atomicAdd(&bucket_count[b], 1u);
@@ -467,6 +655,81 @@ __global__ void ivf_order_kernel(const uint32_t* __restrict__ bucket,
         const uint32_t b = bucket[q];
         order[s_off[b] + atomicAdd(&bucket_fill[b], 1u)] = q;
     }
+}
+
+// Exclusive prefix sums of two per-thread counts over a 1024-thread
+// workgroup; returns the totals in tot0/tot1.
+__device__ __forceinline__ void block_scan2(uint32_t& v0, uint32_t& v1, uint32_t& tot0,
+                                            uint32_t& tot1, uint32_t* s0, uint32_t* s1) {
+    const uint32_t tid = threadIdx.x;
+    s0[tid] = v0;
+    s1[tid] = v1;
+    __syncthreads();
+    for (uint32_t off = 1; off < blockDim.x; off <<= 1) {
+        const uint32_t a0 = tid >= off ? s0[tid - off] : 0u, a1 = tid >= off ? s1[tid - off] : 0u;
+        __syncthreads();
+        s0[tid] += a0;
+        s1[tid] += a1;
+        __syncthreads();
+    }
+    tot0 = s0[blockDim.x - 1];
+    tot1 = s1[blockDim.x - 1];
+    v0 = s0[tid] - v0;
+    v1 = s1[tid] - v1;
+    __syncthreads();
+}
+
+// Work items of the batch (one workgroup of 1024 threads). Chunk length C =
+// max(chunk_min, ceil(total / target_items), ceil(2 * total / max_slots)); a
+// query of more than C entries becomes ceil(cost / C) items, every other
+// query one item. The last bound keeps the split queries' items, at most
+// 2 * total / C, within the max_slots partial slots. target_items = 0 or
+// max_slots = 0 splits nothing.
+__global__ __launch_bounds__(1024) void ivf_items_kernel(
+    const uint32_t* __restrict__ cost, const uint32_t* __restrict__ order, uint32_t nq,
+    uint32_t chunk_min, uint32_t target_items, uint32_t max_slots,
+    uint32_t* __restrict__ item_start, uint32_t* __restrict__ slot_start) {
+    __shared__ uint32_t s0[1024], s1[1024];
+    __shared__ unsigned long long s_total;
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) s_total = 0;
+    __syncthreads();
+    uint64_t my = 0;
+    for (uint32_t i = tid; i < nq; i += blockDim.x) my += cost[i];
+    atomicAdd(&s_total, (unsigned long long)my);
+    __syncthreads();
+    const uint64_t total = s_total;
+    uint64_t C = ~0ull;
+    if (target_items != 0u && max_slots != 0u) {
+        C = chunk_min ? chunk_min : 1u;
+        const uint64_t c1 = (total + target_items - 1u) / target_items;
+        const uint64_t c2 = (2u * total + max_slots - 1u) / max_slots;
+        C = C > c1 ? C : c1;
+        C = C > c2 ? C : c2;
+    }
+    auto items_of = [&](uint32_t i) -> uint32_t {
+        const uint64_t c = cost[order[i]];
+        return c > C ? (uint32_t)((c + C - 1u) / C) : 1u;
+    };
+    // Each thread owns a contiguous run of positions.
+    const uint32_t per = (nq + blockDim.x - 1u) / blockDim.x;
+    const uint32_t i0 = min(nq, tid * per), i1 = min(nq, i0 + per);
+    uint32_t ni = 0, ns = 0;
+    for (uint32_t i = i0; i < i1; ++i) {
+        const uint32_t n = items_of(i);
+        ni += n;
+        ns += n > 1u ? n : 0u;
+    }
+    uint32_t tot_i, tot_s;
+    block_scan2(ni, ns, tot_i, tot_s, s0, s1);
+    for (uint32_t i = i0; i < i1; ++i) {
+        const uint32_t n = items_of(i);
+        item_start[i] = ni;
+        slot_start[order[i]] = ns;
+        ni += n;
+        ns += n > 1u ? n : 0u;
+    }
+    if (tid == 0) item_start[nq] = tot_i;
 }
 
 // Probe list of each query from the coarse top-k results: the first np_q
@@ -510,7 +773,13 @@ IvfPqPlan plan_ivfpq(const IvfPqGpuTables* t, uint32_t nq, uint32_t max_nprobe,
     // Dynamic part only: with 16 sub-quantizers the LUT is static LDS.
     p.lds_scan = (t->m_pq == 16u ? 0 : lut_bytes) + (p.gbuf ? 0 : (size_t)p.cap * 8) +
                  kIvfCtlBytes + (size_t)t->dim * sizeof(float);
-    p.grid = nq < kMaxScanGrid ? nq : kMaxScanGrid;
+    p.grid = kMaxScanGrid;
+    // Partial shortlists of split queries: at most kMaxPartBytes of slots.
+    const size_t slot_bytes = (size_t)S * 8u;
+    const size_t slots = kMaxPartBytes / slot_bytes;
+    p.max_slots = (uint32_t)(slots < kMaxSlots ? slots : kMaxSlots);
+    p.target_items = kDefaultTargetItems;
+    p.chunk_min = kDefaultChunkMin;
     size_t off = 0;
     auto take = [&](size_t bytes) {
         const size_t o = off;
@@ -520,8 +789,15 @@ IvfPqPlan plan_ivfpq(const IvfPqGpuTables* t, uint32_t nq, uint32_t max_nprobe,
     p.off_probes = take((size_t)nq * max_nprobe * 4);
     p.off_bucket = take((size_t)nq * 4);
     p.off_order = take((size_t)nq * 4);
-    p.off_counters = take(68 * 4);  // 33 counts, 33 fills, next query, pad
+    p.off_counters = take(68 * 4);  // 33 counts, 33 fills, next item, pad
     p.off_gbuf = take(p.gbuf ? (size_t)p.grid * p.cap * 8 : 0);
+    p.off_cost = take((size_t)nq * 4);
+    p.off_items = take(((size_t)nq + 1) * 4);
+    p.off_slots = take((size_t)nq * 4);
+    p.off_done = take((size_t)nq * 4);
+    p.off_qbound = take((size_t)nq * 8);
+    p.off_part_cnt = take((size_t)p.max_slots * 4);
+    p.off_part_keys = take((size_t)p.max_slots * slot_bytes);
     if (max_nprobe <= kTiledCoarseMaxProbes && t->nlist >= 64u) p.coarse_mode = kCoarseTiled;
     else if (p.npow <= kLdsCoarseMaxLists) p.coarse_mode = kCoarseLds;
     else p.coarse_mode = kCoarseGlobal;
@@ -537,31 +813,20 @@ IvfPqPlan plan_ivfpq(const IvfPqGpuTables* t, uint32_t nq, uint32_t max_nprobe,
 }
 
 template <bool LO, bool GB, bool M16>
-static void launch_scan_m(int metric, dim3 g, size_t lds, hipStream_t s, const IvfPqGpuTables& t,
-                          const uint8_t* lo, const float* Qm, uint32_t nq, const uint32_t* order,
-                          uint32_t* next_q, const uint32_t* probes, uint32_t mnp,
-                          const uint32_t* k_q, uint32_t k_u, uint32_t S, uint32_t cap, int rr,
-                          const uint32_t* mask, uint64_t* gbuf, GpuSearchResult* out,
-                          uint32_t stride) {
+static void launch_scan_m(int metric, dim3 g, size_t lds, hipStream_t s, const IvfScanArgs& a) {
     if (metric == METRIC_INNER_PRODUCT)
-        ivf_scan_kernel<LO, GB, METRIC_INNER_PRODUCT, M16><<<g, kIvfThreads, lds, s>>>(t, lo, Qm, nq, order, next_q, probes, mnp, k_q, k_u, S, cap, rr, mask, gbuf, out, stride);
+        ivf_scan_kernel<LO, GB, METRIC_INNER_PRODUCT, M16><<<g, kIvfThreads, lds, s>>>(a);
     else if (metric == METRIC_COSINE)
-        ivf_scan_kernel<LO, GB, METRIC_COSINE, M16><<<g, kIvfThreads, lds, s>>>(t, lo, Qm, nq, order, next_q, probes, mnp, k_q, k_u, S, cap, rr, mask, gbuf, out, stride);
+        ivf_scan_kernel<LO, GB, METRIC_COSINE, M16><<<g, kIvfThreads, lds, s>>>(a);
     else
-        ivf_scan_kernel<LO, GB, METRIC_L2, M16><<<g, kIvfThreads, lds, s>>>(t, lo, Qm, nq, order, next_q, probes, mnp, k_q, k_u, S, cap, rr, mask, gbuf, out, stride);
+        ivf_scan_kernel<LO, GB, METRIC_L2, M16><<<g, kIvfThreads, lds, s>>>(a);
 }
 
 template <bool LO, bool GB>
 static void launch_scan_lo_gb(bool m16, int metric, dim3 g, size_t lds, hipStream_t s,
-                              const IvfPqGpuTables& t, const uint8_t* lo, const float* Qm,
-                              uint32_t nq, const uint32_t* order, uint32_t* next_q,
-                              const uint32_t* probes, uint32_t mnp, const uint32_t* k_q,
-                              uint32_t k_u, uint32_t S, uint32_t cap, int rr, const uint32_t* mask,
-                              uint64_t* gbuf, GpuSearchResult* out, uint32_t stride) {
-    if (m16)
-        launch_scan_m<LO, GB, true>(metric, g, lds, s, t, lo, Qm, nq, order, next_q, probes, mnp, k_q, k_u, S, cap, rr, mask, gbuf, out, stride);
-    else
-        launch_scan_m<LO, GB, false>(metric, g, lds, s, t, lo, Qm, nq, order, next_q, probes, mnp, k_q, k_u, S, cap, rr, mask, gbuf, out, stride);
+                              const IvfScanArgs& a) {
+    if (m16) launch_scan_m<LO, GB, true>(metric, g, lds, s, a);
+    else launch_scan_m<LO, GB, false>(metric, g, lds, s, a);
 }
 
 hipError_t launch_ivfpq(const IvfPqGpuTables* t, const uint8_t* codes_lo, const float* Qm,
@@ -573,10 +838,10 @@ hipError_t launch_ivfpq(const IvfPqGpuTables* t, const uint8_t* codes_lo, const 
     if ((t->m_pq == 16u ? 16384u : 0u) + p.lds_scan > 160u * 1024u) return hipErrorInvalidValue;
     char* ws = static_cast<char*>(workspace);
     uint32_t* probes = reinterpret_cast<uint32_t*>(ws + p.off_probes);
-    uint64_t* gbuf = p.gbuf ? reinterpret_cast<uint64_t*>(ws + p.off_gbuf) : nullptr;
     uint32_t* bucket = reinterpret_cast<uint32_t*>(ws + p.off_bucket);
     uint32_t* order = reinterpret_cast<uint32_t*>(ws + p.off_order);
     uint32_t* ctr = reinterpret_cast<uint32_t*>(ws + p.off_counters);
+    uint32_t* cost = reinterpret_cast<uint32_t*>(ws + p.off_cost);
     if (p.coarse_mode == kCoarseTiled) {
         // Tiled exact scan with fmaf accumulation: same values and (distance,
         // cluster) order as the per-query coarse kernel, at GEMM-like reuse.
@@ -596,24 +861,52 @@ hipError_t launch_ivfpq(const IvfPqGpuTables* t, const uint8_t* codes_lo, const 
             t->d_centroids, t->nlist, t->dim, Qm, nprobe_q, nprobe_u, p.max_nprobe,
             reinterpret_cast<uint64_t*>(ws + p.off_coarse_keys), probes);
     }
+    // Counters, the split queries' finished-item counts and shared bounds
+    // start cleared (fill kernels, so the sequence can be captured).
     fill_u32(ctr, 68, 0u, stream);
+    fill_u32(reinterpret_cast<uint32_t*>(ws + p.off_done), nq, 0u, stream);
+    fill_u32(reinterpret_cast<uint32_t*>(ws + p.off_qbound), (size_t)nq * 2, 0xFFFFFFFFu, stream);
     ivf_cost_kernel<<<(nq + 255) / 256, 256, 0, stream>>>(probes, p.max_nprobe, t->d_ivf_offsets,
-                                                         t->nlist, nq, bucket, ctr);
+                                                         t->nlist, nq, cost, bucket, ctr);
     ivf_order_kernel<<<(nq + 255) / 256 < 64 ? (nq + 255) / 256 : 64, 256, 0, stream>>>(
         bucket, ctr, nq, ctr + 33, order);
-    uint32_t* next_q = ctr + 66;
+    uint32_t* item_start = reinterpret_cast<uint32_t*>(ws + p.off_items);
+    uint32_t* slot_start = reinterpret_cast<uint32_t*>(ws + p.off_slots);
+    ivf_items_kernel<<<1, 1024, 0, stream>>>(cost, order, nq, p.chunk_min, p.target_items,
+                                             p.max_slots, item_start, slot_start);
+    IvfScanArgs a;
+    a.t = *t;
+    a.codes_lo = codes_lo;
+    a.Qm = Qm;
+    a.probes = probes;
+    a.max_nprobe = p.max_nprobe;
+    a.k_q = k_q;
+    a.k_u = k_u;
+    a.shortlist = p.shortlist;
+    a.cap = p.cap;
+    a.do_rerank = do_rerank;
+    a.mask = mask;
+    a.gbuf = p.gbuf ? reinterpret_cast<uint64_t*>(ws + p.off_gbuf) : nullptr;
+    a.out = out;
+    a.out_stride = out_stride;
+    a.nq = nq;
+    a.order = order;
+    a.item_start = item_start;
+    a.cost = cost;
+    a.slot_start = slot_start;
+    a.next_item = ctr + 66;
+    a.done = reinterpret_cast<uint32_t*>(ws + p.off_done);
+    a.qbound = reinterpret_cast<uint64_t*>(ws + p.off_qbound);
+    a.part_keys = reinterpret_cast<uint64_t*>(ws + p.off_part_keys);
+    a.part_cnt = reinterpret_cast<uint32_t*>(ws + p.off_part_cnt);
     void* timer = timer_begin(stream);
     const dim3 g(p.grid);
     const bool lo = codes_lo != nullptr;
     const bool m16 = t->m_pq == 16u;
-    if (lo && p.gbuf)
-        launch_scan_lo_gb<true, true>(m16, metric, g, p.lds_scan, stream, *t, codes_lo, Qm, nq, order, next_q, probes, p.max_nprobe, k_q, k_u, p.shortlist, p.cap, do_rerank, mask, gbuf, out, out_stride);
-    else if (lo)
-        launch_scan_lo_gb<true, false>(m16, metric, g, p.lds_scan, stream, *t, codes_lo, Qm, nq, order, next_q, probes, p.max_nprobe, k_q, k_u, p.shortlist, p.cap, do_rerank, mask, gbuf, out, out_stride);
-    else if (p.gbuf)
-        launch_scan_lo_gb<false, true>(m16, metric, g, p.lds_scan, stream, *t, codes_lo, Qm, nq, order, next_q, probes, p.max_nprobe, k_q, k_u, p.shortlist, p.cap, do_rerank, mask, gbuf, out, out_stride);
-    else
-        launch_scan_lo_gb<false, false>(m16, metric, g, p.lds_scan, stream, *t, codes_lo, Qm, nq, order, next_q, probes, p.max_nprobe, k_q, k_u, p.shortlist, p.cap, do_rerank, mask, gbuf, out, out_stride);
+    if (lo && p.gbuf) launch_scan_lo_gb<true, true>(m16, metric, g, p.lds_scan, stream, a);
+    else if (lo) launch_scan_lo_gb<true, false>(m16, metric, g, p.lds_scan, stream, a);
+    else if (p.gbuf) launch_scan_lo_gb<false, true>(m16, metric, g, p.lds_scan, stream, a);
+    else launch_scan_lo_gb<false, false>(m16, metric, g, p.lds_scan, stream, a);
     timer_end(timer, stream);
     return hipGetLastError();
 }
@@ -673,12 +966,13 @@ size_t fv_ivf_pq_workspace_size(const IvfPqGpuTables* tables, uint32_t num_queri
     return fv::plan_ivfpq(tables, num_queries, np, shortlist, max_top_k).workspace_bytes;
 }
 
-int fv_ivf_pq_search(const IvfPqGpuTables* tables, const uint8_t* d_codes_list_order,
-                     const float* d_queries, uint32_t num_queries, const uint32_t* d_top_k,
-                     uint32_t max_top_k, const uint32_t* d_nprobe, uint32_t max_nprobe,
-                     uint32_t shortlist, int rerank, MetricType metric,
-                     const uint32_t* d_valid_mask, GpuSearchResult* d_out_results,
-                     void* d_workspace, size_t workspace_bytes, void* stream) {
+int fv_ivf_pq_search_tuned(const IvfPqGpuTables* tables, const uint8_t* d_codes_list_order,
+                           const float* d_queries, uint32_t num_queries, const uint32_t* d_top_k,
+                           uint32_t max_top_k, const uint32_t* d_nprobe, uint32_t max_nprobe,
+                           uint32_t shortlist, int rerank, MetricType metric,
+                           const uint32_t* d_valid_mask, GpuSearchResult* d_out_results,
+                           void* d_workspace, size_t workspace_bytes,
+                           const FvIvfPqTuning* tuning, void* stream) {
     if (!tables || !d_queries || !d_out_results) return -1;
     if (!fv::tables_ok(tables) || max_top_k == 0 || max_top_k > FV_MAX_TOPK ||
         shortlist > FV_MAX_SHORTLIST || max_nprobe == 0)
@@ -687,12 +981,28 @@ int fv_ivf_pq_search(const IvfPqGpuTables* tables, const uint8_t* d_codes_list_o
     uint32_t np = max_nprobe > tables->nlist ? tables->nlist : max_nprobe;
     fv::IvfPqPlan p = fv::plan_ivfpq(tables, num_queries, np, shortlist, max_top_k);
     if (!d_workspace || workspace_bytes < p.workspace_bytes) return (int)hipErrorInvalidValue;
+    if (tuning) {
+        p.target_items = tuning->target_items;
+        if (tuning->chunk_min) p.chunk_min = tuning->chunk_min;
+    }
     const int do_rerank = rerank && tables->d_vectors != nullptr;
     hipError_t e = fv::launch_ivfpq(tables, d_codes_list_order, d_queries, num_queries, d_top_k,
                                     max_top_k, d_nprobe, np, do_rerank, (int)metric, d_valid_mask,
                                     d_out_results, max_top_k, d_workspace, p,
                                     static_cast<hipStream_t>(stream));
     return e == hipSuccess ? 0 : (int)e;
+}
+
+int fv_ivf_pq_search(const IvfPqGpuTables* tables, const uint8_t* d_codes_list_order,
+                     const float* d_queries, uint32_t num_queries, const uint32_t* d_top_k,
+                     uint32_t max_top_k, const uint32_t* d_nprobe, uint32_t max_nprobe,
+                     uint32_t shortlist, int rerank, MetricType metric,
+                     const uint32_t* d_valid_mask, GpuSearchResult* d_out_results,
+                     void* d_workspace, size_t workspace_bytes, void* stream) {
+    return fv_ivf_pq_search_tuned(tables, d_codes_list_order, d_queries, num_queries, d_top_k,
+                                  max_top_k, d_nprobe, max_nprobe, shortlist, rerank, metric,
+                                  d_valid_mask, d_out_results, d_workspace, workspace_bytes,
+                                  nullptr, stream);
 }
 
 }  // extern "C"

@@ -63,6 +63,15 @@ def committed(index, top_k, metric, rows, nq, f64_boundary=False, shortlist_fact
     return (to_tuning(entry) if entry else None), key
 
 
+def committed_entry(key: str):
+    """The committed plan entry (a dict) under `key`, or None."""
+    try:
+        with open(COMMITTED) as f:
+            return json.load(f).get(key)
+    except (OSError, ValueError):
+        return None
+
+
 def to_tuning(entry: dict):
     t = L.FvFlatTuning()
     t.shortlist_factor = entry["shortlist_factor"]

@@ -108,6 +108,17 @@ class FvEngineInfo(ctypes.Structure):
         "flat_dtype")]
 
 
+class FvIvfPqTuning(ctypes.Structure):
+    """include/flashvector/fv_ext.h FvIvfPqTuning."""
+
+    _fields_ = [
+        ("target_items", ctypes.c_uint32),
+        ("chunk_min", ctypes.c_uint32),
+        ("ms", ctypes.c_float),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
 class FvFlatTuning(ctypes.Structure):
     """include/flashvector/fv_ext.h FvFlatTuning."""
 
@@ -180,6 +191,8 @@ SIGNATURES = {
     "fv_ivf_pq_list_order_codes": (I, [P, P, P]),
     "fv_ivf_pq_workspace_size": (SZ, [P, U, U, U, U]),
     "fv_ivf_pq_search": (I, [P, P, P, U, P, U, P, U, U, I, I, P, P, P, SZ, P]),
+    "fv_ivf_pq_search_tuned": (I, [P, P, P, U, P, U, P, U, U, I, I, P, P, P, SZ, P, P]),
+    "fv_ivf_pq_autotune": (I, [P, P, P, U, P, U, P, U, U, I, I, P, U, P, P]),
     "fv_flat_autotune": (I, [P, P, U, P, ctypes.c_float, U, P, P]),
     "fvh_create": (I, [P, P]),
     "fvh_destroy": (None, [P]),

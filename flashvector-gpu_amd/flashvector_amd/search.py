@@ -256,8 +256,10 @@ class IvfPqIndex:
         return out
 
     def search(self, queries, max_top_k, max_nprobe, shortlist, top_k=None, nprobe=None,
-               rerank=True, metric="l2", valid_mask=None, use_list_order=True, out=None):
-        """fv_ivf_pq_search: optional per-query top_k / nprobe (uint32 device tensors)."""
+               rerank=True, metric="l2", valid_mask=None, use_list_order=True, out=None,
+               tuning=None):
+        """fv_ivf_pq_search: optional per-query top_k / nprobe (uint32 device
+        tensors); `tuning` (an FvIvfPqTuning) overrides the work-item plan."""
         torch = L._torch()
         lib = L.load()
         nq = queries.shape[0]
@@ -269,9 +271,26 @@ class IvfPqIndex:
             self._ws = torch.empty(need, dtype=torch.uint8, device=queries.device)
         out = L.result_tensor(nq, max_top_k, queries.device) if out is None else out
         lo = self.codes_lo if use_list_order else None
-        rc = lib.fv_ivf_pq_search(ctypes.byref(self.tables), L.ptr(lo), L.ptr(queries), nq,
-                                  L.ptr(top_k), max_top_k, L.ptr(nprobe), max_nprobe, shortlist,
-                                  int(rerank), _metric(metric), L.ptr(valid_mask), L.ptr(out),
-                                  L.ptr(self._ws), self._ws.numel(), L.stream_handle())
-        L.check(rc, "fv_ivf_pq_search")
+        rc = lib.fv_ivf_pq_search_tuned(ctypes.byref(self.tables), L.ptr(lo), L.ptr(queries), nq,
+                                        L.ptr(top_k), max_top_k, L.ptr(nprobe), max_nprobe,
+                                        shortlist, int(rerank), _metric(metric),
+                                        L.ptr(valid_mask), L.ptr(out), L.ptr(self._ws),
+                                        self._ws.numel(),
+                                        None if tuning is None else ctypes.byref(tuning),
+                                        L.stream_handle())
+        L.check(rc, "fv_ivf_pq_search_tuned")
         return out
+
+    def autotune(self, queries, max_top_k, max_nprobe, shortlist, top_k=None, nprobe=None,
+                 rerank=True, metric="l2", valid_mask=None, use_list_order=True, trials=3):
+        """fv_ivf_pq_autotune for this batch shape; returns an FvIvfPqTuning."""
+        lib = L.load()
+        best = L.FvIvfPqTuning()
+        lo = self.codes_lo if use_list_order else None
+        rc = lib.fv_ivf_pq_autotune(ctypes.byref(self.tables), L.ptr(lo), L.ptr(queries),
+                                    queries.shape[0], L.ptr(top_k), max_top_k, L.ptr(nprobe),
+                                    max_nprobe, shortlist, int(rerank), _metric(metric),
+                                    L.ptr(valid_mask), trials, ctypes.byref(best),
+                                    L.stream_handle())
+        L.check(rc, "fv_ivf_pq_autotune")
+        return best

@@ -154,7 +154,7 @@ int fv_flat_autotune(const FvFlatTables* tables, const float* d_queries, uint32_
  * Output rows have stride max_top_k and are padded past k_i. When
  * d_codes_list_order is non-NULL it must hold the codes in posting-list order
  * (fv_ivf_pq_list_order_codes), which turns the per-entry gathers into
- * coalesced streaming reads. nlist <= 8192. */
+ * coalesced streaming reads. Any nlist. */
 int fv_ivf_pq_list_order_codes(const IvfPqGpuTables* tables, uint8_t* d_codes_out, void* stream);
 size_t fv_ivf_pq_workspace_size(const IvfPqGpuTables* tables, uint32_t num_queries,
                                 uint32_t max_nprobe, uint32_t shortlist, uint32_t max_top_k);
@@ -164,6 +164,41 @@ int fv_ivf_pq_search(const IvfPqGpuTables* tables, const uint8_t* d_codes_list_o
                      uint32_t shortlist, int rerank, MetricType metric,
                      const uint32_t* d_valid_mask, GpuSearchResult* d_out_results,
                      void* d_workspace, size_t workspace_bytes, void* stream);
+
+/* Work-item plan of the IVF-PQ scan. The batch is scanned by persistent
+ * workgroups over work items: a query whose probed lists hold more than C
+ * entries, C = max(chunk_min, total / target_items, 2 * total / slots) with
+ * `total` the batch's probed entries and `slots` fixed by the shortlist
+ * depth, is cut into items of at most C entries (its lists concatenated in
+ * probe order) that several workgroups scan at once; the item that finishes
+ * last merges their shortlists. target_items = 0 keeps one item per query.
+ * Results do not depend on the plan. */
+typedef struct {
+    uint32_t target_items;
+    uint32_t chunk_min;           /* entries; 0 = library default */
+    float ms;                     /* autotuner: median time of the batch with this plan */
+    uint32_t reserved;
+} FvIvfPqTuning;
+/* fv_ivf_pq_search with an explicit work-item plan (NULL = library default). */
+int fv_ivf_pq_search_tuned(const IvfPqGpuTables* tables, const uint8_t* d_codes_list_order,
+                           const float* d_queries, uint32_t num_queries, const uint32_t* d_top_k,
+                           uint32_t max_top_k, const uint32_t* d_nprobe, uint32_t max_nprobe,
+                           uint32_t shortlist, int rerank, MetricType metric,
+                           const uint32_t* d_valid_mask, GpuSearchResult* d_out_results,
+                           void* d_workspace, size_t workspace_bytes,
+                           const FvIvfPqTuning* tuning, void* stream);
+/* Autotuner of the IVF-PQ work-item plan for one batch shape: times the
+ * search (median of `trials` runs after a warm-up) over target_items in
+ * {0, 256, 1024, 4096, 16384} x chunk_min in {4096, 16384, 65536} and
+ * returns the fastest, keeping the library default unless a candidate is at
+ * least 3% faster. Allocates and frees its own device memory and
+ * synchronizes the stream: a set-up call, not for the search path. */
+int fv_ivf_pq_autotune(const IvfPqGpuTables* tables, const uint8_t* d_codes_list_order,
+                       const float* d_queries, uint32_t num_queries, const uint32_t* d_top_k,
+                       uint32_t max_top_k, const uint32_t* d_nprobe, uint32_t max_nprobe,
+                       uint32_t shortlist, int rerank, MetricType metric,
+                       const uint32_t* d_valid_mask, uint32_t trials, FvIvfPqTuning* out_best,
+                       void* stream);
 
 /* ---- top-k inner-product backward ----------------------------------------
  * Scores s_ij = q_i . x_{ids[i][j]} of the rows a top-k search selected

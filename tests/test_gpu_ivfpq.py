@@ -212,3 +212,35 @@ def test_clamps_and_empty_lists(torch, case):
     out = ei.search(q, 32, 12, 300, top_k=dev(torch, ks.view(np.int32)),
                     nprobe=dev(torch, nps.view(np.int32)), use_list_order=True)
     assert_same(L.split_results(out), O.ivf_pq_search_uneven(eh, q40, ks, nps, 300, out_stride=32))
+
+
+# Work-item plans: queries split into chunks of their probed entries, scanned
+# by several workgroups and merged, must give exactly the unsplit results.
+@pytest.mark.parametrize("target_items,chunk_min", [(0, 16384), (1024, 1), (65536, 1),
+                                                    (65536, 300)])
+@pytest.mark.parametrize("list_order,masked", [(True, False), (False, False), (True, True)])
+def test_split_plans_vs_oracle(torch, case, target_items, chunk_min, list_order, masked):
+    db, qs, t, idx, h = case
+    rng = np.random.default_rng(47)
+    ks = rng.choice([1, 5, 10, 64], size=qs.shape[0]).astype(np.uint32)
+    nps = rng.choice([1, 4, 16, 64], size=qs.shape[0]).astype(np.uint32)
+    mask = S.random_valid_mask(db.shape[0], 0.3, 48) if masked else None
+    tu = L.FvIvfPqTuning(target_items, chunk_min, 0.0, 0)
+    out = idx.search(dev(torch, qs), 64, 64, 200, top_k=dev(torch, ks.view(np.int32)),
+                     nprobe=dev(torch, nps.view(np.int32)), use_list_order=list_order,
+                     valid_mask=None if mask is None else dev(torch, mask), tuning=tu)
+    torch.cuda.synchronize()
+    exp = O.ivf_pq_search_uneven(h, qs, ks, nps, 200, out_stride=64, valid_mask=mask)
+    assert_same(L.split_results(out), exp)
+
+
+def test_autotune_plan_vs_oracle(torch, case):
+    db, qs, t, idx, h = case
+    best = idx.autotune(dev(torch, qs), 10, 32, 100, trials=1)
+    assert best.ms > 0.0
+    out = idx.search(dev(torch, qs), 10, 32, 100, tuning=best)
+    torch.cuda.synchronize()
+    ks = np.full(qs.shape[0], 10, np.uint32)
+    nps = np.full(qs.shape[0], 32, np.uint32)
+    exp = O.ivf_pq_search_uneven(h, qs, ks, nps, 100, out_stride=10)
+    assert_same(L.split_results(out), exp)
