@@ -1244,15 +1244,16 @@ static uint32_t deepest_shortlist(const FlatSearchArgs& a) {
 static uint32_t plan_ks(uint32_t padded_dim, int dtype) {
     if (dtype == FV_DTYPE_FP8_E4M3) {
         const uint32_t ks = (padded_dim + 127) / 128;
-        return ks <= 2 ? ks : (ks <= 4 ? 4 : (ks <= 6 ? 6 : 0));
+        if (ks <= 2) return ks;
+        for (uint32_t c : {4u, 6u, 8u, 12u, 16u, 24u})
+            if (ks <= c) return c;
+        return 0;
     }
     const uint32_t ks = (padded_dim + 31) / 32;
     if (ks <= 4) return ks;
-    if (ks <= 6) return 6;
-    if (ks <= 8) return 8;
-    if (ks <= 12) return 12;
-    if (ks <= 16) return 16;
-    return ks <= 24 ? 24 : 0;
+    for (uint32_t c : {6u, 8u, 12u, 16u, 24u, 32u, 48u})
+        if (ks <= c) return c;
+    return 0;
 }
 
 FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t nq,
@@ -1427,10 +1428,13 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
                  t->d_row_scales, Qh, qscale, nq, sa, rows, nrows, t->num_vectors, gthr, counts,
                  cand, minima, stream};
     hipError_t e = hipErrorInvalidValue;
-    const bool large = p.ks > 8;
-    if (t->dtype == FV_DTYPE_F16) e = large ? scan_f16_large(L) : scan_f16_small(L);
-    else if (t->dtype == FV_DTYPE_BF16) e = large ? scan_bf16_large(L) : scan_bf16_small(L);
-    else if (t->dtype == FV_DTYPE_FP8_E4M3) e = scan_fp8(L);
+    const bool large = p.ks > 8, wide = p.ks > 24;
+    if (t->dtype == FV_DTYPE_F16)
+        e = wide ? scan_f16_wide(L) : (large ? scan_f16_large(L) : scan_f16_small(L));
+    else if (t->dtype == FV_DTYPE_BF16)
+        e = wide ? scan_bf16_wide(L) : (large ? scan_bf16_large(L) : scan_bf16_small(L));
+    else if (t->dtype == FV_DTYPE_FP8_E4M3)
+        e = p.ks > 6 ? scan_fp8_wide(L) : scan_fp8(L);
     if (e != hipSuccess) return e;
     const float* X32 = t->d_vectors;
     if (p.tail) {
@@ -1474,13 +1478,19 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
     return hipGetLastError();
 }
 
+// Widest rows the matrix-core scan takes: 1536 dims of 16-bit codes, 3072
+// of FP8 (16 queries per wave at the widest).
+static uint32_t max_scan_dim(uint32_t dtype) {
+    return dtype == FV_DTYPE_FP8_E4M3 ? 3072u : 1536u;
+}
+
 static int check_tables(const FvFlatTables* t) {
     if (!t->d_codes || !t->d_norms_sq) return -1;
     if (t->dtype == FV_DTYPE_FP8_E4M3 && !t->d_row_scales) return -1;
     if (t->dtype != FV_DTYPE_F16 && t->dtype != FV_DTYPE_BF16 && t->dtype != FV_DTYPE_FP8_E4M3)
         return (int)hipErrorInvalidValue;
-    if (t->padded_dim == 0 || t->padded_dim % 16 || t->padded_dim > 768 || t->dim == 0 ||
-        t->dim > t->padded_dim)
+    if (t->padded_dim == 0 || t->padded_dim % 16 || t->dim == 0 || t->dim > t->padded_dim ||
+        t->padded_dim > max_scan_dim(t->dtype))
         return (int)hipErrorInvalidValue;
     return 0;
 }
@@ -1564,6 +1574,22 @@ int cuda_fp16_tc_search_batch(const Fp16GpuTables* tables, const float* d_querie
     if (num_queries == 0 || top_k == 0 || tables->num_vectors == 0 || tables->padded_dim == 0)
         return 0;  // ivf_quant_lookup.cu:1047-1049: silent no-op
     if (top_k > 256u) top_k = 256u;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (tables->padded_dim > fv::max_scan_dim(FV_DTYPE_F16) && tables->d_vectors &&
+        tables->dim > 0 && tables->dim <= tables->padded_dim) {
+        // Rows wider than the matrix-core scan takes: the exact FP32 scan of
+        // the FP32 rows, which returns what the exact rerank of a complete
+        // shortlist returns (the k best by exact distance, ties to the lower row).
+        const fv::FlatExactPlan ep = fv::plan_flat_exact(tables->num_vectors, tables->dim,
+                                                         num_queries, top_k, (int)metric);
+        fv::WorkspaceLease ws;
+        hipError_t e = fv::stream_workspace(s, ep.workspace_bytes, ws);
+        if (e != hipSuccess) return (int)e;
+        e = fv::launch_flat_exact(tables->d_vectors, nullptr, tables->num_vectors, tables->dim,
+                                  d_queries, num_queries, top_k, (int)metric, d_valid_mask,
+                                  d_out_results, top_k, ws.ptr, ep, s, false);
+        return e == hipSuccess ? 0 : (int)e;
+    }
     FvFlatTables t{};
     t.d_codes = tables->d_vectors_fp16;
     t.d_norms_sq = tables->d_vec_norms_sq;
@@ -1583,7 +1609,6 @@ int cuda_fp16_tc_search_batch(const Fp16GpuTables* tables, const float* d_querie
     sp.top_k = top_k;
     fv::FlatSearchArgs a;
     if (!fv::make_args(&t, &sp, a)) return (int)hipErrorInvalidValue;
-    hipStream_t s = static_cast<hipStream_t>(stream);
     fv::FlatMfmaPlan p = fv::plan_flat_mfma(t.num_vectors, t.padded_dim, FV_DTYPE_F16, num_queries,
                                             a, d_valid_mask != nullptr, sp.valid_count);
     if (p.lcap > 8192u || p.ks == 0) return (int)hipErrorInvalidValue;

@@ -117,8 +117,11 @@ void launch_bound_select(const ScanLaunch& L);
 // Scan launchers, one per dtype and K-step range (fv_flat_mfma_*.hip).
 hipError_t scan_f16_small(const ScanLaunch& L);   // ks in {1,2,3,4,6,8}
 hipError_t scan_f16_large(const ScanLaunch& L);   // ks in {12,16,24}
+hipError_t scan_f16_wide(const ScanLaunch& L);    // ks in {32,48}
 hipError_t scan_bf16_small(const ScanLaunch& L);
 hipError_t scan_bf16_large(const ScanLaunch& L);
+hipError_t scan_bf16_wide(const ScanLaunch& L);
 hipError_t scan_fp8(const ScanLaunch& L);         // ks in {1,2,4,6}
+hipError_t scan_fp8_wide(const ScanLaunch& L);    // ks in {8,12,16,24}
 
 }  // namespace fv

@@ -14,8 +14,8 @@
 //    v_mfma_f32_16x16x128_f8f6f4 accumulates code products from zero and the
 //    epilogue applies D = c_row + (s_row * s_query) * acc.
 //  * Masked and out-of-range rows get c_row = +inf and never pass a threshold.
-//  * A workgroup owns 4 waves x QW queries (QW = 64, or 32 when 64 queries'
-//    fragments would not fit the register file) and one chunk of the row
+//  * A workgroup owns 4 waves x QW queries (QW = 64, or 32 or 16 when more
+//    queries' fragments would not fit the register file) and one chunk of the row
 //    space. The queries' B fragments live in registers for the whole kernel;
 //    row tiles are double-buffered through LDS (rows padded by 16 bytes so the
 //    fragment reads are conflict-free), one barrier per tile.
@@ -93,13 +93,17 @@ struct ScanShape {
     static constexpr int kLaneBytes = MfmaOps<DT>::kLaneBytes;
     static constexpr int kStepBytes = 4 * kLaneBytes;  // bytes of one row per K step
     static constexpr int kRowBytes = KS * kStepBytes;
-    // Queries per wave: 64 while their fragments take at most 192 VGPRs.
-    static constexpr int kQW = 4 * KS * (kLaneBytes / 4) <= 192 ? 64 : 32;
+    // Queries per wave: 64, 32 or 16, the most whose fragments take at most
+    // 192 VGPRs (16 queries reach 1536 dims of 16-bit codes, 3072 of FP8).
+    static constexpr int kRegs16 = KS * (kLaneBytes / 4);  // VGPRs per 16 queries
+    static constexpr int kQW = 4 * kRegs16 <= 192 ? 64 : (2 * kRegs16 <= 192 ? 32 : 16);
     static constexpr int kQB = 4 * kQW;
     static constexpr int kBRegs = (kQW / 16) * KS * (kLaneBytes / 4);
     static constexpr int kOcc = kBRegs < 128 ? 2 : 1;
-    // Rows per LDS tile: 64, or 32 when two 64-row tiles would exceed 100 KB.
-    static constexpr int kRT = 2 * 64 * (kRowBytes + 16) <= 102400 ? 64 : 32;
+    // Rows per LDS tile: 64, 32 or 16, the most whose two tiles fit 100 KB.
+    static constexpr int kRT = 2 * 64 * (kRowBytes + 16) <= 102400
+                                   ? 64
+                                   : (2 * 32 * (kRowBytes + 16) <= 102400 ? 32 : 16);
     // Workgroup-prune instantiations (deep shortlists) need up to 82 KB of
     // prune scratch, so their double-buffered tiles stay within 50 KB.
     static constexpr int kRTDeep = 2 * 64 * (kRowBytes + 16) <= 51200
@@ -119,10 +123,20 @@ __device__ __forceinline__ void lds_dma16(const void* src, void* lds_base) {
                                      0, 0);
 }
 
-// Host-side mirror of ScanShape (queries per wave and rows per tile).
+// s_waitcnt immediate of the gfx9 family (gfx950): vmcnt in bits 3:0 and
+// 15:14, expcnt in bits 6:4, lgkmcnt in bits 11:8; a counter given its field's
+// maximum is not waited on.
+constexpr int waitcnt_imm(int vmcnt, int expcnt, int lgkmcnt) {
+    return (vmcnt & 0xF) | ((vmcnt >> 4) << 14) | ((expcnt & 0x7) << 4) | ((lgkmcnt & 0xF) << 8);
+}
+constexpr int kWaitVmcnt0 = waitcnt_imm(0, 7, 15);  // every vector-memory load retired
+static_assert(kWaitVmcnt0 == 0x0F70, "vmcnt(0) expcnt(7) lgkmcnt(15)");
+
+// Host-side mirror of ScanShape (queries per wave).
 inline int scan_qw(int dt, uint32_t ks) {
     const int lane = dt == FV_DTYPE_FP8_E4M3 ? 32 : 16;
-    return 4 * (int)ks * (lane / 4) <= 192 ? 64 : 32;
+    const int regs16 = (int)ks * (lane / 4);
+    return 4 * regs16 <= 192 ? 64 : (2 * regs16 <= 192 ? 32 : 16);
 }
 inline int scan_row_bytes(int dt, uint32_t ks) {
     return (int)ks * 4 * (dt == FV_DTYPE_FP8_E4M3 ? 32 : 16);
@@ -290,8 +304,8 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     }
     // Retire the fragment loads here: otherwise the wait-count tracking keeps
     // them "in flight" around the tile loop and every tile waits for its own
-    // prefetches (vmcnt(0)) before its last MMAs. 0x0F70 = vmcnt(0).
-    __builtin_amdgcn_s_waitcnt(0x0F70);
+    // prefetches (vmcnt(0)) before its last MMAs.
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     // Per-query state; lane l < QW of wave w owns local query w*QW + l.
     const bool owner = lane < QW;
     const uint32_t my_ql = (uint32_t)(wave * QW + lane);

@@ -53,7 +53,9 @@ int cuda_ivf_pq_search_batch(const IvfPqGpuTables* tables, const float* d_querie
 /* ---- FP16 matrix-core flat scan. Replaces cuda_bridge.h:138-147
  * (kernels/src/ivf_quant_lookup.cu:887-1058,1106-1123). top_k <= 256. When
  * tables->d_vectors is non-NULL, a shortlist of 8*top_k is reranked exactly
- * in FP32 (the reference never reads d_vectors and returns FP16 scores). */
+ * in FP32 (the reference never reads d_vectors and returns FP16 scores).
+ * Rows of up to 1536 padded dims are scored on the matrix cores; wider rows
+ * need tables->d_vectors and are scanned exactly in FP32 (same k best). */
 int cuda_fp16_tc_search_batch(const Fp16GpuTables* tables, const float* d_queries,
                               uint32_t num_queries, uint32_t top_k, MetricType metric,
                               const uint32_t* d_valid_mask, GpuSearchResult* d_out_results,

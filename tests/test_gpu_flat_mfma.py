@@ -197,3 +197,41 @@ def test_fp16_tc_entry_clamps_and_no_ops(torch):
     ids, d = L.split_results(F.fp16_tc_search_batch(d_codes, d_norms, d_q, 8, 64, "l2",
                                                     dev(torch, zero), d_db))
     assert (ids == 0xFFFFFFFF).all() and (d == np.float32(1e30)).all()
+
+
+@pytest.mark.parametrize("dim,metric", [(1024, "l2"), (1536, "l2"), (1000, "ip"), (1536, "cosine"),
+                                        (2048, "l2")])
+def test_fp16_tc_entry_wide_rows(torch, dim, metric):
+    """cuda_fp16_tc_search_batch on rows wider than 768 dims (the reference's
+    kernel takes any padded width): 16 queries per wave up to 1536 padded
+    dims, the exact FP32 scan beyond. Integer rows: every score is exact."""
+    rng = np.random.default_rng(dim)
+    n, nq = 3000, 40
+    db = rng.integers(-6, 7, size=(n, dim)).astype(np.float32)
+    qs = rng.integers(-6, 7, size=(nq, dim)).astype(np.float32)
+    codes, norms = O.encode_fp16_padded(db)
+    mask = S.random_valid_mask(n, 0.5, dim + 1)
+    for m in (None, mask):
+        out = F.fp16_tc_search_batch(dev(torch, codes.view(np.int16)), dev(torch, norms),
+                                     dev(torch, qs), 10, dim, metric,
+                                     None if m is None else dev(torch, m), dev(torch, db))
+        torch.cuda.synchronize()
+        assert_same(L.split_results(out), O.flat_topk(db, qs, 10, metric, valid_mask=m))
+
+
+@pytest.mark.parametrize("dtype,dim", [("f16", 1024), ("bf16", 1536), ("fp8", 1024),
+                                       ("fp8", 3072)])
+def test_wide_rows_flat_search(torch, dtype, dim):
+    """fv_flat_search on wide rows (16 or 32 queries per wave), k up to 100."""
+    rng = np.random.default_rng(dim + 7)
+    n, nq = 4000, 50
+    hi = 14 if dtype == "fp8" else 6
+    db = rng.integers(-hi, hi + 1, size=(n, dim)).astype(np.float32)
+    db[:, 0] = hi
+    qs = rng.integers(-hi, hi + 1, size=(nq, dim)).astype(np.float32)
+    qs[:, 1] = hi
+    idx = F.FlatIndex(dev(torch, db), dtype)
+    for k in (1, 100):
+        out = idx.search(dev(torch, qs), k, "ip")
+        torch.cuda.synchronize()
+        assert_same(L.split_results(out), O.flat_topk(db, qs, k, "ip"))
