@@ -325,6 +325,48 @@ __device__ __forceinline__ uint32_t wave_prune(uint64_t* __restrict__ buf, uint3
     return kept;
 }
 
+// Exact prune of one query's candidate buffer (count <= 64*R distinct keys,
+// global memory) by one wavefront in registers: bisection over the whole
+// 64-bit key finds the keep-th smallest key t (keys are distinct, so exactly
+// `keep` keys are at or below it, however many share its distance), then the
+// keys above t are dropped by an in-order compaction. Returns the new count;
+// *bound = the distance bits of t (0xFFFFFFFF when nothing was dropped).
+template <int R>
+__device__ __forceinline__ uint32_t wave_key_prune(uint64_t* __restrict__ buf, uint32_t count,
+                                                   uint32_t keep, int lane, uint32_t* bound) {
+    uint64_t v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t i = (uint32_t)(r * kWave + lane);
+        v[r] = i < count ? buf[i] : kEmptyKey;
+    }
+    if (count <= keep) {
+        *bound = 0xFFFFFFFFu;
+        return count;
+    }
+    uint64_t lo = 0u, hi = kEmptyKey - 1u;  // buffered keys are below kEmptyKey
+    while (lo < hi) {
+        const uint64_t mid = lo + ((hi - lo) >> 1);
+        uint32_t c = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) c += __popcll(__ballot(v[r] <= mid));
+        if (c >= keep) hi = mid;
+        else lo = mid + 1u;
+    }
+    uint32_t base = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const bool k = v[r] <= hi;
+        const uint64_t m = __ballot(k);
+        const uint32_t pos = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (k) buf[pos] = v[r];
+        base += __popcll(m);
+    }
+    *bound = (uint32_t)(hi >> 32);
+    return base;
+}
+
 // Prune of one query's candidate buffer (count <= 64*R keys in global memory)
 // by one wavefront without sorting: bisection over the ordered distance
 // finds a bound t with at least `keep` keys at or below it (ballot +

@@ -55,7 +55,43 @@ __global__ void norms_kernel(const float* __restrict__ x, uint32_t n, uint32_t d
     if (out64) out64[r] = b;
 }
 
+// |decode(code_row) - x_row| per row: FP64 sum of squared differences, its
+// square root rounded up to FP32 (a non-finite difference gives +inf).
+__global__ void encode_error_kernel(const float* __restrict__ x, const uint16_t* __restrict__ codes,
+                                    uint32_t n, uint32_t dim, uint32_t pdim, uint32_t dtype,
+                                    float* __restrict__ out) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const float* row = x + (size_t)r * dim;
+    const uint16_t* c = codes + (size_t)r * pdim;
+    double acc = 0.0;
+    for (uint32_t d = 0; d < dim; ++d) {
+        const float dec = dtype == FV_DTYPE_F16 ? (float)__builtin_bit_cast(_Float16, c[d])
+                                                : __uint_as_float((uint32_t)c[d] << 16);
+        const double diff = (double)dec - (double)row[d];
+        acc = __builtin_fma(diff, diff, acc);
+    }
+    float e = (float)sqrt(acc);
+    if (!(e < __builtin_inff())) e = __builtin_inff();
+    else if ((double)e < sqrt(acc)) e = __uint_as_float(__float_as_uint(e) + 1u);
+    out[r] = e;
+}
+
 }  // namespace fv
+
+extern "C" int fv_encode_error(const float* d_vectors, const void* d_codes, uint32_t num_vectors,
+                               uint32_t dim, uint32_t padded_dim, uint32_t dtype, float* d_err_out,
+                               void* stream) {
+    if (!d_vectors || !d_codes || !d_err_out) return -1;
+    if (dtype != FV_DTYPE_F16 && dtype != FV_DTYPE_BF16) return (int)hipErrorInvalidValue;
+    if (padded_dim < dim || dim == 0) return (int)hipErrorInvalidValue;
+    if (num_vectors == 0) return 0;
+    fv::encode_error_kernel<<<(num_vectors + 255) / 256, 256, 0, static_cast<hipStream_t>(stream)>>>(
+        d_vectors, static_cast<const uint16_t*>(d_codes), num_vectors, dim, padded_dim, dtype,
+        d_err_out);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : (int)e;
+}
 
 extern "C" int fv_encode_flat(const float* d_vectors, uint32_t num_vectors, uint32_t dim,
                               uint32_t padded_dim, uint32_t dtype, void* d_codes_out,

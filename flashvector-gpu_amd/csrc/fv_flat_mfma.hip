@@ -746,12 +746,163 @@ __device__ __forceinline__ void wave_key_ranks(const uint64_t* keys, uint32_t n,
     }
 }
 
+// The kth smallest (1 <= kth <= count) of distinct keys held NR per lane
+// (kEmptyKey past the list): bisection over the distance bits, then, when
+// several keys share that distance, over the row bits among them.
+template <int NR>
+__device__ __forceinline__ uint64_t wave_kth_key(const uint64_t (&v)[NR], uint32_t kth) {
+    uint32_t lo = 0u, hi = 0xFFFFFFFFu;
+    while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        uint32_t c = 0;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) c += (uint32_t)__popcll(__ballot((uint32_t)(v[r] >> 32) <= mid));
+        if (c >= kth) hi = mid;
+        else lo = mid + 1u;
+    }
+    const uint32_t dk = hi;
+    uint32_t cle = 0;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) cle += (uint32_t)__popcll(__ballot((uint32_t)(v[r] >> 32) <= dk));
+    if (cle == kth) return ((uint64_t)dk << 32) | 0xFFFFFFFFull;
+    uint32_t slo = 0u, shi = 0xFFFFFFFFu;
+    while (slo < shi) {
+        const uint32_t mid = slo + ((shi - slo) >> 1);
+        uint32_t c = 0;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const uint32_t d = (uint32_t)(v[r] >> 32);
+            c += (uint32_t)__popcll(__ballot(d < dk || (d == dk && (uint32_t)v[r] <= mid)));
+        }
+        if (c >= kth) shi = mid;
+        else slo = mid + 1u;
+    }
+    return ((uint64_t)dk << 32) | shi;
+}
+
+// Rounding of one query element as the scan's query encoder does it
+// (encode_queries_kernel), decoded back to FP32.
+__device__ __forceinline__ float coded_query_value(float v, uint32_t dt) {
+    if (dt == FV_DTYPE_F16) return (float)(_Float16)v;
+    uint32_t u = __float_as_uint(v);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return __uint_as_float(u & 0xFFFF0000u);
+}
+
+// Bounded rerank set of one query (one wavefront; keys[0..have) hold the
+// shortlist by scan score, have <= 256, k < have). For L2 the scan score is
+// s = 0.5|x|^2 - q~.x~ accumulated in FP32 from the FP32 norm (q~, x~ the
+// FP16/BF16 codes), so E = |q|^2 + 2s estimates the exact distance; for the
+// inner product E = s = -q~.x~. The value R the exact FP32 rerank returns
+// satisfies |R - E| <= B with
+//   B = c_m (|q~| e_x + e_q |x|) + 8 g (|q~| + |x~|)^2,   g = (1.25 d + 8) 2^-24,
+// c_m = 2 (L2) or 1 (IP): the first term bounds |q~.x~ - q.x| (e_x =
+// |x~ - x| from enc_err or, without it, half an ulp per element, 2^-11 |x|
+// (FP16; 2^-8 BF16) plus 2^-24 sqrt(d) for truncated FP16 subnormals: codes
+// rounded to nearest, as the encoders here and the reference's encode_fp16_padded
+// produce them; e_q = |q~ - q| computed here), the
+// second, with a factor-of-8 margin, the FP32 rounding of the MFMA
+// accumulation, of the norms and of the reference-order sums of the rerank
+// (each at most g times (|q| + |x|)^2). With U the k-th smallest E + B, a row
+// with E - B > U has k rows with R <= U < its own R, so it is not among the k
+// best. Those rows get keys above every admitted one; the others are moved to
+// keys[0..m) and m (>= k) is returned. Non-finite scores are always reranked.
+template <int METRIC>
+__device__ __noinline__ uint32_t bounded_rerank_set(uint64_t* keys, uint32_t have, uint32_t k,
+                                                    const float* __restrict__ qv, uint32_t dim,
+                                                    const float* __restrict__ norms,
+                                                    const float* __restrict__ enc_err,
+                                                    uint32_t code_dtype, uint32_t lane) {
+    double nq2 = 0.0, eq2 = 0.0;
+    for (uint32_t d = lane; d < dim; d += kWave) {
+        const float v = qv[d];
+        const double e = (double)coded_query_value(-v, code_dtype) + (double)v;
+        nq2 = __builtin_fma((double)v, (double)v, nq2);
+        eq2 = __builtin_fma(e, e, eq2);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        nq2 += __shfl_xor(nq2, off);
+        eq2 += __shfl_xor(eq2, off);
+    }
+    const double slack = 1.0 + 0x1p-40;
+    const double qn = sqrt(nq2) * slack, eq = sqrt(eq2) * slack, qtn = qn + eq;
+    const double g8 = 8.0 * (1.25 * (double)dim + 8.0) * 0x1p-24;
+    const double cm = METRIC == METRIC_L2 ? 2.0 : 1.0;
+    const double u_rel = (code_dtype == FV_DTYPE_F16 ? 0x1p-11 : 0x1p-8) * (1.0 + 0x1p-10);
+    const double u_abs = code_dtype == FV_DTYPE_F16 ? 0x1p-24 * sqrt((double)dim) * (1.0 + 0x1p-10) : 0.0;
+    constexpr int NR = 4;  // have <= 256 = NR * 64
+    uint64_t kv[NR];
+    uint32_t hi_o[NR], lo_o[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const uint32_t i = (uint32_t)r * kWave + lane;
+        kv[r] = i < have ? keys[i] : kEmptyKey;
+        hi_o[r] = 0xFFFFFFFFu;  // past the list: never counted, never reranked
+        lo_o[r] = 0xFFFFFFFFu;
+        if (i < have) {
+            const uint32_t row = key_slot(kv[r]);
+            const double sc = (double)key_dist(kv[r]);
+            const double E = METRIC == METRIC_L2 ? nq2 + 2.0 * sc : sc;
+            const double xn = sqrt((double)norms[row]) * slack;
+            const double ex = enc_err ? (double)enc_err[row] : u_rel * xn + u_abs;
+            const double xtn = xn + ex;
+            const double B = cm * (qtn * ex + eq * xn) + g8 * (qtn + xtn) * (qtn + xtn);
+            const double hi = E + B, lo = E - B;
+            if (key_unadmitted(kv[r]) || !(hi < 1e300) || !(lo > -1e300) || !(B < 1e300)) {
+                hi_o[r] = 0xFFFFFFFEu;  // counted as unbounded, always reranked
+                lo_o[r] = 0u;
+            } else {
+                // FP32 rounded outward, as ordered bits.
+                float hf = (float)hi, lf = (float)lo;
+                if ((double)hf < hi) hf = next_up(hf);
+                if ((double)lf > lo) lf = -next_up(-lf);
+                hi_o[r] = ord_f32(hf);
+                lo_o[r] = ord_f32(lf);
+            }
+        }
+    }
+    // U: the smallest ordered value with at least k upper ends at or below it.
+    uint32_t blo = 0u, bhi = 0xFFFFFFFEu;
+    while (blo < bhi) {
+        const uint32_t mid = blo + ((bhi - blo) >> 1);
+        uint32_t c = 0;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) c += (uint32_t)__popcll(__ballot(hi_o[r] <= mid));
+        if (c >= k) bhi = mid;
+        else blo = mid + 1u;
+    }
+    const uint32_t U = bhi;
+    wave_lds_sync();
+    // Rows to rerank first (in order), the others after them with keys above
+    // every admitted key.
+    uint32_t m = 0;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) m += (uint32_t)__popcll(__ballot(lo_o[r] <= U));
+    uint32_t pa = 0, pb = m;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const bool live = (uint32_t)r * kWave + lane < have;
+        const bool re = live && lo_o[r] <= U;
+        const bool out = live && !re;
+        const uint64_t ba = __ballot(re), bb = __ballot(out);
+        const uint32_t ra = __builtin_amdgcn_mbcnt_hi((uint32_t)(ba >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ba, 0u));
+        const uint32_t rb = __builtin_amdgcn_mbcnt_hi((uint32_t)(bb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bb, 0u));
+        if (re) keys[pa + ra] = kv[r];
+        if (out) keys[pb + rb] = unadmitted_key(key_slot(kv[r]));
+        pa += (uint32_t)__popcll(ba);
+        pb += (uint32_t)__popcll(bb);
+    }
+    wave_lds_sync();
+    return m;
+}
+
 template <int METRIC>
 __global__ __launch_bounds__(256) void flat_tail_kernel(
     const uint64_t* __restrict__ cand, const uint32_t* __restrict__ counts, uint32_t n_chunks,
     uint32_t nq, uint32_t cap, const uint32_t* __restrict__ gthr, FlatSearchArgs a, uint32_t lw,
     uint32_t hw, uint32_t wbytes, const float* __restrict__ X32, const float* __restrict__ Qm,
-    GpuSearchResult* __restrict__ out) {
+    GpuSearchResult* __restrict__ out, const float* __restrict__ norms,
+    const float* __restrict__ enc_err, uint32_t code_dtype) {
     extern __shared__ uint64_t s_mem[];
     const uint32_t lane = threadIdx.x & (kWave - 1);
     // Wave-uniform, so the query row is read with scalar loads.
@@ -823,14 +974,26 @@ __global__ __launch_bounds__(256) void flat_tail_kernel(
     wave_lds_sync();
     if (n > S) {
         if (n <= (uint32_t)(kRankSlots * kWave)) {
-            // The S smallest by rank, written in order to keys[0..S).
+            // The S smallest (keys are distinct): the S-th smallest key by
+            // register bisection, then the keys at or below it, compacted.
             uint64_t mk[kRankSlots];
-            uint32_t rk[kRankSlots];
-            wave_key_ranks(keys, n, lane, mk, rk);
-            wave_lds_sync();
 #pragma unroll
-            for (int s2 = 0; s2 < kRankSlots; ++s2)
-                if (rk[s2] < S) keys[rk[s2]] = mk[s2];
+            for (int s2 = 0; s2 < kRankSlots; ++s2) {
+                const uint32_t i = (uint32_t)s2 * kWave + lane;
+                mk[s2] = i < n ? keys[i] : kEmptyKey;
+            }
+            const uint64_t t = wave_kth_key<kRankSlots>(mk, S);
+            wave_lds_sync();
+            uint32_t base = 0;
+#pragma unroll
+            for (int s2 = 0; s2 < kRankSlots; ++s2) {
+                const bool kp = mk[s2] <= t;
+                const uint64_t bm = __ballot(kp);
+                if (kp)
+                    keys[base + __builtin_amdgcn_mbcnt_hi((uint32_t)(bm >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u))] = mk[s2];
+                base += (uint32_t)__popcll(bm);
+            }
             n = S;
             wave_lds_sync();
         } else {
@@ -841,23 +1004,29 @@ __global__ __launch_bounds__(256) void flat_tail_kernel(
     const uint32_t dim = a.dim;
     const float* qv = Qm + (size_t)q * dim;
 
+    // Bounded rerank: rows that cannot be among the k best are not reranked.
+    uint32_t nre = have;  // rows to rerank, first in keys[]
+    if (METRIC != METRIC_COSINE && X32 && code_dtype && norms &&
+        !(a.flags & (FV_SEARCH_F64_BOUNDARY | FV_SEARCH_FULL_RERANK)) && have > k)
+        nre = bounded_rerank_set<METRIC>(keys, have, k, qv, dim, norms, enc_err, code_dtype, lane);
+
     // Exact rerank in place: keys[i] becomes (distance, row), errs[i] its bound.
     if (X32) {
         const bool vec = (dim & 3u) == 0u && (reinterpret_cast<uintptr_t>(X32) & 15u) == 0u &&
                          (reinterpret_cast<uintptr_t>(Qm) & 15u) == 0u;
-        for (uint32_t i0 = 0; i0 < have; i0 += 2u * kWave) {
+        for (uint32_t i0 = 0; i0 < nre; i0 += 2u * kWave) {
             const uint32_t ia = i0 + lane, ib = i0 + kWave + lane;
             // Lanes past the list take a valid row; their results are dropped.
-            const uint32_t ra = ia < have ? key_slot(keys[ia]) : key_slot(keys[0]);
-            const uint32_t rb = ib < have ? key_slot(keys[ib]) : ra;
+            const uint32_t ra = ia < nre ? key_slot(keys[ia]) : key_slot(keys[0]);
+            const uint32_t rb = ib < nre ? key_slot(keys[ib]) : ra;
             float d[2], e[2];
-            if (i0 + kWave < have) {
+            if (i0 + kWave < nre) {
                 const uint32_t rows[2] = {ra, rb};
-                const bool live[2] = {ia < have, ib < have};
+                const bool live[2] = {ia < nre, ib < nre};
                 rerank_rows<METRIC, 2>(X32, qv, dim, vec, rows, live, d, e);
             } else {
                 const uint32_t rows[1] = {ra};
-                const bool live[1] = {ia < have};
+                const bool live[1] = {ia < nre};
                 float d1[1], e1[1];
                 rerank_rows<METRIC, 1>(X32, qv, dim, vec, rows, live, d1, e1);
                 d[0] = d1[0];
@@ -865,11 +1034,11 @@ __global__ __launch_bounds__(256) void flat_tail_kernel(
                 d[1] = 0.0f;
                 e[1] = 0.0f;
             }
-            if (ia < have) {
+            if (ia < nre) {
                 keys[ia] = d[0] < kAdmit ? make_key(d[0], ra) : unadmitted_key(ra);
                 errs[ia] = e[0];
             }
-            if (ib < have) {
+            if (ib < nre) {
                 keys[ib] = d[1] < kAdmit ? make_key(d[1], rb) : unadmitted_key(rb);
                 errs[ib] = e[1];
             }
@@ -887,9 +1056,34 @@ __global__ __launch_bounds__(256) void flat_tail_kernel(
     if (!((a.flags & FV_SEARCH_F64_BOUNDARY) && X32 != nullptr && have > 1)) {
         // Without the FP64 boundary only the first k in (distance, row)
         // order are needed: each lane writes its keys of rank < k.
+        // Rows left out of the bounded rerank rank after the nre reranked
+        // ones (nre >= k), so the first k are among those: the k smallest
+        // (bisection to the k-th key), then their ranks.
         uint64_t mk[kRankSlots];
         uint32_t rk[kRankSlots];
-        wave_key_ranks(keys, have, lane, mk, rk);
+        uint32_t nr = nre;
+        if (nre > 64u && nre > 2u * kk_out && kk_out > 0 && nre <= (uint32_t)(kRankSlots * kWave)) {
+#pragma unroll
+            for (int s2 = 0; s2 < kRankSlots; ++s2) {
+                const uint32_t i = (uint32_t)s2 * kWave + lane;
+                mk[s2] = i < nre ? keys[i] : kEmptyKey;
+            }
+            const uint64_t t = wave_kth_key<kRankSlots>(mk, kk_out);
+            wave_lds_sync();
+            uint32_t base = 0;
+#pragma unroll
+            for (int s2 = 0; s2 < kRankSlots; ++s2) {
+                const bool kp = mk[s2] <= t;
+                const uint64_t bm = __ballot(kp);
+                if (kp)
+                    keys[base + __builtin_amdgcn_mbcnt_hi((uint32_t)(bm >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u))] = mk[s2];
+                base += (uint32_t)__popcll(bm);
+            }
+            nr = kk_out;
+            wave_lds_sync();
+        }
+        wave_key_ranks(keys, nr, lane, mk, rk);
 #pragma unroll
         for (int s2 = 0; s2 < kRankSlots; ++s2) {
             if (rk[s2] < kk_out) {
@@ -1274,10 +1468,8 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
     p.block_prune = p.prune_regs == 0;
     // Per wave: prune scratch and staging list share one region; workgroup
     // prunes use one shared scratch followed by four staging lists.
-    static_assert(kRecBytes <= 1024u * 8u, "records must fit a wave's 1024-key prune scratch");
-    const size_t stage_bytes = (size_t)kRecBytes;
     p.lds_scan = p.block_prune ? (size_t)kScanSelBytes + 4 * (size_t)kRecBytes
-                               : 4 * ((size_t)p.cap * 8 > stage_bytes ? (size_t)p.cap * 8 : stage_bytes);
+                               : 4 * (size_t)kRecBytes;
     // Shallow shortlists: one fused wavefront per query after the scan.
     p.tail = smax <= kTailMaxS;
     p.tail_lw = next_pow2(smax + kWave);
@@ -1440,15 +1632,20 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
     if (p.tail) {
         const dim3 tg((nq + kTailWaves - 1) / kTailWaves);
         const unsigned tb = kTailWaves * kWave;
+        // Bounded rerank for 16-bit codes (scores in distance units).
+        const uint32_t cdt = (t->dtype == FV_DTYPE_F16 || t->dtype == FV_DTYPE_BF16) ? t->dtype : 0u;
         if (a.metric == METRIC_L2)
             flat_tail_kernel<METRIC_L2><<<tg, tb, p.lds_tail, stream>>>(
-                cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.tail_lw, p.tail_hw, p.tail_wbytes, X32, Qm, out);
+                cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.tail_lw, p.tail_hw, p.tail_wbytes, X32, Qm, out,
+                t->d_norms_sq, t->d_enc_err, cdt);
         else if (a.metric == METRIC_INNER_PRODUCT)
             flat_tail_kernel<METRIC_INNER_PRODUCT><<<tg, tb, p.lds_tail, stream>>>(
-                cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.tail_lw, p.tail_hw, p.tail_wbytes, X32, Qm, out);
+                cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.tail_lw, p.tail_hw, p.tail_wbytes, X32, Qm, out,
+                t->d_norms_sq, t->d_enc_err, cdt);
         else
             flat_tail_kernel<METRIC_COSINE><<<tg, tb, p.lds_tail, stream>>>(
-                cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.tail_lw, p.tail_hw, p.tail_wbytes, X32, Qm, out);
+                cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.tail_lw, p.tail_hw, p.tail_wbytes, X32, Qm, out,
+                t->d_norms_sq, t->d_enc_err, 0u);
         return hipGetLastError();
     }
     uint64_t* sel = reinterpret_cast<uint64_t*>(ws + p.off_sel);
@@ -1508,7 +1705,9 @@ static bool make_args(const FvFlatTables* t, const FvFlatSearchParams* sp, FlatS
     a.n_chunks = sp->n_chunks;
     a.bound_div = sp->bound_div;
     if (a.top_k == 0 || a.top_k > FV_MAX_TOPK) return false;
-    if (sp->metric > METRIC_INNER_PRODUCT || (sp->flags & ~FV_SEARCH_F64_BOUNDARY) != 0) return false;
+    if (sp->metric > METRIC_INNER_PRODUCT ||
+        (sp->flags & ~(FV_SEARCH_F64_BOUNDARY | FV_SEARCH_FULL_RERANK)) != 0)
+        return false;
     a.s_max = deepest_shortlist(a);
     return true;
 }

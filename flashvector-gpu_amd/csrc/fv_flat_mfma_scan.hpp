@@ -49,7 +49,7 @@ constexpr int kMfmaThreads = 256;
 // Staged candidate records per wave: a lane whose 4 scores for one query
 // column have a minimum under the threshold writes them (16 B) and their
 // position (4 B); keys are built when the records are flushed.
-constexpr uint32_t kRec = 256;
+constexpr uint32_t kRec = 128;
 constexpr uint32_t kRecBytes = kRec * 20;
 // Workgroup-prune instantiations: the radix selection's LDS block, then the
 // four waves' record staging (16-byte aligned).
@@ -100,7 +100,8 @@ struct ScanShape {
     static constexpr int kQB = 4 * kQW;
     static constexpr int kBRegs = (kQW / 16) * KS * (kLaneBytes / 4);
     static constexpr int kOcc = kBRegs < 128 ? 2 : 1;
-    // Rows per LDS tile: 64, 32 or 16, the most whose two tiles fit 100 KB.
+    // Rows per LDS tile: 64, 32 or 16, the most whose two tiles fit 100 KB
+    // (taller tiles beat a second workgroup per CU: measured at 768 dims).
     static constexpr int kRT = 2 * 64 * (kRowBytes + 16) <= 102400
                                    ? 64
                                    : (2 * 32 * (kRowBytes + 16) <= 102400 ? 32 : 16);
@@ -285,7 +286,8 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     const uint32_t qb0 = qblock * QB;
     const uint32_t qw0 = qb0 + wave * QW;
     constexpr bool kBlockPrune = R == 0;
-    uint64_t* my_scratch = s_scratch + (size_t)wave * cap;  // wave-prune instantiations only
+    // Wave-prune instantiations: each wave's record staging.
+    uint64_t* my_scratch = s_scratch + (size_t)wave * (kRecBytes / 8);
     BlockSelectLds& sel = *reinterpret_cast<BlockSelectLds*>(s_scratch);  // workgroup prunes
 
     // B fragments: lane (l16, grp) holds query qw0 + 16j + l16, bytes
@@ -470,6 +472,21 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
         constexpr int buf = decltype(bc)::value;
         using Next = std::integral_constant<int, buf ^ 1>;
         const uint32_t p0 = r_begin + t * RT;
+        // This tile's thresholds, C inputs and scales go to registers before
+        // the next tile's loads are issued: an LDS read after a direct-to-LDS
+        // load that the compiler cannot tell apart from its destination waits
+        // for that load, which would serialize the prefetch.
+        float thr[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) thr[j] = s_thr[wave * QW + 16 * j + l16];
+        constexpr int NS = RT / 16;
+        floatx4 cin_t[NS], scl_t[NS];
+#pragma unroll
+        for (int rt = 0; rt < NS; ++rt) {
+            cin_t[rt] = *reinterpret_cast<const floatx4*>(&s_cinit[buf][rt * 16 + 4 * grp]);
+            if (kFP8 || METRIC == METRIC_COSINE)
+                scl_t[rt] = *reinterpret_cast<const floatx4*>(&s_scale[buf][rt * 16 + 4 * grp]);
+        }
         if (t + 1 < ntiles) {
             load_tile(p0 + RT, Next{});
             if (t + 2 < ntiles) fetch_rid(p0 + 2 * RT);
@@ -477,10 +494,6 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
         // Prefetch the shared bound; it is consumed after this tile's MMAs.
         const uint32_t gpre =
             __hip_atomic_load(&gthr[my_live ? my_q : 0u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-
-        float thr[NJ];
-#pragma unroll
-        for (int j = 0; j < NJ; ++j) thr[j] = s_thr[wave * QW + 16 * j + l16];
 
         // Row tile rt (16 rows): MMAs into acc; scores after the FP8 / cosine
         // scaling; then selection. The MMAs of tile rt+1 are issued before
@@ -512,7 +525,7 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
 #pragma unroll
                 for (int s = 0; s < KS; ++s) afrag[s] = ld_vec<vec>(arow + LB * s);
             }
-            const floatx4 cin = *reinterpret_cast<const floatx4*>(&s_cinit[buf][rt * 16 + 4 * grp]);
+            const floatx4 cin = cin_t[rt];
 #pragma unroll
             for (int j = 0; j < NJ; ++j) acc[j] = kFP8 ? floatx4{0.0f, 0.0f, 0.0f, 0.0f} : cin;
             // NJ independent accumulation chains, interleaved.
@@ -523,8 +536,8 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
         };
         auto select_tile = [&](int rt, floatx4* acc) {
             if (kFP8 || METRIC == METRIC_COSINE) {
-                const floatx4 scl = *reinterpret_cast<const floatx4*>(&s_scale[buf][rt * 16 + 4 * grp]);
-                const floatx4 cin = *reinterpret_cast<const floatx4*>(&s_cinit[buf][rt * 16 + 4 * grp]);
+                const floatx4 scl = scl_t[rt];
+                const floatx4 cin = cin_t[rt];
 #pragma unroll
                 for (int j = 0; j < NJ; ++j) {
                     if (kFP8) acc[j] = cin + acc[j] * (scl * qsc[j]);
@@ -593,8 +606,8 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
                 uint32_t kept = wave_bisect_prune<(R > 0 ? R : 1)>(bq, s_cnt[qlb], keep, lane, &th);
                 if (kept + pk * RT > cap) {
                     // Heavy ties at the bound: exact (distance, row) selection.
-                    kept = wave_prune(bq, kept, keep, my_scratch, lane);
-                    th = kept == keep ? (uint32_t)(my_scratch[keep - 1] >> 32) : 0xFFFFFFFFu;
+                    wave_lds_sync();
+                    kept = wave_key_prune<(R > 0 ? R : 1)>(bq, kept, keep, lane, &th);
                 }
                 if (lane == 0) {
                     s_cnt[qlb] = kept;

@@ -129,7 +129,7 @@ struct FvEngine {
     int mode = 0;  // bits: kFlat (flat matrix-core tables), kIvf (IVF-PQ tables)
     uint32_t n_main = 0;
     // flat tables
-    fvh::DevArray<float> rows, norms, scales;
+    fvh::DevArray<float> rows, norms, scales, enc_err;
     fvh::DevArray<double> norms64;
     fvh::DevArray<uint8_t> codes;
     FvFlatTables flat{};
@@ -491,18 +491,23 @@ static int attach_flat(FvEngine* e, uint32_t dtype) {
         return (int)hipErrorInvalidValue;
     const uint32_t step = dtype == FV_DTYPE_FP8_E4M3 ? 128u : 32u;
     const uint32_t pd = (dim + step - 1) / step * step;
-    if (pd > 768) return (int)hipErrorInvalidValue;
-    const size_t esz = dtype == FV_DTYPE_FP8_E4M3 ? 1 : 2;
+    if (pd > (dtype == FV_DTYPE_FP8_E4M3 ? 3072u : 1536u)) return (int)hipErrorInvalidValue;
+    const bool fp8 = dtype == FV_DTYPE_FP8_E4M3;
+    const size_t esz = fp8 ? 1 : 2;
     hipError_t he;
     if ((he = e->codes.alloc((size_t)n * pd * esz)) != hipSuccess ||
         (he = e->norms.alloc(n)) != hipSuccess || (he = e->norms64.alloc(n)) != hipSuccess ||
-        (he = e->scales.alloc(dtype == FV_DTYPE_FP8_E4M3 ? n : 0)) != hipSuccess)
+        (he = e->scales.alloc(fp8 ? n : 0)) != hipSuccess ||
+        (he = e->enc_err.alloc(fp8 ? 0 : n)) != hipSuccess)
         return (int)he;
     int rc = fv_encode_flat(e->rows.p, n, dim, pd, dtype, e->codes.p, e->norms.p, e->norms64.p,
                             e->scales.p, nullptr);
+    if (rc == 0 && !fp8)
+        rc = fv_encode_error(e->rows.p, e->codes.p, n, dim, pd, dtype, e->enc_err.p, nullptr);
     if (rc) return rc;
     if ((he = hipDeviceSynchronize()) != hipSuccess) return (int)he;
-    e->flat = FvFlatTables{e->codes.p, e->scales.p, e->norms.p, e->norms64.p, e->rows.p, n, dim, pd, dtype};
+    e->flat = FvFlatTables{e->codes.p, e->scales.p, e->norms.p, e->norms64.p, e->rows.p, n, dim, pd,
+                           dtype, fp8 ? nullptr : e->enc_err.p};
     e->mode |= kFlat;
     return 0;
 }

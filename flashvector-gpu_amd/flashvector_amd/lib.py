@@ -65,6 +65,7 @@ class FvFlatTables(ctypes.Structure):
         ("dim", ctypes.c_uint32),
         ("padded_dim", ctypes.c_uint32),
         ("dtype", ctypes.c_uint32),
+        ("d_enc_err", ctypes.c_void_p),
     ]
 
 
@@ -185,6 +186,7 @@ SIGNATURES = {
     "fv_flat_exact_search": (I, [P, U, U, P, U, U, I, P, P, P, SZ, P]),
     "cuda_fp16_tc_search_batch": (I, [P, P, U, U, I, P, P, P]),
     "fv_encode_flat": (I, [P, U, U, U, U, P, P, P, P, P]),
+    "fv_encode_error": (I, [P, P, U, U, U, U, P, P]),
     "fv_flat_workspace_size": (SZ, [P, U, P]),
     "fv_flat_search": (I, [P, P, U, P, P, P, SZ, P]),
     "cuda_ivf_pq_search_batch": (I, [P, P, U, U, U, U, I, P, P, P]),

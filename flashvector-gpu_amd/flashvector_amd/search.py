@@ -93,6 +93,7 @@ def device_count() -> int:
 
 
 FV_SEARCH_F64_BOUNDARY = 1
+FV_SEARCH_FULL_RERANK = 2
 # valid_count value: compact the mask with its popcount taken on the device.
 VALID_COUNT_DEVICE = 0xFFFFFFFF
 
@@ -125,14 +126,22 @@ class FlatIndex:
                                    L.ptr(self.codes), L.ptr(self.norms), L.ptr(self.norms64),
                                    L.ptr(self.row_scales), L.stream_handle()), "fv_encode_flat")
         self.keep_fp32 = keep_fp32
+        # Per-row encoding error of 16-bit codes: tightens the bounded rerank.
+        self.enc_err = None
+        if not fp8:
+            self.enc_err = torch.empty(n, dtype=torch.float32, device=dev)
+            L.check(lib.fv_encode_error(L.ptr(self.vectors), L.ptr(self.codes), n, dim, pd,
+                                        L.DTYPES[dtype], L.ptr(self.enc_err), L.stream_handle()),
+                    "fv_encode_error")
         self.tables = L.FvFlatTables(L.ptr(self.codes), L.ptr(self.row_scales), L.ptr(self.norms),
                                      L.ptr(self.norms64),
                                      L.ptr(self.vectors) if keep_fp32 else None,
-                                     n, dim, pd, L.DTYPES[dtype])
+                                     n, dim, pd, L.DTYPES[dtype], L.ptr(self.enc_err))
         self._ws = None
 
     def params(self, top_k, metric="l2", valid_mask=None, valid_count=0, shortlist=0,
-               shortlist_factor=0, top_k_per_query=None, f64_boundary=False, tuning=None):
+               shortlist_factor=0, top_k_per_query=None, f64_boundary=False, tuning=None,
+               full_rerank=False):
         """FvFlatSearchParams; `tuning` (an FvFlatTuning, e.g. from autotune)
         supplies the shortlist factor and plan overrides when the caller
         gives no shortlist of its own."""
@@ -144,7 +153,8 @@ class FlatIndex:
         return L.FvFlatSearchParams(L.ptr(valid_mask), L.ptr(top_k_per_query), int(valid_count),
                                     _metric(metric), int(top_k), int(shortlist_factor),
                                     int(shortlist),
-                                    FV_SEARCH_F64_BOUNDARY if f64_boundary else 0,
+                                    (FV_SEARCH_F64_BOUNDARY if f64_boundary else 0)
+                                    | (FV_SEARCH_FULL_RERANK if full_rerank else 0),
                                     n_chunks, bound_div)
 
     def autotune(self, queries, top_k, metric="l2", valid_mask=None, valid_count=0,
@@ -166,16 +176,16 @@ class FlatIndex:
 
     def search(self, queries, top_k, metric="l2", valid_mask=None, shortlist=0, out=None,
                valid_count=0, shortlist_factor=0, top_k_per_query=None, f64_boundary=False,
-               tuning=None):
+               tuning=None, full_rerank=False):
         """fv_flat_search. `valid_count` (set bits of the mask, if known) lets
         sparse masks be compacted; `top_k_per_query` is an optional int32
         device tensor of per-query k (each <= top_k); `f64_boundary` sets
-        FV_SEARCH_F64_BOUNDARY."""
+        FV_SEARCH_F64_BOUNDARY, `full_rerank` FV_SEARCH_FULL_RERANK."""
         torch = L._torch()
         lib = L.load()
         nq = queries.shape[0]
         prm = self.params(top_k, metric, valid_mask, valid_count, shortlist, shortlist_factor,
-                          top_k_per_query, f64_boundary, tuning)
+                          top_k_per_query, f64_boundary, tuning, full_rerank)
         need = self.workspace_size(nq, prm)
         if need == 0:
             raise L.FvError("invalid flat search shape")

@@ -55,7 +55,10 @@ int cuda_ivf_pq_search_batch(const IvfPqGpuTables* tables, const float* d_querie
  * tables->d_vectors is non-NULL, a shortlist of 8*top_k is reranked exactly
  * in FP32 (the reference never reads d_vectors and returns FP16 scores).
  * Rows of up to 1536 padded dims are scored on the matrix cores; wider rows
- * need tables->d_vectors and are scanned exactly in FP32 (same k best). */
+ * need tables->d_vectors and are scanned exactly in FP32 (same k best).
+ * The codes must be the binary16 encoding of d_vectors rounded to nearest
+ * (crates/engine/src/quant.rs encode_fp16_padded): the exact rerank then
+ * skips rows that this rounding bound proves cannot be among the k best. */
 int cuda_fp16_tc_search_batch(const Fp16GpuTables* tables, const float* d_queries,
                               uint32_t num_queries, uint32_t top_k, MetricType metric,
                               const uint32_t* d_valid_mask, GpuSearchResult* d_out_results,

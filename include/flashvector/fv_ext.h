@@ -74,6 +74,8 @@ typedef struct {
     uint32_t dim;
     uint32_t padded_dim;
     uint32_t dtype;               /* FvDtype */
+    const float* d_enc_err;       /* [num_vectors] |decode(code) - x| of FP16/BF16 rows
+                                     (fv_encode_error), or NULL (a bound from |x| is used) */
 } FvFlatTables;
 
 /* Encodes FP32 rows into FvFlatTables storage on the device: codes (padded
@@ -83,6 +85,11 @@ int fv_encode_flat(const float* d_vectors, uint32_t num_vectors, uint32_t dim,
                    uint32_t padded_dim, uint32_t dtype, void* d_codes_out,
                    float* d_norms_out, double* d_norms_f64_out, float* d_row_scales_out,
                    void* stream);
+/* Per-row Euclidean norm of the encoding error of FP16/BF16 codes,
+ * |decode(code_row) - x_row|, rounded up (FP64 sums), for the bounded rerank
+ * of fv_flat_search. Other dtypes: hipErrorInvalidValue. */
+int fv_encode_error(const float* d_vectors, const void* d_codes, uint32_t num_vectors, uint32_t dim,
+                    uint32_t padded_dim, uint32_t dtype, float* d_err_out, void* stream);
 
 /* Search parameters of the flat scans. Shortlist of query i: factor * k_i
  * when shortlist_factor != 0, else `shortlist` (0 means 8 * k_i), never below
@@ -96,8 +103,15 @@ int fv_encode_flat(const float* d_vectors, uint32_t num_vectors, uint32_t dim,
  * distances are within twice their rounding-error bound of a neighbour are
  * re-scored in FP64 and ordered by (FP64 distance, row), so membership and
  * order of the returned top k follow the exact distances; those rows report
- * the FP64 distance rounded to FP32. */
+ * the FP64 distance rounded to FP32.
+ * Without it, L2 and inner-product searches over FP16/BF16 codes rerank only
+ * the shortlisted rows whose exact-distance interval (scan score +- a
+ * rigorous bound from the codes' rounding error, d_enc_err when given, and
+ * FP32 accumulation) reaches the k-th smallest upper end: no other row can
+ * be among the k best, so the results equal those of reranking every row.
+ * FV_SEARCH_FULL_RERANK turns that off (every shortlisted row is reranked). */
 #define FV_SEARCH_F64_BOUNDARY 1u
+#define FV_SEARCH_FULL_RERANK 2u
 #define FV_VALID_COUNT_DEVICE 0xFFFFFFFFu
 typedef struct {
     const uint32_t* d_valid_mask; /* LSB-first u32 words, NULL = every row */

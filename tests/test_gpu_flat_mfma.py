@@ -235,3 +235,70 @@ def test_wide_rows_flat_search(torch, dtype, dim):
         out = idx.search(dev(torch, qs), k, "ip")
         torch.cuda.synchronize()
         assert_same(L.split_results(out), O.flat_topk(db, qs, k, "ip"))
+
+
+@pytest.mark.parametrize("dtype,k", [("f16", 1), ("f16", 10), ("f16", 64), ("bf16", 10),
+                                     ("fp8", 10)])
+def test_heavy_ties_exact_prune(torch, dtype, k):
+    """Rows copied from 12 distinct integer vectors: thousands of rows share
+    each distance, so the scan's buffers overflow with keys tied at the bound
+    and are cut by exact (distance, row) selection. Ties go to the lower row.
+    Values are exact in every storage type (FP8: row maximum 14, so each
+    code is an integer multiple of 32)."""
+    rng = np.random.default_rng(71)
+    hi = 14 if dtype == "fp8" else 5
+    base = rng.integers(-hi, hi + 1, size=(12, 128)).astype(np.float32)
+    base[:, 0] = hi
+    db = base[rng.integers(0, 12, size=60000)]
+    qs = rng.integers(-hi, hi + 1, size=(48, 128)).astype(np.float32)
+    qs[:, 1] = hi
+    idx = F.FlatIndex(dev(torch, db), dtype)
+    for metric in ("l2", "ip"):
+        out = idx.search(dev(torch, qs), k, metric)
+        torch.cuda.synchronize()
+        assert_same(L.split_results(out), O.flat_topk(db, qs, k, metric))
+
+
+@pytest.mark.parametrize("dtype", ["f16", "bf16"])
+@pytest.mark.parametrize("metric", ["l2", "ip"])
+@pytest.mark.parametrize("data", ["gauss", "near_dups"])
+def test_bounded_rerank_equals_full_rerank(torch, dtype, metric, data):
+    """The bounded rerank (only rows whose exact-distance interval reaches the
+    k-th upper bound are reranked) returns exactly what reranking the whole
+    shortlist returns: ids and FP32 distances, with and without a mask, on
+    Gaussian rows and on near-duplicates whose spread is below the codes'
+    rounding error."""
+    rng = np.random.default_rng(81)
+    n, nq, d = 20000, 64, 96
+    if data == "gauss":
+        db = rng.normal(0.0, 3.0, size=(n, d)).astype(np.float32)
+        qs = rng.normal(0.0, 3.0, size=(nq, d)).astype(np.float32)
+    else:
+        base = rng.normal(0.0, 3.0, size=(40, d))
+        db = (base[rng.integers(0, 40, size=n)] + rng.normal(0.0, 1e-3, size=(n, d))).astype(np.float32)
+        qs = (base[rng.integers(0, 40, size=nq)] + rng.normal(0.0, 0.5, size=(nq, d))).astype(np.float32)
+    idx = F.FlatIndex(dev(torch, db), dtype)
+    mask = S.random_valid_mask(n, 0.4, 82)
+    for m in (None, mask):
+        for k in (1, 10, 32):
+            dm = None if m is None else dev(torch, m)
+            got = idx.search(dev(torch, qs), k, metric, dm, shortlist_factor=8)
+            ref = idx.search(dev(torch, qs), k, metric, dm, shortlist_factor=8, full_rerank=True)
+            torch.cuda.synchronize()
+            assert_same(L.split_results(got), L.split_results(ref))
+
+
+def test_bounded_rerank_legacy_entry_without_encoding_error(torch):
+    """cuda_fp16_tc_search_batch has no per-row encoding error (the bound
+    falls back to 2^-10 |x|): same results as the full rerank."""
+    rng = np.random.default_rng(83)
+    db = rng.normal(0.0, 2.0, size=(15000, 64)).astype(np.float32)
+    qs = rng.normal(0.0, 2.0, size=(50, 64)).astype(np.float32)
+    codes, norms = O.encode_fp16_padded(db)
+    idx = F.FlatIndex(dev(torch, db), "f16")
+    for metric in ("l2", "ip"):
+        got = F.fp16_tc_search_batch(dev(torch, codes.view(np.int16)), dev(torch, norms),
+                                     dev(torch, qs), 10, 64, metric, None, dev(torch, db))
+        ref = idx.search(dev(torch, qs), 10, metric, shortlist_factor=8, full_rerank=True)
+        torch.cuda.synchronize()
+        assert_same(L.split_results(got), L.split_results(ref))
