@@ -100,11 +100,14 @@ struct ScanShape {
     static constexpr int kQB = 4 * kQW;
     static constexpr int kBRegs = (kQW / 16) * KS * (kLaneBytes / 4);
     static constexpr int kOcc = kBRegs < 128 ? 2 : 1;
-    // Rows per LDS tile: 64, 32 or 16, the most whose two tiles fit 100 KB
-    // (taller tiles beat a second workgroup per CU: measured at 768 dims).
-    static constexpr int kRT = 2 * 64 * (kRowBytes + 16) <= 102400
-                                   ? 64
-                                   : (2 * 32 * (kRowBytes + 16) <= 102400 ? 32 : 16);
+    // Rows per LDS tile: 128, 64, 32 or 16, the most whose two tiles fit
+    // 100 KB (taller tiles beat a second workgroup per CU: measured at 768
+    // dims).
+    static constexpr int kRT = 2 * 128 * (kRowBytes + 16) <= 102400
+                                   ? 128
+                                   : (2 * 64 * (kRowBytes + 16) <= 102400
+                                          ? 64
+                                          : (2 * 32 * (kRowBytes + 16) <= 102400 ? 32 : 16));
     // Workgroup-prune instantiations (deep shortlists) need up to 82 KB of
     // prune scratch, so their double-buffered tiles stay within 50 KB.
     static constexpr int kRTDeep = 2 * 64 * (kRowBytes + 16) <= 51200
