@@ -502,8 +502,8 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
         // scaling; then selection. The MMAs of tile rt+1 are issued before
         // the selection of tile rt, so that VALU work overlaps the matrix
         // pipe.
-        auto mma_tile = [&](int rt, floatx4* acc) {
-            vec afrag[KS];
+        // A fragments of row tile rt (16 rows) from LDS.
+        auto load_frags = [&](int rt, vec* afrag) {
             if constexpr (DMA) {
                 // Row rt*16 + l16 (so its swizzle is l16); the lane's K bytes
                 // LB*KS*grp + LB*s are chunks (LB/16)*(KS*grp + s) (+1).
@@ -528,6 +528,8 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
 #pragma unroll
                 for (int s = 0; s < KS; ++s) afrag[s] = ld_vec<vec>(arow + LB * s);
             }
+        };
+        auto mma_frags = [&](int rt, const vec* afrag, floatx4* acc) {
             const floatx4 cin = cin_t[rt];
 #pragma unroll
             for (int j = 0; j < NJ; ++j) acc[j] = kFP8 ? floatx4{0.0f, 0.0f, 0.0f, 0.0f} : cin;
@@ -573,13 +575,22 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
                 if (nrec > kRec - kWave) flush_records(buf);
             }
         };
+        // Software pipeline over the row tiles of this tile: the fragments of
+        // row tile rt+2 are read from LDS while the MMAs of rt+1 run and the
+        // selection of rt is done, so no MMA waits on its LDS read.
         floatx4 acc0[NJ], acc1[NJ];
-        mma_tile(0, acc0);
+        vec fr0[KS], fr1[KS];
+        load_frags(0, fr0);
+        if (NS > 1) load_frags(1, fr1);
+        mma_frags(0, fr0, acc0);
 #pragma unroll
-        for (int rt = 0; rt < RT / 16; ++rt) {
+        for (int rt = 0; rt < NS; ++rt) {
             floatx4* cur = (rt & 1) ? acc1 : acc0;
             floatx4* nxt = (rt & 1) ? acc0 : acc1;
-            if (rt + 1 < RT / 16) mma_tile(rt + 1, nxt);
+            const vec* fn = (rt & 1) ? fr0 : fr1;  // row tile rt+1, loaded a step ahead
+            vec* fnn = (rt & 1) ? fr1 : fr0;       // free: row tile rt's MMAs are issued
+            if (rt + 1 < NS) mma_frags(rt + 1, fn, nxt);
+            if (rt + 2 < NS) load_frags(rt + 2, fnn);
             select_tile(rt, cur);
         }
         if (!BOUND) flush_records(buf);
