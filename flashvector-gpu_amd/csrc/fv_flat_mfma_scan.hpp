@@ -234,7 +234,9 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     __shared__ __attribute__((aligned(16))) uint8_t s_tile0[RT * kLdsRow];
     __shared__ __attribute__((aligned(16))) uint8_t s_tile1[RT * kLdsRow];
     __shared__ __attribute__((aligned(16))) float s_cinit[2][RT];
-    __shared__ __attribute__((aligned(16))) float s_scale[2][RT];
+    // Row scales: FP8 and cosine only (one slot otherwise, never read).
+    constexpr bool kScaled = kFP8 || METRIC == METRIC_COSINE;
+    __shared__ __attribute__((aligned(16))) float s_scale[2][kScaled ? RT : 1];
     __shared__ uint32_t s_rowid[2][RT];
     __shared__ uint32_t s_cnt[QB];
     __shared__ uint32_t s_keep[QB];
@@ -409,7 +411,7 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
                 if (kFP8) sc = sc * pre_rs;
             }
             s_cinit[buf][st_row] = c;
-            s_scale[buf][st_row] = sc;
+            if constexpr (kScaled) s_scale[buf][st_row] = sc;
             s_rowid[buf][st_row] = pre_row;
         }
     };
