@@ -244,3 +244,20 @@ def test_autotune_plan_vs_oracle(torch, case):
     nps = np.full(qs.shape[0], 32, np.uint32)
     exp = O.ivf_pq_search_uneven(h, qs, ks, nps, 100, out_stride=10)
     assert_same(L.split_results(out), exp)
+
+
+@pytest.mark.parametrize("target_items,chunk_min", [(0, 16384), (4096, 1)])
+def test_split_plans_deep_shortlist_vs_oracle(torch, case, target_items, chunk_min):
+    """Split queries with a candidate buffer in global memory (shortlist 9000)
+    and k up to 256: the merged item shortlists give the oracle's results."""
+    db, qs, t, idx, h = case
+    rng = np.random.default_rng(49)
+    nq = 40
+    ks = rng.choice([10, 256], size=nq).astype(np.uint32)
+    nps = rng.choice([16, 64], size=nq).astype(np.uint32)
+    tu = L.FvIvfPqTuning(target_items, chunk_min, 0.0, 0)
+    out = idx.search(dev(torch, qs[:nq]), 256, 64, 9000, top_k=dev(torch, ks.view(np.int32)),
+                     nprobe=dev(torch, nps.view(np.int32)), tuning=tu)
+    torch.cuda.synchronize()
+    exp = O.ivf_pq_search_uneven(h, qs[:nq], ks, nps, 9000, out_stride=256)
+    assert_same(L.split_results(out), exp)
