@@ -36,6 +36,19 @@ __device__ __forceinline__ uint64_t make_key(float dist, uint32_t slot) {
 __device__ __forceinline__ float key_dist(uint64_t k) { return unord_f32((uint32_t)(k >> 32)); }
 __device__ __forceinline__ uint32_t key_slot(uint64_t k) { return (uint32_t)k; }
 
+// Buffer resource over `bytes` bytes at p (raw, 32-bit elements): loads past
+// the end return zero.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
+                                             0x00020000);
+}
+
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // kernels/src/distance_metrics.cuh:17-20: LSB-first u32 words, NULL allows all.
 __device__ __forceinline__ bool mask_allows(const uint32_t* __restrict__ mask, uint32_t idx) {
     return mask == nullptr || ((__ldg(mask + (idx >> 5)) >> (idx & 31u)) & 1u) != 0u;

@@ -124,11 +124,15 @@ struct ScanShape {
 template <int DT, int KS>
 constexpr bool scan_dma_ok() { return ScanShape<DT, KS>::kRowBytes % 256 == 0; }
 
-// One 16-byte direct-to-LDS load per lane: lane l's 16 bytes from `src` land
-// at lds_base + 16 * l (lds_base uniform across the wavefront).
-__device__ __forceinline__ void lds_dma16(const void* src, void* lds_base) {
-    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)lds_base, 16,
-                                     0, 0);
+// One 16-byte direct-to-LDS load per lane: lane l's 16 bytes at byte offset
+// `off` of the buffer land at lds_base + 16 * l (lds_base uniform across the
+// wavefront). A buffer (not global) load: the compiler then tracks it on the
+// vector-memory counter only, so LDS reads issued after it keep counted
+// (lgkmcnt(N)) waits instead of waiting for every outstanding LDS read.
+__device__ __forceinline__ void lds_dma16(__amdgpu_buffer_rsrc_t rsrc, uint32_t off,
+                                          void* lds_base) {
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+        rsrc, (__attribute__((address_space(3))) void*)lds_base, 16, off, 0, 0, 0);
 }
 
 // s_waitcnt immediate of the gfx9 family (gfx950): vmcnt in bits 3:0 and
@@ -210,6 +214,9 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     constexpr int QW = Sh::kQW;
     constexpr int QB = Sh::kQB;
     constexpr int NJ = QW / 16;                  // 16-query column blocks per wave
+    // Rows of 12 or more 16-byte fragment reads per lane stream their A
+    // fragments (register budget: the query fragments take 192 VGPRs there).
+    constexpr bool kStreamA = KS * (MfmaOps<DT>::kLaneBytes / 16) >= 12;
     // Rows per tile. The first pass (no dynamic LDS) and the workgroup-prune
     // instantiations keep their double-buffered tiles within 50 KB, so that
     // several workgroups share a CU.
@@ -312,7 +319,14 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     for (int j = 0; j < NJ; ++j) {
         const uint8_t* qrow = Qh + (size_t)(qw0 + 16 * j + l16) * kRowBytes + LB * grp;
 #pragma unroll
-        for (int s = 0; s < KS; ++s) bfrag[j][s] = ld_vec<vec>(qrow + LB * 4 * s);
+        for (int s = 0; s < KS; ++s) {
+            bfrag[j][s] = ld_vec<vec>(qrow + LB * 4 * s);
+            // One wave per SIMD (large query fragments): keep the query
+            // fragments in accumulation registers (AGPRs). The MMAs read their
+            // B operand from there, and the architectural VGPRs stay free for
+            // the row fragments, addresses and selection.
+            if constexpr (Sh::kOcc == 1) __asm__ volatile("" : "+a"(bfrag[j][s]));
+        }
         qsc[j] = kFP8 ? qscale[qw0 + 16 * j + l16] : 1.0f;
     }
     // Retire the fragment loads here: otherwise the wait-count tracking keeps
@@ -358,6 +372,9 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     // dma_row[i], global byte offset dma_off[i] within that row (the swizzle
     // applied in reverse); its row id is fetched a tile ahead like rid_next.
     uint32_t dma_row[kDmaPerWave], dma_off[kDmaPerWave], dma_rid[kDmaPerWave];
+    // The table as one buffer (the launch takes this path only for tables of
+    // less than 4 GiB, so every row offset fits the 32-bit buffer offset).
+    const __amdgpu_buffer_rsrc_t xrsrc = raw_rsrc(X, DMA ? n * row_bytes : 0u);
 #pragma unroll
     for (int i = 0; i < kDmaPerWave; ++i) {
         const uint32_t o = (uint32_t)((wave * kDmaPerWave + i) * 1024 + 16 * lane);
@@ -385,7 +402,7 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
             // Rows past the chunk end load a valid row (their C is +inf).
 #pragma unroll
             for (int i = 0; i < kDmaPerWave; ++i)
-                lds_dma16(X + (size_t)dma_rid[i] * row_bytes + dma_off[i],
+                lds_dma16(xrsrc, dma_rid[i] * row_bytes + dma_off[i],
                           tile_ptr(dbc) + (wave * kDmaPerWave + i) * 1024);
         } else {
             const uint8_t* src = X + (size_t)pre_row * row_bytes;
@@ -537,6 +554,26 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
                 for (int s = 0; s < KS; ++s) afrag[s] = ld_vec<vec>(arow + LB * 4 * s);
             }
         };
+        auto load_frag_step = [&](int rt, int s) -> vec {
+            if constexpr (DMA) {
+                const uint8_t* rb = tile_ptr(bc) + (rt * 16 + l16) * kRowBytes;
+                constexpr int CPL = LB / 16;
+                const uint32_t u = (uint32_t)(CPL * (4 * s + grp));
+                if constexpr (CPL == 1) {
+                    return ld_vec<vec>(rb + ((u ^ (uint32_t)l16) * 16u));
+                } else {
+                    struct Two {
+                        uint4 a, b;
+                    } t2;
+                    t2.a = *reinterpret_cast<const uint4*>(rb + ((u ^ (uint32_t)l16) * 16u));
+                    t2.b = *reinterpret_cast<const uint4*>(rb + (((u + 1u) ^ (uint32_t)l16) * 16u));
+                    return __builtin_bit_cast(vec, t2);
+                }
+            } else {
+                const uint8_t* arow = tile_ptr(bc) + (rt * 16 + l16) * kLdsRow + grp * LB;
+                return ld_vec<vec>(arow + LB * 4 * s);
+            }
+        };
         auto mma_frags = [&](int rt, const vec* afrag, floatx4* acc) {
             const floatx4 cin = cin_t[rt];
 #pragma unroll
@@ -546,6 +583,30 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
             for (int s = 0; s < KS; ++s)
 #pragma unroll
                 for (int j = 0; j < NJ; ++j) acc[j] = Ops::mma(afrag[s], bfrag[j][s], acc[j]);
+        };
+        // A fragments of row tile rt read kAhead steps ahead of their MMAs
+        // (a register ring); the scheduling groups keep that order (one LDS
+        // read, then the step's NJ MMAs), which the scheduler would otherwise
+        // collapse into read-wait-MMA pairs.
+        auto mma_stream = [&](int rt, floatx4* acc) {
+            constexpr int kAhead = KS < 4 ? KS : 4;
+            constexpr int kReads = MfmaOps<DT>::kLaneBytes / 16;  // ds_read_b128 per fragment
+            const floatx4 cin = cin_t[rt];
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) acc[j] = kFP8 ? floatx4{0.0f, 0.0f, 0.0f, 0.0f} : cin;
+            vec ring[kAhead];
+#pragma unroll
+            for (int s = 0; s < kAhead; ++s) ring[s] = load_frag_step(rt, s);
+            __builtin_amdgcn_sched_group_barrier(0x100, kAhead * kReads, 0);
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const vec a = ring[s % kAhead];
+                if (s + kAhead < KS) ring[s % kAhead] = load_frag_step(rt, s + kAhead);
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) acc[j] = Ops::mma(a, bfrag[j][s], acc[j]);
+                if (s + kAhead < KS) __builtin_amdgcn_sched_group_barrier(0x100, kReads, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, NJ, 0);
+            }
         };
         auto select_tile = [&](int rt, floatx4* acc) {
             if (kFP8 || METRIC == METRIC_COSINE) {
@@ -583,23 +644,37 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
                 if (nrec > kRec - kWave) flush_records(buf);
             }
         };
-        // Software pipeline over the row tiles of this tile: the fragments of
-        // row tile rt+2 are read from LDS while the MMAs of rt+1 run and the
-        // selection of rt is done, so no MMA waits on its LDS read.
         floatx4 acc0[NJ], acc1[NJ];
-        vec fr0[KS], fr1[KS];
-        load_frags(0, fr0);
-        if (NS > 1) load_frags(1, fr1);
-        mma_frags(0, fr0, acc0);
+        if constexpr (kStreamA) {
+            // Wide rows: the A fragments of a row tile are read step by step
+            // next to their MMAs (no whole-tile fragment sets in registers);
+            // the MMAs of rt+1 are still issued before the selection of rt.
+            mma_stream(0, acc0);
 #pragma unroll
-        for (int rt = 0; rt < NS; ++rt) {
-            floatx4* cur = (rt & 1) ? acc1 : acc0;
-            floatx4* nxt = (rt & 1) ? acc0 : acc1;
-            const vec* fn = (rt & 1) ? fr0 : fr1;  // row tile rt+1, loaded a step ahead
-            vec* fnn = (rt & 1) ? fr1 : fr0;       // free: row tile rt's MMAs are issued
-            if (rt + 1 < NS) mma_frags(rt + 1, fn, nxt);
-            if (rt + 2 < NS) load_frags(rt + 2, fnn);
-            select_tile(rt, cur);
+            for (int rt = 0; rt < NS; ++rt) {
+                floatx4* cur = (rt & 1) ? acc1 : acc0;
+                floatx4* nxt = (rt & 1) ? acc0 : acc1;
+                if (rt + 1 < NS) mma_stream(rt + 1, nxt);
+                select_tile(rt, cur);
+            }
+        } else {
+            // Software pipeline over the row tiles of this tile: the fragments
+            // of row tile rt+2 are read from LDS while the MMAs of rt+1 run and
+            // the selection of rt is done, so no MMA waits on its LDS read.
+            vec fr0[KS], fr1[KS];
+            load_frags(0, fr0);
+            if (NS > 1) load_frags(1, fr1);
+            mma_frags(0, fr0, acc0);
+#pragma unroll
+            for (int rt = 0; rt < NS; ++rt) {
+                floatx4* cur = (rt & 1) ? acc1 : acc0;
+                floatx4* nxt = (rt & 1) ? acc0 : acc1;
+                const vec* fn = (rt & 1) ? fr0 : fr1;  // row tile rt+1, loaded a step ahead
+                vec* fnn = (rt & 1) ? fr1 : fr0;       // free: row tile rt's MMAs are issued
+                if (rt + 1 < NS) mma_frags(rt + 1, fn, nxt);
+                if (rt + 2 < NS) load_frags(rt + 2, fnn);
+                select_tile(rt, cur);
+            }
         }
         if (!BOUND) flush_records(buf);
         if (t + 1 < ntiles) store_tile(Next{});
@@ -742,11 +817,13 @@ static hipError_t launch_scan_d(const ScanLaunch& L) {
 }
 
 // Direct-to-LDS staging when the rows are whole 256-byte multiples with no
-// K padding beyond the table's own (row stride == staged row bytes).
+// K padding beyond the table's own (row stride == staged row bytes) and the
+// table spans less than 4 GiB (32-bit buffer offsets).
 template <int DT, int METRIC, int KS>
 static hipError_t launch_scan(const ScanLaunch& L) {
     if constexpr (scan_dma_ok<DT, KS>()) {
-        if (L.row_bytes == (uint32_t)ScanShape<DT, KS>::kRowBytes)
+        if (L.row_bytes == (uint32_t)ScanShape<DT, KS>::kRowBytes &&
+            (uint64_t)L.n * L.row_bytes <= 0xFFFFFFFFull)
             return launch_scan_d<DT, METRIC, KS, true>(L);
     }
     return launch_scan_d<DT, METRIC, KS, false>(L);

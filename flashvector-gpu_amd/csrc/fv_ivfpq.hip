@@ -215,19 +215,6 @@ struct IvfScanArgs {
     uint32_t* part_cnt;           // [slots]
 };
 
-// Buffer resource over `bytes` bytes at p (raw, 32-bit elements): loads past
-// the end return zero.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void* p, uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
-                                             0x00020000);
-}
-
-__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
-
 // Persistent: each workgroup repeatedly takes the next work item through a
 // global atomic counter. A query whose probed lists hold more entries than
 // the batch's chunk length is split into items of at most that many entries
