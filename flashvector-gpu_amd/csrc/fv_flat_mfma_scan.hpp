@@ -249,7 +249,10 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     // Row scales: FP8 and cosine only (one slot otherwise, never read).
     constexpr bool kScaled = kFP8 || METRIC == METRIC_COSINE;
     __shared__ __attribute__((aligned(16))) float s_scale[2][kScaled ? RT : 1];
-    __shared__ uint32_t s_rowid[2][RT];
+    // Row ids of the last four tiles (slot = tile index mod 4): records of a
+    // tile are flushed up to one tile later, while other waves already store
+    // the ids of the tile after next.
+    __shared__ uint32_t s_rowid[4][RT];
     __shared__ uint32_t s_cnt[QB];
     __shared__ uint32_t s_keep[QB];
     __shared__ float s_thr[QB];
@@ -353,7 +356,12 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     // Tiles between buffer checks: after a prune a buffer holds at most
     // S + 64 keys (bisection slack), so pk more tiles fit in cap.
     const uint32_t smax_c = a.s_max < cap ? a.s_max : cap;
-    const uint32_t pk = cap > smax_c + 64u + RT ? (cap - smax_c - 64u) / RT : 1u;
+    uint32_t pk = cap > smax_c + 64u + RT ? (cap - smax_c - 64u) / RT : 1u;
+    // Wide rows (few row tiles per tile, long tiles): with an even check
+    // interval, records are flushed every second tile (the checks fall on
+    // flushed tiles); otherwise after every tile.
+    const bool flush_pairs = kStreamA && pk >= 2u;
+    if (flush_pairs) pk &= ~1u;
     const int st_row = tid / kThrPerRow;
     const int st_seg = tid % kThrPerRow;
     // Staging registers of the next tile. Nothing loaded here is used until
@@ -417,7 +425,7 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
         if (kFP8) pre_rs = row_scales[pre_row];
         pre_mword = (a.mask && !rows) ? a.mask[pre_row >> 5] : 0xFFFFFFFFu;
     };
-    auto store_tile = [&](auto bc) {
+    auto store_tile = [&](auto bc, uint32_t slot) {
         constexpr int buf = decltype(bc)::value;
         if constexpr (!DMA) {
             uint8_t* dst = tile_ptr(bc) + st_row * kLdsRow + st_seg * kSegBytes;
@@ -435,7 +443,7 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
             }
             s_cinit[buf][st_row] = c;
             if constexpr (kScaled) s_scale[buf][st_row] = sc;
-            s_rowid[buf][st_row] = pre_row;
+            s_rowid[slot][st_row] = pre_row;
         }
     };
 
@@ -452,12 +460,13 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     floatx4* rec_score = reinterpret_cast<floatx4*>(rec_base);
     uint32_t* rec_meta = reinterpret_cast<uint32_t*>(rec_base + (size_t)kRec * 16);
     uint32_t nrec = 0;
-    auto flush_records = [&](int buf) {
+    auto flush_records = [&]() {
         wave_lds_sync();
         for (uint32_t e = lane; e < nrec; e += kWave) {
             const uint32_t meta = rec_meta[e];
             const uint32_t ql = wave * QW + (meta & 0xFFu);
             const uint32_t rbase = ((meta >> 8) & 0xFFu);  // first of the 4 rows within the tile
+            const uint32_t rslot = meta >> 16;             // row-id slot of the record's tile
             const float th = s_thr[ql];
             const floatx4 sc = rec_score[e];
             uint64_t* bq = cand + ((size_t)chunk * nq + qb0 + ql) * cap;
@@ -475,7 +484,7 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
                 uint32_t pos = atomicAdd(&s_cnt[ql], npass);
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    if (ok[i]) bq[pos++] = make_key(sc[i], s_rowid[buf][rbase + i]);
+                    if (ok[i]) bq[pos++] = make_key(sc[i], s_rowid[rslot][rbase + i]);
             }
         }
         nrec = 0;
@@ -486,7 +495,7 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
         fetch_rid(r_begin);
         load_tile(r_begin, B0{});
         if (ntiles > 1) fetch_rid(r_begin + RT);
-        store_tile(B0{});
+        store_tile(B0{}, 0u);
     }
     __syncthreads();
 
@@ -638,10 +647,11 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
                     const uint32_t idx = nrec + __builtin_amdgcn_mbcnt_hi(
                         (uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
                     rec_score[idx] = acc[j];
-                    rec_meta[idx] = (uint32_t)(16 * j + l16) | ((uint32_t)(rt * 16 + 4 * grp) << 8);
+                    rec_meta[idx] = (uint32_t)(16 * j + l16) | ((uint32_t)(rt * 16 + 4 * grp) << 8) |
+                                    ((t & 3u) << 16);
                 }
                 nrec += (uint32_t)__popcll(bm);
-                if (nrec > kRec - kWave) flush_records(buf);
+                if (nrec > kRec - kWave) flush_records();
             }
         };
         floatx4 acc0[NJ], acc1[NJ];
@@ -676,8 +686,10 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
                 select_tile(rt, cur);
             }
         }
-        if (!BOUND) flush_records(buf);
-        if (t + 1 < ntiles) store_tile(Next{});
+        // A pair of tiles shares one flush (the row ids of both stay in
+        // their slots until two tiles later).
+        if (!BOUND && (buf == 1 || !flush_pairs)) flush_records();
+        if (t + 1 < ntiles) store_tile(Next{}, (t + 1) & 3u);
         __syncthreads();
         if (BOUND) return;
 
@@ -769,6 +781,7 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
         }
         return;
     }
+    flush_records();  // an odd last tile
     if (my_live) counts[(size_t)chunk * nq + my_q] = s_cnt[my_ql];
 }
 
