@@ -594,26 +594,28 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
                 for (int j = 0; j < NJ; ++j) acc[j] = Ops::mma(afrag[s], bfrag[j][s], acc[j]);
         };
         // A fragments of row tile rt read kAhead steps ahead of their MMAs
-        // (a register ring); the scheduling groups keep that order (one LDS
-        // read, then the step's NJ MMAs), which the scheduler would otherwise
-        // collapse into read-wait-MMA pairs.
-        auto mma_stream = [&](int rt, floatx4* acc) {
-            constexpr int kAhead = KS < 4 ? KS : 4;
-            constexpr int kReads = MfmaOps<DT>::kLaneBytes / 16;  // ds_read_b128 per fragment
+        // through a register ring that runs on into the next row tile (its
+        // first steps are read during this tile's last MMAs); the scheduling
+        // groups keep that order (one LDS read, then the step's NJ MMAs),
+        // which the scheduler would otherwise collapse into read-wait-MMA
+        // pairs.
+        constexpr int kAhead = KS < 4 ? KS : 4;
+        constexpr int kReads = MfmaOps<DT>::kLaneBytes / 16;  // ds_read_b128 per fragment
+        auto mma_stream = [&](int rt, floatx4* acc, vec* ring) {
             const floatx4 cin = cin_t[rt];
 #pragma unroll
             for (int j = 0; j < NJ; ++j) acc[j] = kFP8 ? floatx4{0.0f, 0.0f, 0.0f, 0.0f} : cin;
-            vec ring[kAhead];
-#pragma unroll
-            for (int s = 0; s < kAhead; ++s) ring[s] = load_frag_step(rt, s);
-            __builtin_amdgcn_sched_group_barrier(0x100, kAhead * kReads, 0);
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
-                const vec a = ring[s % kAhead];
-                if (s + kAhead < KS) ring[s % kAhead] = load_frag_step(rt, s + kAhead);
+                // Ring slots follow the running step index rt*KS + s.
+                const int slot = (rt * KS + s) % kAhead;
+                const vec a = ring[slot];
+                const bool more = s + kAhead < KS || rt + 1 < NS;
+                if (s + kAhead < KS) ring[slot] = load_frag_step(rt, s + kAhead);
+                else if (rt + 1 < NS) ring[slot] = load_frag_step(rt + 1, s + kAhead - KS);
 #pragma unroll
                 for (int j = 0; j < NJ; ++j) acc[j] = Ops::mma(a, bfrag[j][s], acc[j]);
-                if (s + kAhead < KS) __builtin_amdgcn_sched_group_barrier(0x100, kReads, 0);
+                if (more) __builtin_amdgcn_sched_group_barrier(0x100, kReads, 0);
                 __builtin_amdgcn_sched_group_barrier(0x008, NJ, 0);
             }
         };
@@ -659,12 +661,16 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
             // Wide rows: the A fragments of a row tile are read step by step
             // next to their MMAs (no whole-tile fragment sets in registers);
             // the MMAs of rt+1 are still issued before the selection of rt.
-            mma_stream(0, acc0);
+            vec ring[kAhead];
+#pragma unroll
+            for (int s = 0; s < kAhead; ++s) ring[s] = load_frag_step(0, s);
+            __builtin_amdgcn_sched_group_barrier(0x100, kAhead * kReads, 0);
+            mma_stream(0, acc0, ring);
 #pragma unroll
             for (int rt = 0; rt < NS; ++rt) {
                 floatx4* cur = (rt & 1) ? acc1 : acc0;
                 floatx4* nxt = (rt & 1) ? acc0 : acc1;
-                if (rt + 1 < NS) mma_stream(rt + 1, nxt);
+                if (rt + 1 < NS) mma_stream(rt + 1, nxt, ring);
                 select_tile(rt, cur);
             }
         } else {
