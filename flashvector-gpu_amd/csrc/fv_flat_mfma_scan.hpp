@@ -378,8 +378,11 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     // Direct-to-LDS: load i of this lane covers LDS bytes
     // (wave * kDmaPerWave + i) * 1024 + 16 * lane of the tile, i.e. row
     // dma_row[i], global byte offset dma_off[i] within that row (the swizzle
-    // applied in reverse); its row id is fetched a tile ahead like rid_next.
+    // applied in reverse); its row id is fetched a tile ahead like rid_next,
+    // and its byte offset in the table when the tile's loads start.
     uint32_t dma_row[kDmaPerWave], dma_off[kDmaPerWave], dma_rid[kDmaPerWave];
+    uint32_t dma_voff[kDmaPerWave];
+    const uint32_t wave_u = __builtin_amdgcn_readfirstlane((uint32_t)wave);
     // The table as one buffer (the launch takes this path only for tables of
     // less than 4 GiB, so every row offset fits the 32-bit buffer offset).
     const __amdgpu_buffer_rsrc_t xrsrc = raw_rsrc(X, DMA ? n * row_bytes : 0u);
@@ -390,6 +393,7 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
         const uint32_t unit = (o % (uint32_t)kRowBytes) / 16u;
         dma_off[i] = (unit ^ (dma_row[i] & 15u)) * 16u;
         dma_rid[i] = 0;
+        dma_voff[i] = 0;
     }
     auto fetch_rid = [&](uint32_t p0) {
         const uint32_t p = p0 + st_row;
@@ -402,16 +406,29 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
             }
         }
     };
+    constexpr int kDmaFirst = (DMA && kStreamA) ? (kDmaPerWave + 1) / 2 : kDmaPerWave;
+    auto dma_rest = [&](auto dbc) {
+#pragma unroll
+        for (int i = kDmaFirst; i < kDmaPerWave; ++i)
+            lds_dma16(xrsrc, dma_voff[i], tile_ptr(dbc) + (wave_u * kDmaPerWave + i) * 1024);
+    };
     auto load_tile = [&](uint32_t p0, auto dbc) {
         const uint32_t p = p0 + st_row;
         pre_in = p < r_end;
         pre_row = pre_in ? rid_next : (rows ? rows[r_begin] : r_begin);
         if constexpr (DMA) {
             // Rows past the chunk end load a valid row (their C is +inf).
+            // Wide rows issue the second half after the first row tile's
+            // MMAs (dma_rest): the four waves' loads queue on the CU's one
+            // memory pipeline, and a wave stalls until its loads are taken.
+            // The offsets are computed here, once the row ids have arrived
+            // (a later write of a register that a direct-to-LDS load reads
+            // would wait for that load).
 #pragma unroll
-            for (int i = 0; i < kDmaPerWave; ++i)
-                lds_dma16(xrsrc, dma_rid[i] * row_bytes + dma_off[i],
-                          tile_ptr(dbc) + (wave * kDmaPerWave + i) * 1024);
+            for (int i = 0; i < kDmaPerWave; ++i) dma_voff[i] = dma_rid[i] * row_bytes + dma_off[i];
+#pragma unroll
+            for (int i = 0; i < kDmaFirst; ++i)
+                lds_dma16(xrsrc, dma_voff[i], tile_ptr(dbc) + (wave_u * kDmaPerWave + i) * 1024);
         } else {
             const uint8_t* src = X + (size_t)pre_row * row_bytes;
 #pragma unroll
@@ -494,9 +511,14 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     if (ntiles > 0) {
         fetch_rid(r_begin);
         load_tile(r_begin, B0{});
+        if constexpr (DMA) dma_rest(B0{});
         if (ntiles > 1) fetch_rid(r_begin + RT);
         store_tile(B0{}, 0u);
     }
+    // Every wave's direct-to-LDS loads of the tile have landed before any
+    // wave passes the barrier (the compiler's own waits cover only the
+    // issuing wave's later LDS reads).
+    if constexpr (DMA) __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     __syncthreads();
 
     floatx4 runmin[NJ];
@@ -526,7 +548,7 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
         }
         if (t + 1 < ntiles) {
             load_tile(p0 + RT, Next{});
-            if (t + 2 < ntiles) fetch_rid(p0 + 2 * RT);
+            if (!kStreamA && t + 2 < ntiles) fetch_rid(p0 + 2 * RT);
         }
         // Prefetch the shared bound; it is consumed after this tile's MMAs.
         const uint32_t gpre =
@@ -666,6 +688,8 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
             for (int s = 0; s < kAhead; ++s) ring[s] = load_frag_step(0, s);
             __builtin_amdgcn_sched_group_barrier(0x100, kAhead * kReads, 0);
             mma_stream(0, acc0, ring);
+            if (DMA && t + 1 < ntiles) dma_rest(Next{});
+            if (t + 2 < ntiles) fetch_rid(p0 + 2 * RT);
 #pragma unroll
             for (int rt = 0; rt < NS; ++rt) {
                 floatx4* cur = (rt & 1) ? acc1 : acc0;
@@ -696,6 +720,7 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
         // their slots until two tiles later).
         if (!BOUND && (buf == 1 || !flush_pairs)) flush_records();
         if (t + 1 < ntiles) store_tile(Next{}, (t + 1) & 3u);
+        if constexpr (DMA) __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
         __syncthreads();
         if (BOUND) return;
 
