@@ -406,7 +406,10 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
             }
         }
     };
-    constexpr int kDmaFirst = (DMA && kStreamA) ? (kDmaPerWave + 1) / 2 : kDmaPerWave;
+    // Wide rows issue the next tile's direct-to-LDS loads one by one between
+    // the MMAs of the first row tile (mma_stream); others at the tile start.
+    constexpr bool kDmaSpread = DMA && kStreamA;
+    constexpr int kDmaFirst = kDmaSpread ? 0 : kDmaPerWave;
     auto dma_rest = [&](auto dbc) {
 #pragma unroll
         for (int i = kDmaFirst; i < kDmaPerWave; ++i)
@@ -418,12 +421,11 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
         pre_row = pre_in ? rid_next : (rows ? rows[r_begin] : r_begin);
         if constexpr (DMA) {
             // Rows past the chunk end load a valid row (their C is +inf).
-            // Wide rows issue the second half after the first row tile's
-            // MMAs (dma_rest): the four waves' loads queue on the CU's one
-            // memory pipeline, and a wave stalls until its loads are taken.
-            // The offsets are computed here, once the row ids have arrived
-            // (a later write of a register that a direct-to-LDS load reads
-            // would wait for that load).
+            // The four waves' loads queue on the CU's one memory pipeline,
+            // and a wave stalls until its loads are taken, so wide rows spread
+            // them over their MMAs. The offsets are computed here, once the
+            // row ids have arrived (a later write of a register that a
+            // direct-to-LDS load reads would wait for that load).
 #pragma unroll
             for (int i = 0; i < kDmaPerWave; ++i) dma_voff[i] = dma_rid[i] * row_bytes + dma_off[i];
 #pragma unroll
@@ -623,7 +625,11 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
         // pairs.
         constexpr int kAhead = KS < 4 ? KS : 4;
         constexpr int kReads = MfmaOps<DT>::kLaneBytes / 16;  // ds_read_b128 per fragment
+        // Loads of the next tile, or zero-size ones into the idle buffer
+        // after the last tile (keeps the MMA stream free of branches).
+        const __amdgpu_buffer_rsrc_t dma_rs = t + 1 < ntiles ? xrsrc : raw_rsrc(X, 0u);
         auto mma_stream = [&](int rt, floatx4* acc, vec* ring) {
+            constexpr int D = kDmaSpread ? kDmaPerWave : 0;
             const floatx4 cin = cin_t[rt];
 #pragma unroll
             for (int j = 0; j < NJ; ++j) acc[j] = kFP8 ? floatx4{0.0f, 0.0f, 0.0f, 0.0f} : cin;
@@ -639,6 +645,20 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
                 for (int j = 0; j < NJ; ++j) acc[j] = Ops::mma(a, bfrag[j][s], acc[j]);
                 if (more) __builtin_amdgcn_sched_group_barrier(0x100, kReads, 0);
                 __builtin_amdgcn_sched_group_barrier(0x008, NJ, 0);
+                if (rt == 0 && D > 0) {
+                    // Loads i in [s*D/KS, (s+1)*D/KS), fenced off from the
+                    // scheduler so they stay between the steps' MMAs.
+                    const int i0 = s * D / KS, i1 = (s + 1) * D / KS;
+#pragma unroll
+                    for (int i = 0; i < D; ++i) {
+                        if (i >= i0 && i < i1) {
+                            __builtin_amdgcn_sched_barrier(0);
+                            lds_dma16(dma_rs, dma_voff[i],
+                                      tile_ptr(Next{}) + (wave_u * kDmaPerWave + i) * 1024);
+                            __builtin_amdgcn_sched_barrier(0);
+                        }
+                    }
+                }
             }
         };
         auto select_tile = [&](int rt, floatx4* acc) {
@@ -688,7 +708,6 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
             for (int s = 0; s < kAhead; ++s) ring[s] = load_frag_step(0, s);
             __builtin_amdgcn_sched_group_barrier(0x100, kAhead * kReads, 0);
             mma_stream(0, acc0, ring);
-            if (DMA && t + 1 < ntiles) dma_rest(Next{});
             if (t + 2 < ntiles) fetch_rid(p0 + 2 * RT);
 #pragma unroll
             for (int rt = 0; rt < NS; ++rt) {
