@@ -924,7 +924,8 @@ def bench_backward(args):
                               "dq_max_rel_err": rel(qs.grad, q64.grad),
                               "dx_max_rel_err": rel(xs.grad, x64.grad)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": 2500.0,
-                     "unit": "TFLOP/s", "frac": round(achieved / 2500.0, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / 2500.0, 4),
+                     "traffic": measured_traffic("flat_mfma_scan_kernel/topk_backward_ce"),
                      "kernel": "flat_mfma_scan_kernel (forward)",
                      "kernel_avg_ms": round(scan_ms / max(launches, 1), 4),
                      "algorithmic_flop_per_launch": flops},
