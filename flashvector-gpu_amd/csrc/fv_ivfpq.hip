@@ -245,6 +245,7 @@ __global__ __launch_bounds__(kIvfThreads, 4) void ivf_scan_kernel(IvfScanArgs a)
     // Dynamic LDS: LUT [M][256] f32 (unless static), candidate buffer (unless
     // in global memory), control words, the probed list's residual [dim].
     __shared__ float s_lut16[M16 ? 16 * 256 : 1];
+    __shared__ float s_lmin[M16 ? 16 : 1];
     extern __shared__ uint8_t s_dyn[];
     const IvfPqGpuTables& t = a.t;
     const uint32_t cap = a.cap;
@@ -352,12 +353,21 @@ __global__ __launch_bounds__(kIvfThreads, 4) void ivf_scan_kernel(IvfScanArgs a)
     };
 
     // Keys of 16-sub-quantizer entries: four lookups per segment; an entry
-    // whose running sum is past the bound's distance skips the rest (LUT
-    // entries are sums of squares, >= 0 or NaN, so the FP32 running sum never
-    // decreases; admitted entries add all 16 terms in order).
+    // whose running sum plus a lower bound of its remaining terms is past the
+    // bound's distance skips the rest; admitted entries add all 16 terms in
+    // order. LUT entries are sums of squares (>= 0 or NaN), so the rounded
+    // running sum never decreases and, rounding being monotone, the final sum
+    // is at least the sequential sum of the partial sum and the remaining
+    // sub-quantizers' LUT minima, itself at least (partial + exact sum of
+    // those minima) * (1 - 12u). lmin[k] under-estimates that exact suffix
+    // sum and thr_hi = thr * (1 + 2^-18) covers the relative errors (at most
+    // 13u with u = 2^-24), so an entry is only abandoned when its final sum
+    // exceeds the bound: results equal those of the plain sequential sums.
+    float lmin[3] = {0.0f, 0.0f, 0.0f};
     auto keys_m16 = [&](uint32_t n, const uint32_t (&ids)[kPassesPerCheck],
                         const uint4 (&cw)[kPassesPerCheck], float thr_d, bool masked,
                         uint64_t (&kk)[kPassesPerCheck]) {
+        const float thr_hi = thr_d * (1.0f + 0x1p-18f);
         auto seg4 = [&](int seg, uint32_t w, float s) {
             float l[4];
 #pragma unroll
@@ -376,11 +386,11 @@ __global__ __launch_bounds__(kIvfThreads, 4) void ivf_scan_kernel(IvfScanArgs a)
             kk[pass] = kEmptyKey;
             if (pass * kIvfThreads + threadIdx.x < n && (!masked || mask_allows(a.mask, ids[pass]))) {
                 float s = seg4(0, cw[pass].x, 0.0f);
-                if (s <= thr_d) {
+                if (s + lmin[0] <= thr_hi) {
                     s = seg4(1, cw[pass].y, s);
-                    if (s <= thr_d) {
+                    if (s + lmin[1] <= thr_hi) {
                         s = seg4(2, cw[pass].z, s);
-                        if (s <= thr_d) {
+                        if (s + lmin[2] <= thr_hi) {
                             s = seg4(3, cw[pass].w, s);
                             if (s <= thr_d && s < kAdmit) kk[pass] = make_key(s, ids[pass]);
                         }
@@ -506,6 +516,36 @@ __global__ __launch_bounds__(kIvfThreads, 4) void ivf_scan_kernel(IvfScanArgs a)
             __syncthreads();  // res visible; the previous list's lookups are done with the LUT
             build_lut(lut, t.d_pq_codebooks, res, M, sub);
             __syncthreads();
+            if constexpr (M16) {
+                static_assert(kIvfThreads == 256, "16 threads per sub-quantizer table");
+                // Minimum of each sub-quantizer's 256 entries (16 threads per
+                // table, conflict-free strided reads, then a 16-lane
+                // reduction), and the lower bounds of the last 12 / 8 / 4
+                // terms.
+                const uint32_t m = threadIdx.x >> 4, r = threadIdx.x & 15u;
+                float v = lut[m * 256u + r];
+#pragma unroll
+                for (uint32_t i = 1; i < 16; ++i) v = fminf(v, lut[m * 256u + i * 16u + r]);
+#pragma unroll
+                for (int o = 8; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o, 16));
+                if (r == 0) s_lmin[m] = v;
+                __syncthreads();
+                float suf = 0.0f;
+#pragma unroll
+                for (int k = 15; k >= 0; --k) {
+                    suf = suf + s_lmin[k];
+                    if ((k & 3) == 0 && k > 0) lmin[k / 4 - 1] = suf * (1.0f - 0x1p-19f);
+                }
+                // No entry of the list can reach the own threshold (which only
+                // falls): skip its scan. The threshold is the same in every
+                // wave (the shared bound is not: each wave reads it itself),
+                // so the whole workgroup skips together.
+                const float all = suf * (1.0f - 0x1p-19f);
+                if (all > dist_of(thr) * (1.0f + 0x1p-18f)) {
+                    cum += len;
+                    continue;
+                }
+            }
             scan_run(lb + (uint32_t)(s0 - cum), lb + (uint32_t)(s1 - cum));
         }
         cum += len;
