@@ -484,6 +484,7 @@ def bench_ivfpq(args):
                "sample": f"{cs} of the {nq} queries, same tables (oracle/fv_oracle.c ADC + rerank, "
                          f"OpenMP over queries), {cpu_s:.2f} s"}
     achieved = nbytes / (scan_avg / 1e3) / 1e9
+    ivf_traffic = measured_traffic("ivf_scan_kernel/ivfpq_uneven_10m")
     res = {
         "metric": "IVF-PQ batched uneven QPS (10M x 128, nlist 4096 Zipf, M=16, shortlist 1000)",
         "value": round(nq / (step_ms / 1e3), 1), "unit": "queries/s", "n_gpus": 1,
@@ -499,9 +500,14 @@ def bench_ivfpq(args):
         "hip_graph": True, "graph_replay_matches_eager": graph_matches_eager,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(achieved / 8000.0, 4),
-                     "traffic": measured_traffic("ivf_scan_kernel/ivfpq_uneven_10m"),
+                     "traffic": ivf_traffic,
                      "kernel": "ivf_scan_kernel", "kernel_avg_ms": round(scan_avg, 4),
-                     "algorithmic_bytes_per_launch": nbytes},
+                     "algorithmic_bytes_per_launch": nbytes,
+                     "convention": "achieved = the probed lists' code and id bytes, query-major "
+                                   "(every probed entry once per query), including lists that the "
+                                   "LUT-minimum bound skips without reading them",
+                     "measured_traffic_rate_gbps": (round(ivf_traffic / (scan_avg / 1e3) / 1e9, 1)
+                                                    if ivf_traffic else None)},
         "reference_best_of_5": per_call,
         "cpu_baseline": cpu,
     }
