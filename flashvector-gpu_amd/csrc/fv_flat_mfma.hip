@@ -97,15 +97,38 @@ __global__ __launch_bounds__(256) void bound_select_kernel(const float* __restri
 __device__ __forceinline__ uint64_t unadmitted_key(uint32_t row) { return 0xFFFFFFFF00000000ull | row; }
 __device__ __forceinline__ bool key_unadmitted(uint64_t k) { return (uint32_t)(k >> 32) == 0xFFFFFFFFu; }
 
+// Rounding of one query element as the scan's query encoder does it
+// (encode_queries_kernel), decoded back to FP32.
+__device__ __forceinline__ float coded_query_value(float v, uint32_t dt) {
+    if (dt == FV_DTYPE_F16) return (float)(_Float16)v;
+    uint32_t u = __float_as_uint(v);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return __uint_as_float(u & 0xFFFF0000u);
+}
+
+// Bounded rerank set for the select path: 16-bit codes, L2 or inner product.
+struct SelectBound {
+    const float* qm;        // FP32 queries [nq][dim]
+    const float* norms;     // |x|^2 per row
+    const float* enc_err;   // |x~ - x| per row, or null (element bound)
+    uint32_t dim;
+    uint32_t code_dtype;    // FV_DTYPE_F16 / FV_DTYPE_BF16; 0 = no bound
+    int metric;
+};
+constexpr int kSelectBoundPerThread = 8;
+
 // One workgroup per query: streams the candidates of every chunk that are
 // within the final shared bound into an LDS top-S (sorted only when more
 // than S arrive) and writes the S best keys (approximate score, row) to
-// sel[q][0..have) and have to sel_n[q].
+// sel[q][0..have) and have to sel_n[q]. With a bound (sb.code_dtype != 0)
+// only the rows whose error interval reaches the k-th smallest upper end are
+// written (the bound of bounded_rerank_set, applied by the whole workgroup):
+// the others cannot be among the k best.
 template <uint32_t kSelectBatch>
 __global__ __launch_bounds__(kSelectBatch) void flat_select_kernel(
     const uint64_t* __restrict__ cand, const uint32_t* __restrict__ counts, uint32_t n_chunks,
     uint32_t nq, uint32_t cap, const uint32_t* __restrict__ gthr, FlatSearchArgs a,
-    uint32_t lcap, uint64_t* __restrict__ sel, uint32_t* __restrict__ sel_n) {
+    uint32_t lcap, uint64_t* __restrict__ sel, uint32_t* __restrict__ sel_n, SelectBound sb) {
     extern __shared__ uint64_t s_k[];
     __shared__ uint32_t s_n;
     __shared__ uint32_t s_g;
@@ -176,8 +199,95 @@ __global__ __launch_bounds__(kSelectBatch) void flat_select_kernel(
     }
     const uint32_t have = n < S ? n : S;
     uint64_t* o = sel + (size_t)q * a.s_max;
-    for (uint32_t i = threadIdx.x; i < have; i += blockDim.x) o[i] = s_k[i];
-    if (threadIdx.x == 0) sel_n[q] = have;
+    if (sb.code_dtype == 0 || have <= k || have > kSelectBoundPerThread * kSelectBatch) {
+        for (uint32_t i = threadIdx.x; i < have; i += blockDim.x) o[i] = s_k[i];
+        if (threadIdx.x == 0) sel_n[q] = have;
+        return;
+    }
+    // The query's norm and coding error (FP64 sums; order does not matter,
+    // only a bound is needed).
+    constexpr uint32_t kWaves = kSelectBatch / kWave;
+    __shared__ double s_red[2][kWaves];
+    const float* qv = sb.qm + (size_t)q * sb.dim;
+    double nq2 = 0.0, eq2 = 0.0;
+    for (uint32_t d = threadIdx.x; d < sb.dim; d += kSelectBatch) {
+        const float v = qv[d];
+        const double e = (double)coded_query_value(-v, sb.code_dtype) + (double)v;
+        nq2 = __builtin_fma((double)v, (double)v, nq2);
+        eq2 = __builtin_fma(e, e, eq2);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        nq2 += __shfl_xor(nq2, off);
+        eq2 += __shfl_xor(eq2, off);
+    }
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    if (lane == 0) {
+        s_red[0][wave] = nq2;
+        s_red[1][wave] = eq2;
+    }
+    __syncthreads();
+    nq2 = 0.0;
+    eq2 = 0.0;
+    for (uint32_t w = 0; w < kWaves; ++w) {
+        nq2 += s_red[0][w];
+        eq2 += s_red[1][w];
+    }
+    const double slack = 1.0 + 0x1p-40;
+    const double qn = sqrt(nq2) * slack, eq = sqrt(eq2) * slack, qtn = qn + eq;
+    const double g8 = 8.0 * (1.25 * (double)sb.dim + 8.0) * 0x1p-24;
+    const double cm = sb.metric == METRIC_L2 ? 2.0 : 1.0;
+    const double u_rel = (sb.code_dtype == FV_DTYPE_F16 ? 0x1p-11 : 0x1p-8) * (1.0 + 0x1p-10);
+    const double u_abs = sb.code_dtype == FV_DTYPE_F16 ? 0x1p-24 * sqrt((double)sb.dim) * (1.0 + 0x1p-10) : 0.0;
+    constexpr int NR = kSelectBoundPerThread;
+    uint32_t hi_o[NR], lo_o[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const uint32_t i = (uint32_t)r * kSelectBatch + threadIdx.x;
+        hi_o[r] = 0xFFFFFFFFu;  // past the list: never counted, never kept
+        lo_o[r] = 0xFFFFFFFFu;
+        if (i < have) {
+            const uint64_t kv = s_k[i];
+            const uint32_t row = key_slot(kv);
+            const double sc = (double)key_dist(kv);
+            const double E = sb.metric == METRIC_L2 ? nq2 + 2.0 * sc : sc;
+            const double xn = sqrt((double)sb.norms[row]) * slack;
+            const double ex = sb.enc_err ? (double)sb.enc_err[row] : u_rel * xn + u_abs;
+            const double xtn = xn + ex;
+            const double B = cm * (qtn * ex + eq * xn) + g8 * (qtn + xtn) * (qtn + xtn);
+            const double hi = E + B, lo = E - B;
+            if (key_unadmitted(kv) || !(hi < 1e300) || !(lo > -1e300) || !(B < 1e300)) {
+                hi_o[r] = 0xFFFFFFFEu;  // counted as unbounded, always kept
+                lo_o[r] = 0u;
+            } else {
+                float hf = (float)hi, lf = (float)lo;
+                if ((double)hf < hi) hf = next_up(hf);
+                if ((double)lf > lo) lf = -next_up(-lf);
+                hi_o[r] = ord_f32(hf);
+                lo_o[r] = ord_f32(lf);
+            }
+        }
+    }
+    // U: the smallest ordered value with at least k upper ends at or below it.
+    uint32_t blo = 0u, bhi = 0xFFFFFFFEu;
+    while (blo < bhi) {
+        const uint32_t mid = blo + ((bhi - blo) >> 1);
+        uint32_t c = 0;
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            if ((uint32_t)r * kSelectBatch < have) c += (uint32_t)__syncthreads_count(hi_o[r] <= mid);
+        if (c >= k) bhi = mid;
+        else blo = mid + 1u;
+    }
+    __shared__ uint32_t s_m;
+    if (threadIdx.x == 0) s_m = 0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const uint32_t i = (uint32_t)r * kSelectBatch + threadIdx.x;
+        if (i < have && lo_o[r] <= bhi) o[atomicAdd(&s_m, 1u)] = s_k[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) sel_n[q] = s_m;
 }
 
 // Exact FP32 distance of one (query, row) pair, summed in the reference CPU
@@ -778,15 +888,6 @@ __device__ __forceinline__ uint64_t wave_kth_key(const uint64_t (&v)[NR], uint32
         else slo = mid + 1u;
     }
     return ((uint64_t)dk << 32) | shi;
-}
-
-// Rounding of one query element as the scan's query encoder does it
-// (encode_queries_kernel), decoded back to FP32.
-__device__ __forceinline__ float coded_query_value(float v, uint32_t dt) {
-    if (dt == FV_DTYPE_F16) return (float)(_Float16)v;
-    uint32_t u = __float_as_uint(v);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return __uint_as_float(u & 0xFFFF0000u);
 }
 
 // Bounded rerank set of one query (one wavefront; keys[0..have) hold the
@@ -1652,12 +1753,21 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
     uint32_t* sel_n = reinterpret_cast<uint32_t*>(ws + p.off_seln);
     uint32_t* sel_off = reinterpret_cast<uint32_t*>(ws + p.off_seloff);
     float* err = (a.flags & FV_SEARCH_F64_BOUNDARY) ? reinterpret_cast<float*>(ws + p.off_err) : nullptr;
+    // Bounded rerank set for 16-bit codes with FP32 rows (L2 and inner
+    // product), unless every selected row is to be reranked. Not with the
+    // FP64 boundary: its band is twice the largest error bound over all the
+    // reranked rows, so leaving rows out could change which rows report an
+    // FP64-rounded distance.
+    SelectBound sb{Qm, t->d_norms_sq, t->d_enc_err, t->dim, 0u, a.metric};
+    if ((t->dtype == FV_DTYPE_F16 || t->dtype == FV_DTYPE_BF16) && X32 && a.metric != METRIC_COSINE &&
+        !(a.flags & (FV_SEARCH_FULL_RERANK | FV_SEARCH_F64_BOUNDARY)))
+        sb.code_dtype = t->dtype;
     if (p.sel_batch == kSelectBatchLarge)
         flat_select_kernel<kSelectBatchLarge><<<nq, kSelectBatchLarge, p.lds_select, stream>>>(
-            cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.lcap, sel, sel_n);
+            cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.lcap, sel, sel_n, sb);
     else
         flat_select_kernel<kSelectBatchSmall><<<nq, kSelectBatchSmall, p.lds_select, stream>>>(
-            cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.lcap, sel, sel_n);
+            cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.lcap, sel, sel_n, sb);
     // Compact work list for the rerank: exclusive prefix of the per-query
     // counts (the last slot holds the total).
     exclusive_scan_kernel<<<1, 256, 0, stream>>>(sel_n, sel_off, nq, sel_off + nq);

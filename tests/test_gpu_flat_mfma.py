@@ -288,6 +288,40 @@ def test_bounded_rerank_equals_full_rerank(torch, dtype, metric, data):
             assert_same(L.split_results(got), L.split_results(ref))
 
 
+@pytest.mark.parametrize("dtype", ["f16", "bf16"])
+@pytest.mark.parametrize("metric", ["l2", "ip"])
+@pytest.mark.parametrize("data", ["gauss", "near_dups"])
+def test_bounded_select_path_equals_full_rerank(torch, dtype, metric, data):
+    """Shortlists deeper than the fused tail's (k=64, 8k = 512 rows, the
+    select / rerank / finalize kernels): the bounded rerank set chosen in the
+    select kernel returns exactly what reranking the whole shortlist returns,
+    ids and FP32 distances, with and without a mask, with one k for all
+    queries and per-query k; with the FP64 boundary every row is reranked,
+    so both calls agree there too."""
+    rng = np.random.default_rng(91)
+    n, nq, d = 30000, 96, 128
+    if data == "gauss":
+        db = rng.normal(0.0, 3.0, size=(n, d)).astype(np.float32)
+        qs = rng.normal(0.0, 3.0, size=(nq, d)).astype(np.float32)
+    else:
+        base = rng.normal(0.0, 3.0, size=(40, d))
+        db = (base[rng.integers(0, 40, size=n)] + rng.normal(0.0, 1e-3, size=(n, d))).astype(np.float32)
+        qs = (base[rng.integers(0, 40, size=nq)] + rng.normal(0.0, 0.5, size=(nq, d))).astype(np.float32)
+    idx = F.FlatIndex(dev(torch, db), dtype)
+    mask = S.random_valid_mask(n, 0.4, 92)
+    per_q = dev(torch, rng.integers(1, 65, size=nq).astype(np.int32))
+    for m in (None, mask):
+        dm = None if m is None else dev(torch, m)
+        for f64 in (False, True):
+            for kq in (None, per_q):
+                got = idx.search(dev(torch, qs), 64, metric, dm, shortlist_factor=8,
+                                 top_k_per_query=kq, f64_boundary=f64)
+                ref = idx.search(dev(torch, qs), 64, metric, dm, shortlist_factor=8,
+                                 top_k_per_query=kq, f64_boundary=f64, full_rerank=True)
+                torch.cuda.synchronize()
+                assert_same(L.split_results(got), L.split_results(ref))
+
+
 def test_bounded_rerank_legacy_entry_without_encoding_error(torch):
     """cuda_fp16_tc_search_batch has no per-row encoding error (the bound
     falls back to 2^-10 |x|): same results as the full rerank."""
