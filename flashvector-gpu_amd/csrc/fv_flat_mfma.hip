@@ -98,7 +98,7 @@ __device__ __forceinline__ uint64_t unadmitted_key(uint32_t row) { return 0xFFFF
 __device__ __forceinline__ bool key_unadmitted(uint64_t k) { return (uint32_t)(k >> 32) == 0xFFFFFFFFu; }
 
 // Rounding of one query element as the scan's query encoder does it
-// (encode_queries_kernel), decoded back to FP32.
+// (encode_queries_block), decoded back to FP32.
 __device__ __forceinline__ float coded_query_value(float v, uint32_t dt) {
     if (dt == FV_DTYPE_F16) return (float)(_Float16)v;
     uint32_t u = __float_as_uint(v);
@@ -1315,9 +1315,10 @@ __global__ __launch_bounds__(256) void flat_tail_kernel(
 // Negated queries as 16-bit codes (round to nearest even), rows of `pdq`
 // elements, zero padded.
 template <int DT>
-__global__ void encode_queries_kernel(const float* __restrict__ Qm, uint32_t nq, uint32_t dim,
-                                      uint32_t pdq, uint32_t total_rows, uint16_t* __restrict__ out) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void encode_queries_block(uint32_t blk, const float* __restrict__ Qm,
+                                                     uint32_t nq, uint32_t dim, uint32_t pdq,
+                                                     uint32_t total_rows, uint16_t* __restrict__ out) {
+    const size_t i = (size_t)blk * blockDim.x + threadIdx.x;
     if (i >= (size_t)total_rows * pdq) return;
     const uint32_t q = (uint32_t)(i / pdq), d = (uint32_t)(i % pdq);
     const float v = (q < nq && d < dim) ? -Qm[(size_t)q * dim + d] : 0.0f;
@@ -1453,12 +1454,14 @@ __global__ void exclusive_scan_kernel(const uint32_t* __restrict__ v, uint32_t* 
 
 // ---- mask compaction: ascending list of the rows whose bit is set ---------
 
-__global__ void mask_block_counts_kernel(const uint32_t* __restrict__ mask, uint32_t nwords,
-                                         uint32_t n, uint32_t* __restrict__ block_counts) {
+// Valid rows of the mask words of block `blk` (blockDim.x words per block).
+__device__ __forceinline__ void mask_block_counts_block(uint32_t blk, const uint32_t* __restrict__ mask,
+                                                       uint32_t nwords, uint32_t n,
+                                                       uint32_t* __restrict__ block_counts) {
     __shared__ uint32_t s;
     if (threadIdx.x == 0) s = 0;
     __syncthreads();
-    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t w = blk * blockDim.x + threadIdx.x;
     uint32_t c = 0;
     if (w < nwords) {
         uint32_t bits = mask[w];
@@ -1467,7 +1470,34 @@ __global__ void mask_block_counts_kernel(const uint32_t* __restrict__ mask, uint
     }
     atomicAdd(&s, c);
     __syncthreads();
-    if (threadIdx.x == 0) block_counts[blockIdx.x] = s;
+    if (threadIdx.x == 0) block_counts[blk] = s;
+}
+
+__global__ void mask_block_counts_kernel(const uint32_t* __restrict__ mask, uint32_t nwords,
+                                         uint32_t n, uint32_t* __restrict__ block_counts) {
+    mask_block_counts_block(blockIdx.x, mask, nwords, n, block_counts);
+}
+
+// Search set-up of the 16-bit path in one launch (256-thread blocks): blocks
+// [0, nb_q) encode the queries, the next nb_g reset the shared bounds, the
+// rest count the valid rows of the mask per block of words. The three jobs
+// are independent; one launch instead of three.
+template <int DT>
+__global__ __launch_bounds__(256) void prep16_kernel(const float* __restrict__ Qm, uint32_t nq,
+                                                     uint32_t dim, uint32_t pdq, uint32_t total_rows,
+                                                     uint16_t* __restrict__ qh, uint32_t nb_q,
+                                                     uint32_t* __restrict__ gthr, uint32_t nb_g,
+                                                     const uint32_t* __restrict__ mask, uint32_t nwords,
+                                                     uint32_t n, uint32_t* __restrict__ block_counts) {
+    const uint32_t b = blockIdx.x;
+    if (b < nb_q) {
+        encode_queries_block<DT>(b, Qm, nq, dim, pdq, total_rows, qh);
+    } else if (b < nb_q + nb_g) {
+        const uint32_t i = (b - nb_q) * blockDim.x + threadIdx.x;
+        if (i < nq) gthr[i] = 0xFFFFFFFFu;
+    } else {
+        mask_block_counts_block(b - nb_q - nb_g, mask, nwords, n, block_counts);
+    }
 }
 
 __global__ void mask_scan_kernel(uint32_t* __restrict__ block_counts, uint32_t nblocks,
@@ -1684,33 +1714,38 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
     uint64_t* cand = reinterpret_cast<uint64_t*>(ws + p.off_cand);
     const uint32_t* rows = nullptr;
     const uint32_t* nrows = nullptr;
-    fill_u32_kernel<<<(nq + 255) / 256, 256, 0, stream>>>(gthr, nq, 0xFFFFFFFFu);
-    if (p.compact) {
-        uint32_t* r = reinterpret_cast<uint32_t*>(ws + p.off_rows);
-        uint32_t* nr = reinterpret_cast<uint32_t*>(ws + p.off_nrows);
-        uint32_t* blk = reinterpret_cast<uint32_t*>(ws + p.off_blocks);
-        const uint32_t nwords = (t->num_vectors + 31) / 32;
-        mask_block_counts_kernel<<<p.mask_blocks, 256, 0, stream>>>(a.mask, nwords, t->num_vectors, blk);
-        mask_scan_kernel<<<1, 256, 0, stream>>>(blk, p.mask_blocks, nr);
-        mask_scatter_kernel<<<p.mask_blocks, 256, 0, stream>>>(a.mask, nwords, t->num_vectors, blk, r);
-        rows = r;
-        nrows = nr;
-    }
     uint8_t* Qh = reinterpret_cast<uint8_t*>(ws + p.off_qh);
     float* qscale = nullptr;
     const uint32_t qrows = p.qblocks * p.qb;
+    uint32_t* blk = reinterpret_cast<uint32_t*>(ws + p.off_blocks);
+    const uint32_t nwords = (t->num_vectors + 31) / 32;
+    const uint32_t nb_g = (nq + 255) / 256;
+    const uint32_t nb_m = p.compact ? p.mask_blocks : 0u;
     if (t->dtype == FV_DTYPE_FP8_E4M3) {
+        fill_u32_kernel<<<nb_g, 256, 0, stream>>>(gthr, nq, 0xFFFFFFFFu);
+        if (nb_m) mask_block_counts_kernel<<<nb_m, 256, 0, stream>>>(a.mask, nwords, t->num_vectors, blk);
         qscale = reinterpret_cast<float*>(ws + p.off_qscale);
         launch_encode_fp8_rows(Qm, nq, t->dim, p.qrow_bytes, qrows, -1.0f, Qh, qscale, stream);
     } else {
         const uint32_t pdq = p.qrow_bytes / 2;
         const size_t total = (size_t)qrows * pdq;
-        const unsigned blocks = (unsigned)((total + 255) / 256);
+        const uint32_t nb_q = (uint32_t)((total + 255) / 256);
         uint16_t* q16 = reinterpret_cast<uint16_t*>(Qh);
+        const dim3 g(nb_q + nb_g + nb_m);
         if (t->dtype == FV_DTYPE_F16)
-            encode_queries_kernel<FV_DTYPE_F16><<<blocks, 256, 0, stream>>>(Qm, nq, t->dim, pdq, qrows, q16);
+            prep16_kernel<FV_DTYPE_F16><<<g, 256, 0, stream>>>(Qm, nq, t->dim, pdq, qrows, q16, nb_q, gthr,
+                                                              nb_g, a.mask, nwords, t->num_vectors, blk);
         else
-            encode_queries_kernel<FV_DTYPE_BF16><<<blocks, 256, 0, stream>>>(Qm, nq, t->dim, pdq, qrows, q16);
+            prep16_kernel<FV_DTYPE_BF16><<<g, 256, 0, stream>>>(Qm, nq, t->dim, pdq, qrows, q16, nb_q, gthr,
+                                                               nb_g, a.mask, nwords, t->num_vectors, blk);
+    }
+    if (p.compact) {
+        uint32_t* r = reinterpret_cast<uint32_t*>(ws + p.off_rows);
+        uint32_t* nr = reinterpret_cast<uint32_t*>(ws + p.off_nrows);
+        mask_scan_kernel<<<1, 256, 0, stream>>>(blk, p.mask_blocks, nr);
+        mask_scatter_kernel<<<p.mask_blocks, 256, 0, stream>>>(a.mask, nwords, t->num_vectors, blk, r);
+        rows = r;
+        nrows = nr;
     }
     float* minima = p.bound_chunks ? reinterpret_cast<float*>(ws + p.off_minima) : nullptr;
     const uint32_t esz = t->dtype == FV_DTYPE_FP8_E4M3 ? 1u : 2u;
