@@ -219,8 +219,9 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     constexpr bool kStreamA = KS * (MfmaOps<DT>::kLaneBytes / 16) >= 12;
     // Rows per tile. The first pass (no dynamic LDS) and the workgroup-prune
     // instantiations keep their double-buffered tiles within 50 KB, so that
-    // several workgroups share a CU.
-    constexpr int RT = (R == 0 || BOUND) ? Sh::kRTDeep : Sh::kRT;
+    // several workgroups share a CU, unless the registers allow only one
+    // wave per SIMD anyway (then the taller tiles of the main pass).
+    constexpr int RT = ((R == 0 || BOUND) && Sh::kOcc > 1) ? Sh::kRTDeep : Sh::kRT;
     constexpr int LB = Sh::kLaneBytes;
     constexpr int kRowBytes = Sh::kRowBytes;
     // Register staging: 16-bit rows padded by 32 bytes (conflict-free
