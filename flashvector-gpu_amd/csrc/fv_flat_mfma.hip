@@ -1789,13 +1789,15 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
     uint32_t* sel_off = reinterpret_cast<uint32_t*>(ws + p.off_seloff);
     float* err = (a.flags & FV_SEARCH_F64_BOUNDARY) ? reinterpret_cast<float*>(ws + p.off_err) : nullptr;
     // Bounded rerank set for 16-bit codes with FP32 rows (L2 and inner
-    // product), unless every selected row is to be reranked. Not with the
-    // FP64 boundary: its band is twice the largest error bound over all the
-    // reranked rows, so leaving rows out could change which rows report an
-    // FP64-rounded distance.
+    // product), unless every selected row is to be reranked. With the FP64
+    // boundary the rows left out are beyond k kept rows in FP64 order too
+    // (the intervals bound the exact distance), so ids and order are those of
+    // the full rerank; the near-tie band is twice the largest error bound of
+    // the reranked rows, so which kept rows report an FP64-rounded distance
+    // can differ from a full rerank (as it does between shortlist depths).
     SelectBound sb{Qm, t->d_norms_sq, t->d_enc_err, t->dim, 0u, a.metric};
     if ((t->dtype == FV_DTYPE_F16 || t->dtype == FV_DTYPE_BF16) && X32 && a.metric != METRIC_COSINE &&
-        !(a.flags & (FV_SEARCH_FULL_RERANK | FV_SEARCH_F64_BOUNDARY)))
+        !(a.flags & FV_SEARCH_FULL_RERANK))
         sb.code_dtype = t->dtype;
     if (p.sel_batch == kSelectBatchLarge)
         flat_select_kernel<kSelectBatchLarge><<<nq, kSelectBatchLarge, p.lds_select, stream>>>(

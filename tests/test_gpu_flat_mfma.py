@@ -296,8 +296,10 @@ def test_bounded_select_path_equals_full_rerank(torch, dtype, metric, data):
     select / rerank / finalize kernels): the bounded rerank set chosen in the
     select kernel returns exactly what reranking the whole shortlist returns,
     ids and FP32 distances, with and without a mask, with one k for all
-    queries and per-query k; with the FP64 boundary every row is reranked,
-    so both calls agree there too."""
+    queries and per-query k. With the FP64 boundary the ids and their order
+    are the same; a distance may be the FP64 value rounded to FP32 in one call
+    and the FP32 rerank in the other (the near-tie band depends on the
+    reranked rows), so distances agree to the FP32 error bound there."""
     rng = np.random.default_rng(91)
     n, nq, d = 30000, 96, 128
     if data == "gauss":
@@ -319,7 +321,13 @@ def test_bounded_select_path_equals_full_rerank(torch, dtype, metric, data):
                 ref = idx.search(dev(torch, qs), 64, metric, dm, shortlist_factor=8,
                                  top_k_per_query=kq, f64_boundary=f64, full_rerank=True)
                 torch.cuda.synchronize()
-                assert_same(L.split_results(got), L.split_results(ref))
+                g, r = L.split_results(got), L.split_results(ref)
+                if not f64:
+                    assert_same(g, r)
+                else:
+                    np.testing.assert_array_equal(g[0], r[0])
+                    scale = np.abs(r[1]).max() + 1.0
+                    np.testing.assert_allclose(g[1], r[1], rtol=0, atol=1e-5 * scale)
 
 
 def test_bounded_rerank_legacy_entry_without_encoding_error(torch):
