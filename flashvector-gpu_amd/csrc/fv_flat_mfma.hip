@@ -141,6 +141,8 @@ __global__ __launch_bounds__(kSelectBatch) void flat_select_kernel(
         s_g = gthr[q];
     }
     __syncthreads();
+    const uint64_t cstride = a.cand_off ? a.cand_stride : (uint64_t)nq * cap;
+    const uint64_t qbase = a.cand_off ? a.cand_off[q] : (uint64_t)q * cap;
     // Candidate counts of every chunk, prefix-summed in LDS (chunks processed
     // kSelectBatch at a time), then the concatenated candidates streamed in
     // rounds of kSelectBatch: one barrier per round instead of one per chunk.
@@ -169,7 +171,7 @@ __global__ __launch_bounds__(kSelectBatch) void flat_select_kernel(
                     if (s_pre[mid] <= e) lo = mid;
                     else hi = mid - 1;
                 }
-                const uint64_t key = cand[((size_t)(c0 + lo) * nq + q) * cap + (e - s_pre[lo])];
+                const uint64_t key = cand[(c0 + lo) * cstride + qbase + (e - s_pre[lo])];
                 if ((uint32_t)(key >> 32) <= g) s_k[atomicAdd(&s_n, 1u)] = key;
             }
             __syncthreads();
@@ -774,6 +776,9 @@ __global__ __launch_bounds__(kFinalThreads) void flat_finalize_kernel(
 // k: the same results as flat_select_kernel, flat_rerank_kernel and
 // flat_finalize_kernel in sequence.
 constexpr uint32_t kTailMaxS = 256;
+// Keys of a per-query candidate buffer beyond the query's shortlist: 64 of
+// bisection slack plus 512 rows between buffer checks.
+constexpr uint32_t kPerQuerySlack = 64u + 512u;
 constexpr uint32_t kTailWaves = 4;
 
 // Exact distances and error bounds of NR rows of one query (rows[] valid,
@@ -1019,6 +1024,8 @@ __global__ __launch_bounds__(256) void flat_tail_kernel(
     query_k_s(a, q, k, S);
     uint32_t g = gthr[q];
     uint32_t n = 0;  // keys in the list (wave-uniform)
+    const uint64_t cstride = a.cand_off ? a.cand_stride : (uint64_t)nq * cap;
+    const uint64_t qbase = a.cand_off ? a.cand_off[q] : (uint64_t)q * cap;
 
     // Keeps the S smallest keys of keys[0..n) (sorted) and tightens g to the
     // S-th. n <= lw, so the padded length never exceeds lw.
@@ -1058,7 +1065,7 @@ __global__ __launch_bounds__(256) void flat_tail_kernel(
                     else hi = mid - 1u;
                 }
                 const uint32_t base = (uint32_t)__shfl((int)excl, (int)lo);
-                kb[r] = e < total ? cand[((size_t)(c0 + lo) * nq + q) * cap + (ee - base)] : kEmptyKey;
+                kb[r] = e < total ? cand[(c0 + lo) * cstride + qbase + (ee - base)] : kEmptyKey;
             }
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -1424,6 +1431,43 @@ __global__ __launch_bounds__(256) void bound_select_block_kernel(const float* __
     if (threadIdx.x == 0) atomicMin(&gthr[q], hi);
 }
 
+// Per-query candidate buffers: query q holds round_up(S_q + slack, 8) keys
+// of every chunk at off[q] (exclusive prefix sum by one workgroup, each
+// thread summing a contiguous run). Offsets are clamped to the planned keys
+// per chunk, so a query whose buffer does not fit (the caller's k sum was
+// too small) gets fewer than S_q + slack keys and the scan keeps none of its
+// candidates.
+__global__ void cand_offsets_kernel(FlatSearchArgs a, uint32_t nq, uint32_t slack, uint32_t budget,
+                                    uint32_t* __restrict__ off) {
+    __shared__ uint64_t s_part[256];
+    auto qkeys = [&](uint32_t q) -> uint64_t {
+        uint32_t k, S;
+        query_k_s(a, q, k, S);
+        return (uint64_t)((S + slack + 7u) & ~7u);
+    };
+    const uint32_t per = (nq + 255) / 256;
+    const uint32_t b0 = threadIdx.x * per;
+    uint64_t sum = 0;
+    for (uint32_t i = 0; i < per && b0 + i < nq; ++i) sum += qkeys(b0 + i);
+    s_part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t run = 0;
+        for (int i = 0; i < 256; ++i) {
+            const uint64_t x = s_part[i];
+            s_part[i] = run;
+            run += x;
+        }
+        off[nq] = (uint32_t)(run < budget ? run : budget);
+    }
+    __syncthreads();
+    uint64_t run = s_part[threadIdx.x];
+    for (uint32_t i = 0; i < per && b0 + i < nq; ++i) {
+        off[b0 + i] = (uint32_t)(run < budget ? run : budget);
+        run += qkeys(b0 + i);
+    }
+}
+
 // Exclusive prefix sum of n counts by one workgroup (each thread sums a
 // contiguous run) into out; the total goes to *total.
 __global__ void exclusive_scan_kernel(const uint32_t* __restrict__ v, uint32_t* __restrict__ out,
@@ -1597,6 +1641,28 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
         p.prune_regs = 0;  // workgroup prunes through shared scratch
     }
     p.block_prune = p.prune_regs == 0;
+    // Deep shortlists with per-query k and a known k sum: each query's
+    // buffers hold its own shortlist plus kPerQuerySlack keys (64 of
+    // bisection slack and 512 rows between buffer checks), laid out by a
+    // prefix sum on the device, instead of the deepest shortlist's
+    // power-of-two capacity for every query. The kernels see cap = s_max +
+    // slack, which sets their check interval to 512 rows.
+    p.percap = false;
+    if (p.block_prune && a.d_top_k && a.top_k_sum) {
+        // Shortlists are at most F * k + C (k >= 1, so top_k_sum + nq
+        // covers k = 0 entries), and never deeper than smax.
+        const uint64_t F = a.factor ? a.factor : (a.shortlist ? 1u : 8u);
+        const uint64_t C = (!a.factor && a.shortlist) ? a.shortlist : 0u;
+        uint64_t ssum = F * ((uint64_t)a.top_k_sum + nq) + C * nq;
+        if (ssum > (uint64_t)nq * smax) ssum = (uint64_t)nq * smax;
+        const uint64_t keys = ssum + (uint64_t)nq * (kPerQuerySlack + 7u);
+        if (keys < (uint64_t)nq * p.cap && keys <= 0xFFFFFFFFull) {
+            p.percap = true;
+            p.slack = kPerQuerySlack;
+            p.cand_keys = keys;
+            p.cap = smax + kPerQuerySlack;
+        }
+    }
     // Per wave: prune scratch and staging list share one region; workgroup
     // prunes use one shared scratch followed by four staging lists.
     p.lds_scan = p.block_prune ? (size_t)kScanSelBytes + 4 * (size_t)kRecBytes
@@ -1669,7 +1735,9 @@ FlatMfmaPlan plan_flat_mfma(uint32_t n, uint32_t padded_dim, int dtype, uint32_t
     };
     p.off_gthr = take((size_t)nq * 4);
     p.off_counts = take((size_t)p.n_chunks * nq * 4);
-    p.off_cand = take((size_t)p.n_chunks * nq * p.cap * 8);
+    if (!p.percap) p.cand_keys = (uint64_t)nq * p.cap;
+    p.off_cand = take((size_t)p.n_chunks * p.cand_keys * 8);
+    if (p.percap) p.off_candoff = take((size_t)(nq + 1) * 4);
     p.off_qh = take((size_t)p.qblocks * p.qb * p.qrow_bytes);
     p.off_sel = take((size_t)nq * smax * 8);
     p.off_seln = take((size_t)nq * 4);
@@ -1752,6 +1820,13 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
     FlatSearchArgs sa = a;
     sa.chunk_want = p.chunk_want;
     sa.chunk_min_rows = p.chunk_min_rows;
+    sa.cand_off = nullptr;
+    sa.cand_stride = p.cand_keys;
+    if (p.percap) {
+        uint32_t* co = reinterpret_cast<uint32_t*>(ws + p.off_candoff);
+        cand_offsets_kernel<<<1, 256, 0, stream>>>(a, nq, p.slack, (uint32_t)p.cand_keys, co);
+        sa.cand_off = co;
+    }
     ScanLaunch L{&p, static_cast<const uint8_t*>(t->d_codes), t->padded_dim * esz, t->d_norms_sq,
                  t->d_row_scales, Qh, qscale, nq, sa, rows, nrows, t->num_vectors, gthr, counts,
                  cand, minima, stream};
@@ -1772,15 +1847,15 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
         const uint32_t cdt = (t->dtype == FV_DTYPE_F16 || t->dtype == FV_DTYPE_BF16) ? t->dtype : 0u;
         if (a.metric == METRIC_L2)
             flat_tail_kernel<METRIC_L2><<<tg, tb, p.lds_tail, stream>>>(
-                cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.tail_lw, p.tail_hw, p.tail_wbytes, X32, Qm, out,
+                cand, counts, p.n_chunks, nq, p.cap, gthr, sa, p.tail_lw, p.tail_hw, p.tail_wbytes, X32, Qm, out,
                 t->d_norms_sq, t->d_enc_err, cdt);
         else if (a.metric == METRIC_INNER_PRODUCT)
             flat_tail_kernel<METRIC_INNER_PRODUCT><<<tg, tb, p.lds_tail, stream>>>(
-                cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.tail_lw, p.tail_hw, p.tail_wbytes, X32, Qm, out,
+                cand, counts, p.n_chunks, nq, p.cap, gthr, sa, p.tail_lw, p.tail_hw, p.tail_wbytes, X32, Qm, out,
                 t->d_norms_sq, t->d_enc_err, cdt);
         else
             flat_tail_kernel<METRIC_COSINE><<<tg, tb, p.lds_tail, stream>>>(
-                cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.tail_lw, p.tail_hw, p.tail_wbytes, X32, Qm, out,
+                cand, counts, p.n_chunks, nq, p.cap, gthr, sa, p.tail_lw, p.tail_hw, p.tail_wbytes, X32, Qm, out,
                 t->d_norms_sq, t->d_enc_err, 0u);
         return hipGetLastError();
     }
@@ -1801,10 +1876,10 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
         sb.code_dtype = t->dtype;
     if (p.sel_batch == kSelectBatchLarge)
         flat_select_kernel<kSelectBatchLarge><<<nq, kSelectBatchLarge, p.lds_select, stream>>>(
-            cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.lcap, sel, sel_n, sb);
+            cand, counts, p.n_chunks, nq, p.cap, gthr, sa, p.lcap, sel, sel_n, sb);
     else
         flat_select_kernel<kSelectBatchSmall><<<nq, kSelectBatchSmall, p.lds_select, stream>>>(
-            cand, counts, p.n_chunks, nq, p.cap, gthr, a, p.lcap, sel, sel_n, sb);
+            cand, counts, p.n_chunks, nq, p.cap, gthr, sa, p.lcap, sel, sel_n, sb);
     // Compact work list for the rerank: exclusive prefix of the per-query
     // counts (the last slot holds the total).
     exclusive_scan_kernel<<<1, 256, 0, stream>>>(sel_n, sel_off, nq, sel_off + nq);
@@ -1851,6 +1926,7 @@ static bool make_args(const FvFlatTables* t, const FvFlatSearchParams* sp, FlatS
     a.flags = sp->flags;
     a.n_chunks = sp->n_chunks;
     a.bound_div = sp->bound_div;
+    a.top_k_sum = sp->d_top_k ? sp->top_k_sum : 0u;
     if (a.top_k == 0 || a.top_k > FV_MAX_TOPK) return false;
     if (sp->metric > METRIC_INNER_PRODUCT ||
         (sp->flags & ~(FV_SEARCH_F64_BOUNDARY | FV_SEARCH_FULL_RERANK)) != 0)

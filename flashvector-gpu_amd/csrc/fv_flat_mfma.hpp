@@ -25,6 +25,15 @@ struct FlatSearchArgs {
     // 0 = the launched chunk count is used as is.
     uint32_t chunk_want;
     uint32_t chunk_min_rows;
+    // Caller's sum of the per-query k (host hint for sizing per-query
+    // candidate buffers; 0 = unknown).
+    uint32_t top_k_sum;
+    // Per-query candidate buffers (workgroup-prune scans only): in chunk c,
+    // query q's buffer starts at key c * cand_stride + cand_off[q] and holds
+    // cand_off[q + 1] - cand_off[q] keys. NULL: every buffer holds `cap` keys
+    // at (c * nq + q) * cap.
+    const uint32_t* cand_off;
+    uint64_t cand_stride;
 };
 
 // Chunks of the main pass for `rows` rows (the planner's rule).
@@ -68,6 +77,10 @@ struct FlatMfmaPlan {
     uint32_t bound_chunks;    // first-pass chunks (0 = no first pass)
     uint32_t bound_rows;      // rows covered by the first pass
     uint32_t mask_blocks;
+    bool percap;              // per-query candidate buffers (FlatSearchArgs::cand_off)
+    uint32_t slack;           // their keys beyond the query's shortlist
+    uint64_t cand_keys;       // candidate keys per chunk
+    size_t off_candoff;
     size_t off_gthr, off_counts, off_cand, off_qh, off_qscale, off_rows, off_nrows, off_blocks,
         off_minima, off_sel, off_seln, off_seloff, off_err;
     size_t workspace_bytes;

@@ -257,7 +257,14 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     __shared__ uint32_t s_cnt[QB];
     __shared__ uint32_t s_keep[QB];
     __shared__ float s_thr[QB];
-    __shared__ uint32_t s_need[QB / 32];
+    // Queries to prune, one bit each; three sets used in turn by successive
+    // checks (workgroup-prune scans only: the wave-prune scans fill the
+    // 160 KB of LDS with two workgroups, to the byte at 128-dim FP16 rows).
+    __shared__ uint32_t s_need[3][R == 0 ? QB / 32 : 1];
+    // Per-query candidate buffers (workgroup-prune scans): start and keys
+    // of each query's buffer within the chunk.
+    __shared__ uint32_t s_cbase[R == 0 ? QB : 1];
+    __shared__ uint32_t s_ccap[R == 0 ? QB : 1];
     extern __shared__ uint64_t s_scratch[];
     auto tile_ptr = [&](auto bc) -> uint8_t* {
         if constexpr (decltype(bc)::value == 0) return s_tile0;
@@ -342,16 +349,46 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     const uint32_t my_ql = (uint32_t)(wave * QW + lane);
     const uint32_t my_q = qb0 + my_ql;
     const bool my_live = owner && my_q < nq;
+    // Candidate buffers: per-query ones (cand_off, workgroup prunes only)
+    // or `cap` keys per query.
+    const bool percap = kBlockPrune && a.cand_off != nullptr;
+    uint64_t* const cand_c = cand + (size_t)chunk * (percap ? a.cand_stride : (uint64_t)nq * cap);
     if (owner) {
         uint32_t k, S;
         if (my_live) query_k_s(a, my_q, k, S);
         else S = 1;
         s_keep[my_ql] = S;
         s_cnt[my_ql] = 0;
-        s_thr[my_ql] = my_live
+        float th = my_live
             ? bound_from_ord(__hip_atomic_load(&gthr[my_q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             : -__builtin_inff();
+        if constexpr (kBlockPrune) {
+            if (percap) {
+                const uint32_t b0 = my_live ? a.cand_off[my_q] : 0u;
+                const uint32_t nk = my_live ? a.cand_off[my_q + 1] - b0 : 0u;
+                s_cbase[my_ql] = b0;
+                s_ccap[my_ql] = nk;
+                // A buffer cut short by the planned total (understated k
+                // sum) keeps no candidates: the query returns no rows.
+                const uint32_t smax_q = a.s_max < cap ? a.s_max : cap;
+                if (nk < S + (cap - smax_q)) th = -__builtin_inff();
+            }
+        }
+        s_thr[my_ql] = th;
     }
+    if constexpr (kBlockPrune)
+        if (tid < QB / 32) s_need[0][tid] = s_need[1][tid] = s_need[2][tid] = 0u;
+    // Buffer of local query ql and its capacity in keys.
+    auto qbuf = [&](uint32_t ql) -> uint64_t* {
+        if constexpr (kBlockPrune)
+            if (percap) return cand_c + s_cbase[ql];
+        return cand_c + (size_t)(qb0 + ql) * cap;
+    };
+    auto qcap = [&](uint32_t ql) -> uint32_t {
+        if constexpr (kBlockPrune)
+            if (percap) return s_ccap[ql];
+        return cap;
+    };
 
     const uint32_t ntiles = (r_end - r_begin + RT - 1) / RT;
     // Tiles between buffer checks: after a prune a buffer holds at most
@@ -489,7 +526,7 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
             const uint32_t rslot = meta >> 16;             // row-id slot of the record's tile
             const float th = s_thr[ql];
             const floatx4 sc = rec_score[e];
-            uint64_t* bq = cand + ((size_t)chunk * nq + qb0 + ql) * cap;
+            uint64_t* bq = qbuf(ql);
             // One LDS atomic per record (its passing scores together), then
             // the key stores: no store is followed by an LDS atomic of the
             // same record, so none has to complete before the next atomic.
@@ -530,6 +567,9 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
 
     // One tile: compute on buffer `buf` while the next tile is staged into
     // the other one.
+    // Tile of the next buffer check, and the set of prune bits it uses.
+    uint32_t next_check = pk - 1u;
+    uint32_t ck = 1u;
     auto step = [&](uint32_t t, auto bc) {
         constexpr int buf = decltype(bc)::value;
         using Next = std::integral_constant<int, buf ^ 1>;
@@ -739,6 +779,19 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
         // A pair of tiles shares one flush (the row ids of both stay in
         // their slots until two tiles later).
         if (!BOUND && (buf == 1 || !flush_pairs)) flush_records();
+        // Workgroup prunes: at a check, each owner marks its query before the
+        // tile barrier (its count is final once its wave has flushed), so the
+        // check needs no barrier of its own. Check i uses set i % 3 and
+        // clears set (i + 1) % 3, last read by check i - 2, before the
+        // barrier that preceded check i - 1.
+        const bool check = t == next_check;
+        if constexpr (kBlockPrune && !BOUND) {
+            if (check) {
+                if (tid < QB / 32) s_need[ck == 2u ? 0u : ck + 1u][tid] = 0u;
+                if (my_live && s_cnt[my_ql] + pk * RT > qcap(my_ql))
+                    atomicOr(&s_need[ck][my_ql >> 5], 1u << (my_ql & 31));
+            }
+        }
         if (t + 1 < ntiles) store_tile(Next{}, (t + 1) & 3u);
         if constexpr (DMA) __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
         __syncthreads();
@@ -746,11 +799,12 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
 
         // A buffer gains at most one key per row, so buffers are checked
         // every pk tiles and pruned when pk more tiles could overflow them.
-        if ((t + 1) % pk != 0) {
+        if (!check) {
             if (my_live) s_thr[my_ql] = fminf(s_thr[my_ql], bound_from_ord(gpre));
             wave_lds_sync();
             return;
         }
+        next_check += pk;
         if (!kBlockPrune) {
             // Wave-private prunes of buffers that could overflow on the next tile.
             const uint32_t c = my_live ? s_cnt[my_ql] : 0u;
@@ -761,7 +815,7 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
                 const uint32_t qlb = wave * QW + bl;
                 const uint32_t qgb = qb0 + qlb;
                 const uint32_t keep = s_keep[qlb];
-                uint64_t* bq = cand + ((size_t)chunk * nq + qgb) * cap;
+                uint64_t* bq = qbuf(qlb);
                 uint32_t th;
                 uint32_t kept = wave_bisect_prune<(R > 0 ? R : 1)>(bq, s_cnt[qlb], keep, lane, &th);
                 if (kept + pk * RT > cap) {
@@ -782,12 +836,9 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
             // Workgroup prunes, one query at a time: radix selection of the
             // `keep` smallest keys in place in the query's buffer (global
             // memory, L2-resident), so no buffer-sized LDS scratch is needed.
-            if (tid < QB / 32) s_need[tid] = 0;
-            __syncthreads();
-            if (my_live && s_cnt[my_ql] + pk * RT > cap) atomicOr(&s_need[my_ql >> 5], 1u << (my_ql & 31));
-            __syncthreads();
+            // The queries were marked in set ck before this tile's barrier.
             for (int w = 0; w < QB / 32; ++w) {
-                uint32_t bits = s_need[w];
+                uint32_t bits = s_need[ck][w];
                 while (bits) {
                     const int bl = __builtin_ctz(bits);
                     bits &= bits - 1;
@@ -795,8 +846,8 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
                     const uint32_t qg = qb0 + ql;
                     const uint32_t c = s_cnt[ql];
                     const uint32_t keep = s_keep[ql];
-                    uint64_t* bq = cand + ((size_t)chunk * nq + qg) * cap;
-                    // The trigger implies c > cap - pk * RT >= s_max + 64 > keep;
+                    uint64_t* bq = qbuf(ql);
+                    // The trigger implies c > cap_q - pk * RT >= S_q + 64 > keep;
                     // the keys of one buffer are distinct (one per row).
                     if (c > keep) {
                         const uint64_t kth = block_select(bq, c, keep, sel);
@@ -810,6 +861,7 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
                     __syncthreads();
                 }
             }
+            ck = ck == 2u ? 0u : ck + 1u;
         }
         // Tighten thresholds with the prefetched shared bound. Each wave only
         // reads its own queries' thresholds, so a wave-level sync suffices.

@@ -83,6 +83,7 @@ class FvFlatSearchParams(ctypes.Structure):
         ("flags", ctypes.c_uint32),
         ("n_chunks", ctypes.c_uint32),
         ("bound_div", ctypes.c_uint32),
+        ("top_k_sum", ctypes.c_uint32),
     ]
 
 

@@ -141,10 +141,11 @@ class FlatIndex:
 
     def params(self, top_k, metric="l2", valid_mask=None, valid_count=0, shortlist=0,
                shortlist_factor=0, top_k_per_query=None, f64_boundary=False, tuning=None,
-               full_rerank=False):
+               full_rerank=False, top_k_sum=0):
         """FvFlatSearchParams; `tuning` (an FvFlatTuning, e.g. from autotune)
         supplies the shortlist factor and plan overrides when the caller
-        gives no shortlist of its own."""
+        gives no shortlist of its own. `top_k_sum` (an upper bound on the
+        sum of top_k_per_query) sizes per-query candidate buffers."""
         n_chunks = bound_div = 0
         if tuning is not None:
             n_chunks, bound_div = int(tuning.n_chunks), int(tuning.bound_div)
@@ -155,7 +156,7 @@ class FlatIndex:
                                     int(shortlist),
                                     (FV_SEARCH_F64_BOUNDARY if f64_boundary else 0)
                                     | (FV_SEARCH_FULL_RERANK if full_rerank else 0),
-                                    n_chunks, bound_div)
+                                    n_chunks, bound_div, int(top_k_sum))
 
     def autotune(self, queries, top_k, metric="l2", valid_mask=None, valid_count=0,
                  target_recall=0.999, trials=5, f64_boundary=False, shortlist_factor=0):
@@ -176,16 +177,19 @@ class FlatIndex:
 
     def search(self, queries, top_k, metric="l2", valid_mask=None, shortlist=0, out=None,
                valid_count=0, shortlist_factor=0, top_k_per_query=None, f64_boundary=False,
-               tuning=None, full_rerank=False):
+               tuning=None, full_rerank=False, top_k_sum=0):
         """fv_flat_search. `valid_count` (set bits of the mask, if known) lets
         sparse masks be compacted; `top_k_per_query` is an optional int32
-        device tensor of per-query k (each <= top_k); `f64_boundary` sets
-        FV_SEARCH_F64_BOUNDARY, `full_rerank` FV_SEARCH_FULL_RERANK."""
+        device tensor of per-query k (each <= top_k), `top_k_sum` an upper
+        bound on its sum when the caller knows one (per-query candidate
+        buffers; queries past an understated sum return no rows);
+        `f64_boundary` sets FV_SEARCH_F64_BOUNDARY, `full_rerank`
+        FV_SEARCH_FULL_RERANK."""
         torch = L._torch()
         lib = L.load()
         nq = queries.shape[0]
         prm = self.params(top_k, metric, valid_mask, valid_count, shortlist, shortlist_factor,
-                          top_k_per_query, f64_boundary, tuning, full_rerank)
+                          top_k_per_query, f64_boundary, tuning, full_rerank, top_k_sum)
         need = self.workspace_size(nq, prm)
         if need == 0:
             raise L.FvError("invalid flat search shape")

@@ -128,6 +128,13 @@ typedef struct {
      * 0xFFFFFFFF = no first pass). */
     uint32_t n_chunks;
     uint32_t bound_div;
+    /* With d_top_k: an upper bound on the sum of the per-query k (0 = not
+     * given). Deep shortlists then get per-query candidate buffers sized from
+     * each query's shortlist, laid out on the device, instead of buffers of
+     * the deepest shortlist for every query (far less workspace when k
+     * varies). A query whose buffer does not fit because the sum was
+     * understated returns no rows (every slot empty). */
+    uint32_t top_k_sum;
 } FvFlatSearchParams;
 
 size_t fv_flat_workspace_size(const FvFlatTables* tables, uint32_t num_queries,

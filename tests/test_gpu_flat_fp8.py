@@ -205,6 +205,55 @@ def test_fp8_deep_mixed_k_exact(torch, f64_boundary):
         assert (ids[i, k:] == 0xFFFFFFFF).all()
 
 
+@pytest.mark.parametrize("dtype,metric,f64", [("fp8", "ip", True), ("fp8", "ip", False),
+                                               ("bf16", "l2", False)])
+def test_per_query_buffers_equal_uniform(torch, dtype, metric, f64):
+    """Deep shortlists with per-query k: with the k sum given, candidate
+    buffers are sized per query (far smaller workspace) and the results are
+    those of the deepest-shortlist buffers, bit for bit; on integer data
+    both equal the oracle."""
+    db = int14(60000, 256, 41)
+    qs = int14(300, 256, 42)
+    ks = S.log_uniform_k(300, 1024, 43)
+    idx = F.FlatIndex(dev(torch, db), dtype)
+    d_q, d_k = dev(torch, qs), dev(torch, ks)
+    kw = dict(shortlist_factor=4, top_k_per_query=d_k, f64_boundary=f64)
+    ws_u = idx.workspace_size(300, idx.params(1024, metric, **kw))
+    ws_p = idx.workspace_size(300, idx.params(1024, metric, top_k_sum=int(ks.sum()), **kw))
+    assert 0 < ws_p < ws_u / 2, (ws_p, ws_u)
+    uni = L.split_results(idx.search(d_q, 1024, metric, **kw))
+    per = L.split_results(idx.search(d_q, 1024, metric, top_k_sum=int(ks.sum()), **kw))
+    torch.cuda.synchronize()
+    assert_same(per, uni)
+    e_ids, e_d = O.flat_topk(db, qs, 1024, metric)
+    for i in range(300):
+        k = ks[i]
+        np.testing.assert_array_equal(per[0][i, :k], e_ids[i, :k])
+        np.testing.assert_array_equal(per[1][i, :k].view(np.uint32), e_d[i, :k].view(np.uint32))
+
+
+def test_per_query_buffers_understated_sum(torch):
+    """A k sum below the true one: the buffers laid out within the planned
+    total are whole, the queries past it return no rows (every slot empty),
+    and nothing is written out of bounds (the queries that fit equal the
+    uniform-buffer results)."""
+    db = int14(40000, 256, 51)
+    qs = int14(200, 256, 52)
+    ks = S.log_uniform_k(200, 1024, 53)
+    idx = F.FlatIndex(dev(torch, db), "fp8")
+    d_q, d_k = dev(torch, qs), dev(torch, ks)
+    kw = dict(shortlist_factor=4, top_k_per_query=d_k)
+    uni = L.split_results(idx.search(d_q, 1024, "ip", **kw))
+    per = L.split_results(idx.search(d_q, 1024, "ip", top_k_sum=max(1, int(ks.sum()) // 8), **kw))
+    torch.cuda.synchronize()
+    empty = (per[0] == 0xFFFFFFFF).all(axis=1)
+    assert empty.any() and not empty.all()
+    first = int(np.argmax(empty))
+    assert empty[first:].all(), "queries past the planned total form a suffix"
+    np.testing.assert_array_equal(per[0][:first], uni[0][:first])
+    np.testing.assert_array_equal(per[1][:first].view(np.uint32), uni[1][:first].view(np.uint32))
+
+
 def test_graph_replay_matches_eager(torch):
     """A whole mixed-k FP8 batch captured as one HIP graph: every replay,
     with other eager searches and synchronizations in between, reproduces
