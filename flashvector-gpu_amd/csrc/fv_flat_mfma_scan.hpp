@@ -260,7 +260,7 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
     // Queries to prune, one bit each; three sets used in turn by successive
     // checks (workgroup-prune scans only: the wave-prune scans fill the
     // 160 KB of LDS with two workgroups, to the byte at 128-dim FP16 rows).
-    __shared__ uint32_t s_need[3][R == 0 ? QB / 32 : 1];
+    __shared__ __attribute__((aligned(16))) uint32_t s_need[3][R == 0 ? QB / 32 : 1];
     // Per-query candidate buffers (workgroup-prune scans): start and keys
     // of each query's buffer within the chunk.
     __shared__ uint32_t s_cbase[R == 0 ? QB : 1];
@@ -837,7 +837,13 @@ __global__ __launch_bounds__(kMfmaThreads, (ScanShape<DT, KS>::kOcc)) void flat_
             // `keep` smallest keys in place in the query's buffer (global
             // memory, L2-resident), so no buffer-sized LDS scratch is needed.
             // The queries were marked in set ck before this tile's barrier.
-            for (int w = 0; w < QB / 32; ++w) {
+            // Most checks find none: the words are read together (one LDS
+            // round trip, not one per word) and the loop is skipped.
+            uint32_t any = 0;
+#pragma unroll
+            for (int w = 0; w < QB / 32; ++w) any |= s_need[ck][w];
+            const int w_end = __builtin_amdgcn_readfirstlane(any) != 0u ? QB / 32 : 0;
+            for (int w = 0; w < w_end; ++w) {
                 uint32_t bits = s_need[ck][w];
                 while (bits) {
                     const int bl = __builtin_ctz(bits);
