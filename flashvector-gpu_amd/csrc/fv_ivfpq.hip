@@ -372,8 +372,11 @@ __global__ __launch_bounds__(kIvfThreads, 4) void ivf_scan_kernel(IvfScanArgs a)
             float l[4];
 #pragma unroll
             for (int b = 0; b < 4; ++b) l[b] = s_lut16[(seg * 4 + b) * 256u + ((w >> (8 * b)) & 255u)];
+            // The first segment starts from its first term: 0 + x == x for
+            // every LUT entry (never -0: fmaf chains of squares from +0).
+            s = seg == 0 ? l[0] : s + l[0];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) s = s + l[b];
+            for (int b = 1; b < 4; ++b) s = s + l[b];
             // Issue the four lookups together (addresses, LDS reads, adds),
             // not one round trip at a time.
             __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);
