@@ -232,6 +232,25 @@ def test_per_query_buffers_equal_uniform(torch, dtype, metric, f64):
         np.testing.assert_array_equal(per[1][i, :k].view(np.uint32), e_d[i, :k].view(np.uint32))
 
 
+def test_per_query_buffers_fixed_shortlist(torch):
+    """A fixed shortlist (2048 rows, at least k_i) with per-query k: the
+    per-query buffers take the shortlist rule's bound on the k sum and
+    return the uniform buffers' results."""
+    db = int14(50000, 256, 61)
+    qs = int14(128, 256, 62)
+    ks = S.log_uniform_k(128, 1024, 63)
+    idx = F.FlatIndex(dev(torch, db), "fp8")
+    d_q, d_k = dev(torch, qs), dev(torch, ks)
+    kw = dict(shortlist=2048, top_k_per_query=d_k)
+    ws_u = idx.workspace_size(128, idx.params(1024, "ip", **kw))
+    ws_p = idx.workspace_size(128, idx.params(1024, "ip", top_k_sum=int(ks.sum()), **kw))
+    assert 0 < ws_p < ws_u, (ws_p, ws_u)
+    uni = L.split_results(idx.search(d_q, 1024, "ip", **kw))
+    per = L.split_results(idx.search(d_q, 1024, "ip", top_k_sum=int(ks.sum()), **kw))
+    torch.cuda.synchronize()
+    assert_same(per, uni)
+
+
 def test_per_query_buffers_understated_sum(torch):
     """A k sum below the true one: the buffers laid out within the planned
     total are whole, the queries past it return no rows (every slot empty),
