@@ -477,12 +477,15 @@ __global__ __launch_bounds__(256) void flat_rerank_kernel(
 // flight; shallow shortlists run 4 wavefronts per workgroup (the kernel
 // strides by blockDim.x).
 constexpr int kFinalThreads = 1024;
+// Deepest shortlist finalized by a 256-thread workgroup when the finalize is
+// split by depth (a power of two: it is that launch's key capacity).
+constexpr uint32_t kFinalSplitS = 512;
 template <int METRIC>
 __global__ __launch_bounds__(kFinalThreads) void flat_finalize_kernel(
     const uint64_t* __restrict__ sel, const uint32_t* __restrict__ sel_n,
     const float* __restrict__ err, uint32_t hcap, FlatSearchArgs a,
     const float* __restrict__ X32, const float* __restrict__ Qm,
-    GpuSearchResult* __restrict__ out) {
+    GpuSearchResult* __restrict__ out, uint32_t s_lo, uint32_t s_hi) {
     extern __shared__ uint64_t s_k[];
     __shared__ uint32_t s_emax;
     __shared__ BlockSelectLds s_bs;
@@ -490,6 +493,9 @@ __global__ __launch_bounds__(kFinalThreads) void flat_finalize_kernel(
     const uint32_t q = blockIdx.x;
     uint32_t k, S;
     query_k_s(a, q, k, S);
+    // Queries whose shortlist depth is outside [s_lo, s_hi) belong to the
+    // other launch (the whole workgroup leaves before any barrier).
+    if (S < s_lo || S >= s_hi) return;
     const uint32_t have_in = sel_n[q];
     const uint64_t* src = sel + (size_t)q * a.s_max;
     for (uint32_t i = threadIdx.x; i < have_in; i += blockDim.x) s_k[i] = src[i];
@@ -1884,15 +1890,32 @@ hipError_t launch_flat_mfma(const FvFlatTables* t, const float* Qm, uint32_t nq,
     // counts (the last slot holds the total).
     exclusive_scan_kernel<<<1, 256, 0, stream>>>(sel_n, sel_off, nq, sel_off + nq);
     const dim3 rg((unsigned)(((size_t)nq * p.s_max + kRerankRows - 1) / kRerankRows));
+    // Finalize. With per-query k and deep shortlists, queries of at most
+    // kFinalSplitS rows go to a launch of 256-thread workgroups with LDS for
+    // that many keys (many per CU) and only the deeper ones to the
+    // 1024-thread launch; each launch skips the other's queries.
+    const bool split = a.d_top_k != nullptr && p.final_threads == (uint32_t)kFinalThreads;
+    const uint32_t hs = kFinalSplitS;
+    const size_t lds_small = (size_t)hs * 8 + ((a.flags & FV_SEARCH_F64_BOUNDARY) ? (size_t)hs * 12 : 0);
+    auto finalize = [&](auto kern) {
+        if (split) {
+            kern<<<nq, 256, lds_small, stream>>>(sel, sel_n, err, hs, a, X32, Qm, out, 0u, hs + 1u);
+            kern<<<nq, p.final_threads, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out,
+                                                               hs + 1u, 0xFFFFFFFFu);
+        } else {
+            kern<<<nq, p.final_threads, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out,
+                                                               0u, 0xFFFFFFFFu);
+        }
+    };
     if (a.metric == METRIC_L2) {
         flat_rerank_kernel<METRIC_L2><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, nq, p.s_max, sel_n, sel_off, sel_off + nq, sel, err);
-        flat_finalize_kernel<METRIC_L2><<<nq, p.final_threads, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
+        finalize(flat_finalize_kernel<METRIC_L2>);
     } else if (a.metric == METRIC_INNER_PRODUCT) {
         flat_rerank_kernel<METRIC_INNER_PRODUCT><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, nq, p.s_max, sel_n, sel_off, sel_off + nq, sel, err);
-        flat_finalize_kernel<METRIC_INNER_PRODUCT><<<nq, p.final_threads, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
+        finalize(flat_finalize_kernel<METRIC_INNER_PRODUCT>);
     } else {
         flat_rerank_kernel<METRIC_COSINE><<<rg, 256, 0, stream>>>(X32, Qm, t->dim, nq, p.s_max, sel_n, sel_off, sel_off + nq, sel, err);
-        flat_finalize_kernel<METRIC_COSINE><<<nq, p.final_threads, p.lds_final, stream>>>(sel, sel_n, err, p.hcap, a, X32, Qm, out);
+        finalize(flat_finalize_kernel<METRIC_COSINE>);
     }
     return hipGetLastError();
 }
